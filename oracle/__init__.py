@@ -1,0 +1,381 @@
+"""CPU oracle of the vexel-hip hot path -- test infrastructure only.
+
+A ctypes face over ``oracle/liboracle.so``, the C++ restatement of the reference's kernels
+(see ``oracle.h`` for the file:line each one follows and what pins it). Only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg import this module, and
+only as the checker: the product library never loads it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_c_i = ctypes.c_int
+_c_i64 = ctypes.c_int64
+_vp = ctypes.c_void_p
+
+
+def build() -> None:
+    """Compiles liboracle.so with the Makefile beside this file."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def _load() -> ctypes.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run `make -C {HERE}`")
+    lib = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "orc_rng_new": (_vp, [_c_i64]),
+        "orc_rng_free": (None, [_vp]),
+        "orc_rng_next_int": (ctypes.c_int32, [_vp]),
+        "orc_rng_next_int_bound": (ctypes.c_int32, [_vp, ctypes.c_int32]),
+        "orc_rng_next_float": (ctypes.c_float, [_vp]),
+        "orc_rng_next_double": (ctypes.c_double, [_vp]),
+        "orc_rng_next_gaussian": (ctypes.c_double, [_vp]),
+        "orc_rng_pattern": (None, [_vp, ctypes.c_char_p, _c_i64, _f32p]),
+        "orc_rng_shuffle_order": (None, [_vp, ctypes.c_int32, _i32p]),
+        "orc_swe_step": (None, [_c_i, _c_i, _i32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i32p,
+                                _i32p, _i32p, _i32p, _i32p, _i32p, _i32p]),
+        "orc_swe_fastest_wave": (ctypes.c_int32, [_c_i64, _f32p, _f32p, _f32p, ctypes.c_double,
+                                                  ctypes.c_double]),
+        "orc_swe_diagnostics": (None, [_c_i64, _f32p, _f32p, _f32p, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_double, _f64p]),
+        "orc_swe_step_f64": (None, [_c_i, _c_i, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double]),
+        "orc_mpm3_clear": (None, [_c_i, _c_i, _c_i, _f32p, _f32p, _f32p, _f32p]),
+        "orc_mpm3_scatter": (None, [_c_i, _c_i, _c_i, _c_i64, _vp, _i32p, _f32p, _f32p, _f32p, _f32p]),
+        "orc_mpm3_grid": (None, [_c_i, _c_i, _c_i, _i32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]),
+        "orc_mpm3_advect": (None, [_c_i, _c_i, _c_i, _c_i64, _vp, _i32p, _f32p, _f32p, _f32p]),
+        "orc_mpm3_step": (None, [_c_i, _c_i, _c_i, _c_i64, _vp, _i32p, _vp]),
+        "orc_mpm3_key_bits": (_c_i, [_c_i, _c_i, _c_i]),
+        "orc_mpm3_keys": (None, [_c_i, _c_i, _c_i, _c_i64, _f32p, _f32p, _f32p, _i32p]),
+        "orc_mpm2_clear": (None, [_c_i, _c_i, _f32p, _f32p, _f32p]),
+        "orc_mpm2_scatter": (None, [_c_i, _c_i, _c_i64, _vp, _i32p, _f32p, _f32p, _f32p]),
+        "orc_mpm2_grid": (None, [_c_i, _c_i, _i32p, _f32p, _f32p, _f32p, _f32p, _f32p]),
+        "orc_mpm2_advect": (None, [_c_i, _c_i, _c_i64, _vp, _i32p, _f32p, _f32p]),
+        "orc_mpm2_step": (None, [_c_i, _c_i, _c_i64, _vp, _i32p, _vp]),
+        "orc_mpm2_cell": (ctypes.c_int32, [ctypes.c_float, ctypes.c_float, _c_i, _c_i]),
+        "orc_mpm2_keys": (None, [_c_i, _c_i, _c_i64, _f32p, _f32p, _i32p]),
+        "orc_mpm2_cell_starts": (_c_i64, [_c_i, _c_i, _c_i64, _f32p, _f32p, _i32p]),
+        "orc_mpm2_sort": (None, [_c_i, _c_i, _c_i64, _vp]),
+        "orc_stable_order": (None, [_c_i64, _i32p, _i32p]),
+        "orc_bucket_starts": (None, [_c_i64, _i32p, ctypes.c_int32, _i32p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+# Particle field orders: core/particle/Flip3.java:112-115 and core/particle/FlipStep.java:95-97.
+FIELDS3 = ("x", "y", "z", "u", "v", "w", "m", "j",
+           "c00", "c01", "c02", "c10", "c11", "c12", "c20", "c21", "c22")
+FIELDS2 = ("x", "y", "u", "v", "m", "j", "c00", "c01", "c10", "c11")
+GRID3 = ("gm", "gmu", "gmv", "gmw", "gu", "gv", "gw")
+GRID2 = ("gm", "gmu", "gmv", "gu", "gv")
+
+
+def f32_bits(values) -> np.ndarray:
+    """Float.floatToRawIntBits of each value, after a (float) cast."""
+    return np.asarray(values, dtype=np.float64).astype(np.float32).view(np.int32).copy()
+
+
+def bits_f32(words) -> np.ndarray:
+    return np.asarray(words, dtype=np.int32).view(np.float32).copy()
+
+
+# ------------------------------------------------------------------ java.util.Random
+
+class JavaRandom:
+    """java.util.Random, through the oracle's C port."""
+
+    def __init__(self, seed: int):
+        self._h = lib.orc_rng_new(int(seed))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib.orc_rng_free(self._h)
+            self._h = None
+
+    def next_int(self, bound: int | None = None) -> int:
+        if bound is None:
+            return lib.orc_rng_next_int(self._h)
+        return lib.orc_rng_next_int_bound(self._h, int(bound))
+
+    def next_float(self) -> float:
+        return lib.orc_rng_next_float(self._h)
+
+    def next_double(self) -> float:
+        return lib.orc_rng_next_double(self._h)
+
+    def next_gaussian(self) -> float:
+        return lib.orc_rng_next_gaussian(self._h)
+
+    def pattern(self, pattern: str, count: int) -> np.ndarray:
+        """``count`` rows of draws in the order ``pattern`` names them ('f', 'g', 'd')."""
+        out = np.empty(count * len(pattern), dtype=np.float32)
+        lib.orc_rng_pattern(self._h, pattern.encode(), count, out)
+        return out.reshape(count, len(pattern))
+
+    def shuffle_order(self, n: int) -> np.ndarray:
+        out = np.empty(n, dtype=np.int32)
+        lib.orc_rng_shuffle_order(self._h, n, out)
+        return out
+
+
+# ------------------------------------------------------------------ shallow water
+
+WALL, OPEN = 0, 1
+
+
+def swe_params(g, courant, dx, dy, dry=1e-6, allow_unstable=False) -> np.ndarray:
+    """ShallowWater.params (core/stencil/ShallowWater.java:248-271)."""
+    if not courant > 0:
+        raise ValueError(f"a Courant number must be positive, got {courant}")
+    if not allow_unstable and not courant <= 0.5:
+        raise ValueError(f"a first-order 2D step needs a Courant number in (0, 1/2], got {courant}")
+    return f32_bits([g, courant, dx, dy, dry])
+
+
+def swe_fastest_wave(h, hu, hv, g, dry) -> int:
+    return lib.orc_swe_fastest_wave(h.size, _c(h), _c(hu), _c(hv), float(g), float(dry))
+
+
+def swe_diagnostics(h, hu, hv, g, dry, dt, dx, dy) -> dict:
+    out = np.zeros(9)
+    lib.orc_swe_diagnostics(h.size, _c(h), _c(hu), _c(hv), g, dry, dt, dx, dy, out)
+    keys = ("volume", "min_depth", "max_depth", "max_speed", "max_froude", "max_courant",
+            "non_finite", "negative", "wet")
+    return dict(zip(keys, out.tolist()))
+
+
+def _c(a, dtype=np.float32):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class SweStepper:
+    """The reference test harness's CPU stepper (test/stencil/Stepper.java:26-221) over the
+    oracle: two states ping-pong, three speed words rotate, two budget words ping-pong."""
+
+    def __init__(self, nx: int, ny: int, edges=(WALL, WALL, WALL, WALL)):
+        if nx < 1 or ny < 1:
+            raise ValueError(f"a patch needs at least one cell each way, got {nx} x {ny}")
+        self.nx, self.ny = nx, ny
+        self.edges = np.asarray(edges, dtype=np.int32)
+        self.dispatched = 0
+        self.state = [[np.zeros(nx * ny, np.float32) for _ in range(3)] for _ in range(2)]
+        self.speed = [np.zeros(1, np.int32) for _ in range(3)]
+        self.budget = [np.zeros(1, np.int32) for _ in range(2)]
+        self._clamped = np.zeros(1, np.int32)
+        self.params = np.zeros(5, np.int32)
+
+    def load(self, h, hu, hv, params, first_speed):
+        self.state[0] = [_c(h).ravel().copy(), _c(hu).ravel().copy(), _c(hv).ravel().copy()]
+        self.state[1] = [np.zeros(self.nx * self.ny, np.float32) for _ in range(3)]
+        self.speed = [np.array([first_speed], np.int32), np.zeros(1, np.int32), np.zeros(1, np.int32)]
+        self.params = np.asarray(params, dtype=np.int32).copy()
+        self._clamped[0] = 0
+        self.dispatched = 0
+
+    def dispatch(self):
+        k = self.dispatched
+        out, inn = self.state[(k + 1) % 2], self.state[k % 2]
+        lib.orc_swe_step(self.nx, self.ny, self.edges, out[0], out[1], out[2], inn[0], inn[1], inn[2],
+                         self.params, self.speed[k % 3], self.speed[(k + 1) % 3], self.speed[(k + 2) % 3],
+                         self.budget[k % 2], self.budget[(k + 1) % 2], self._clamped)
+        self.dispatched += 1
+
+    def step(self, n: int):
+        for _ in range(n):
+            self.dispatch()
+
+    def write_budget(self, ticks: int):
+        self.budget[self.dispatched % 2][0] = ticks
+
+    def read_budget(self) -> int:
+        return int(self.budget[self.dispatched % 2][0])
+
+    def read(self):
+        return [a.copy() for a in self.state[self.dispatched % 2]]
+
+    def clamped(self) -> int:
+        return int(self._clamped[0])
+
+    def fastest_wave_bits(self) -> int:
+        return int(self.speed[self.dispatched % 3][0])
+
+
+def swe_step_f64(nx, ny, edges, h, hu, hv, g, dx, dy, dry, dt):
+    out = [np.empty(nx * ny) for _ in range(3)]
+    lib.orc_swe_step_f64(nx, ny, np.asarray(edges, np.int32), out[0], out[1], out[2],
+                         _c(h, np.float64), _c(hu, np.float64), _c(hv, np.float64), g, dx, dy, dry, dt)
+    return out
+
+
+# ------------------------------------------------------------------ MPM
+
+def mpm3_params(dt, gx, gy, gz, bulk, rho0, sigma=0.0) -> np.ndarray:
+    """Flip3.params (core/particle/Flip3.java:89-103)."""
+    if not (dt > 0) or not (bulk >= 0) or not (rho0 > 0):
+        raise ValueError(f"dt > 0, bulk >= 0 and rho0 > 0, got dt {dt}, bulk {bulk}, rho0 {rho0}")
+    return f32_bits([dt, gx, gy, gz, bulk, rho0, sigma])
+
+
+def mpm2_params(dt, gx, gy, bulk, rho0, sigma=0.0) -> np.ndarray:
+    """Flip.params (core/particle/Flip.java:145-211): 16 values, the slice bounds, the pump's ten."""
+    if not (dt > 0) or not (bulk >= 0) or not (rho0 > 0):
+        raise ValueError(f"dt > 0, bulk >= 0 and rho0 > 0, got dt {dt}, bulk {bulk}, rho0 {rho0}")
+    words = np.zeros(28, np.int32)
+    words[:6] = f32_bits([dt, gx, gy, bulk, rho0, sigma])
+    return words
+
+
+def _ptrs(arrays):
+    for a in arrays:
+        assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return (ctypes.c_void_p * len(arrays))(*[a.ctypes.data for a in arrays])
+
+
+class Mpm3:
+    """A 3D particle state on the CPU, stepped pass by pass as Flip3Step lists them."""
+
+    def __init__(self, nx, ny, nz, fields: dict, params):
+        self.n = (nx, ny, nz)
+        count = len(fields["x"])
+        self.p = {f: _c(fields.get(f, np.zeros(count)), np.float32).copy() for f in FIELDS3}
+        if "j" not in fields:
+            self.p["j"][:] = 1
+        self.count = count
+        nodes = nx * ny * nz
+        self.g = {f: np.zeros(nodes, np.float32) for f in GRID3}
+        self.params = np.asarray(params, np.int32)
+
+    def _pp(self):
+        return _ptrs([self.p[f] for f in FIELDS3])
+
+    def clear(self):
+        lib.orc_mpm3_clear(*self.n, self.g["gm"], self.g["gmu"], self.g["gmv"], self.g["gmw"])
+
+    def scatter(self):
+        lib.orc_mpm3_scatter(*self.n, self.count, self._pp(), self.params,
+                             self.g["gm"], self.g["gmu"], self.g["gmv"], self.g["gmw"])
+
+    def grid(self):
+        lib.orc_mpm3_grid(*self.n, self.params, *[self.g[f] for f in GRID3])
+
+    def advect(self):
+        lib.orc_mpm3_advect(*self.n, self.count, self._pp(), self.params,
+                            self.g["gu"], self.g["gv"], self.g["gw"])
+
+    def step(self, n=1):
+        grid = _ptrs([self.g[f] for f in GRID3])
+        for _ in range(n):
+            lib.orc_mpm3_step(*self.n, self.count, self._pp(), self.params, grid)
+
+    def keys(self):
+        out = np.empty(self.count, np.int32)
+        lib.orc_mpm3_keys(*self.n, self.count, self.p["x"], self.p["y"], self.p["z"], out)
+        return out
+
+    def sort(self):
+        order = stable_order(self.keys())
+        for f in FIELDS3:
+            self.p[f] = self.p[f][order].copy()
+        return order
+
+
+def mpm3_key_bits(nx, ny, nz) -> int:
+    return lib.orc_mpm3_key_bits(nx, ny, nz)
+
+
+class Mpm2:
+    """A 2D particle state on the CPU, stepped as FlipStep lists its passes."""
+
+    def __init__(self, nx, ny, fields: dict, params):
+        self.n = (nx, ny)
+        count = len(fields["x"])
+        self.p = {f: _c(fields.get(f, np.zeros(count)), np.float32).copy() for f in FIELDS2}
+        if "j" not in fields:
+            self.p["j"][:] = 1
+        self.count = count
+        self.g = {f: np.zeros(nx * ny, np.float32) for f in GRID2}
+        self.params = np.asarray(params, np.int32)
+        self.steps = 0
+
+    def _pp(self):
+        return _ptrs([self.p[f] for f in FIELDS2])
+
+    def clear(self):
+        lib.orc_mpm2_clear(*self.n, self.g["gm"], self.g["gmu"], self.g["gmv"])
+
+    def scatter(self):
+        lib.orc_mpm2_scatter(*self.n, self.count, self._pp(), self.params,
+                             self.g["gm"], self.g["gmu"], self.g["gmv"])
+
+    def grid(self):
+        lib.orc_mpm2_grid(*self.n, self.params, *[self.g[f] for f in GRID2])
+
+    def advect(self):
+        lib.orc_mpm2_advect(*self.n, self.count, self._pp(), self.params, self.g["gu"], self.g["gv"])
+
+    def step(self, n=1):
+        grid = _ptrs([self.g[f] for f in GRID2])
+        for _ in range(n):
+            lib.orc_mpm2_step(*self.n, self.count, self._pp(), self.params, grid)
+
+    def sort(self):
+        lib.orc_mpm2_sort(*self.n, self.count, self._pp())
+
+    def advance(self, count, sort_every):
+        """ParticleSimulation.advance (gui/ParticleSimulation.java:175-183)."""
+        for _ in range(count):
+            if self.steps % sort_every == 0:
+                self.sort()
+            self.step()
+            self.steps += 1
+
+
+def mpm2_cell(x, y, nx, ny) -> int:
+    return lib.orc_mpm2_cell(float(x), float(y), nx, ny)
+
+
+def mpm2_keys(x, y, nx, ny) -> np.ndarray:
+    out = np.empty(len(x), np.int32)
+    lib.orc_mpm2_keys(nx, ny, len(x), _c(x), _c(y), out)
+    return out
+
+
+def mpm2_cell_starts(x, y, nx, ny) -> np.ndarray:
+    """Scatter.cellStarts; raises ValueError on particles out of cell order, as the reference does."""
+    starts = np.empty((nx - 1) * (ny - 1) + 1, np.int32)
+    r = lib.orc_mpm2_cell_starts(nx, ny, len(x), _c(x), _c(y), starts)
+    if r != 0:
+        raise ValueError(f"particle {-1 - r} is out of order; the particles must be sorted by cell")
+    return starts
+
+
+def stable_order(keys) -> np.ndarray:
+    keys = _c(keys, np.int32)
+    out = np.empty(len(keys), np.int32)
+    lib.orc_stable_order(len(keys), keys, out)
+    return out
+
+
+def bucket_starts(keys, cells) -> np.ndarray:
+    keys = _c(keys, np.int32)
+    out = np.empty(cells + 1, np.int32)
+    lib.orc_bucket_starts(len(keys), keys, cells, out)
+    return out

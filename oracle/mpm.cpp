@@ -1,0 +1,371 @@
+// MLS-MPM / APIC passes restated from core/particle/Flip3.java (3D) and core/particle/Flip.java
+// (2D), binary32, no contraction, one particle (or node) at a time in index order. The
+// float atomics of the reference become plain sums in particle order here: any order is
+// the reference's (Scatter.java:48-53), and this one is fixed. Test infrastructure only.
+#include "oracle.h"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+namespace {
+
+inline float mn(float a, float b) { return b < a ? b : a; }
+inline float mx(float a, float b) { return a < b ? b : a; }
+inline float clampf(float x, float lo, float hi) { return mn(mx(x, lo), hi); }
+inline float bits_to_float(int32_t w) {
+    float f;
+    std::memcpy(&f, &w, 4);
+    return f;
+}
+
+const float kWall = 1.0f;     // Flip.WALL, core/particle/Flip.java:128
+const float kEmpty = 1e-9f;   // Flip.EMPTY / Flip3.EMPTY
+const float kJBound = 10.0f;  // Flip3.J_BOUND
+
+// Flip3.Stencil3.weights / Flip.Stencil.weights: 1/2(3/2-f)^2, 3/4-(f-1)^2, 1/2(f-1/2)^2
+inline void weights(float f, float w[3]) {
+    float a = 1.5f - f;
+    float c = f - 1.0f;
+    float e = f - 0.5f;
+    w[0] = 0.5f * (a * a);
+    w[1] = 0.75f - (c * c);
+    w[2] = 0.5f * (e * e);
+}
+
+// One axis of Flip3.Stencil3.of (Flip3.java:670-676) / Flip.Stencil.of (Flip.java:573-579).
+inline void axis(float pos, int n, int32_t* cell, float* off, float w[3]) {
+    float x = clampf(pos, 0.5f, (float)n - 1.5f);
+    *cell = (int32_t)mn(x - 0.5f, (float)(n - 3));
+    *off = x - (float)*cell;
+    weights(*off, w);
+}
+
+// keepInside, Flip3.java:644-655 and Flip.java:548-559
+inline void keep_inside(float* at, float* vel, int size) {
+    float lo = kWall;
+    float hi = (float)size - 1.0f - kWall;
+    if (*at < lo) {
+        *at = lo;
+        *vel = mx(*vel, 0.0f);
+    }
+    if (*at > hi) {
+        *at = hi;
+        *vel = mn(*vel, 0.0f);
+    }
+}
+
+enum { DT3 = 0, GX3 = 1, GY3 = 2, GZ3 = 3, BULK3 = 4, RHO03 = 5 };     // Flip3.java:58-64
+enum { DT2 = 0, GX2 = 1, GY2 = 2, BULK2 = 3, RHO02 = 4 };              // Flip.java:102-106
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------- 3D
+
+void orc_mpm3_clear(int nx, int ny, int nz, float* gm, float* gmu, float* gmv, float* gmw) {
+    size_t n = (size_t)nx * ny * nz;
+    std::fill(gm, gm + n, 0.0f);
+    std::fill(gmu, gmu + n, 0.0f);
+    std::fill(gmv, gmv + n, 0.0f);
+    std::fill(gmw, gmw + n, 0.0f);
+}
+
+// Flip3.scatter, Flip3.java:193-246 (the plain branch)
+void orc_mpm3_scatter(int nx, int ny, int nz, int64_t count, float* const p[17], const int32_t params[7],
+                      float* gm, float* gmu, float* gmv, float* gmw) {
+    float* grid[4] = {gm, gmu, gmv, gmw};
+    const float dt = bits_to_float(params[DT3]);
+    const float bulk = bits_to_float(params[BULK3]);
+    const float rho0 = bits_to_float(params[RHO03]);
+    const int size[3] = {nx, ny, nz};
+    for (int64_t q = 0; q < count; q++) {
+        const float m = p[6][q];
+        if (!(m > 0.0f)) continue;
+        int32_t cell[3];
+        float off[3], w[3][3];
+        for (int a = 0; a < 3; a++) axis(p[a][q], size[a], &cell[a], &off[a], w[a]);
+        const int64_t base = ((int64_t)cell[2] * ny + cell[1]) * nx + cell[0];
+        const float vel[3] = {p[3][q], p[4][q], p[5][q]};
+        float c[9];
+        for (int a = 0; a < 9; a++) c[a] = p[8 + a][q];
+        const float s = (4.0f * (dt * (m / rho0))) * (bulk * (1.0f - p[7][q]));
+        for (int k = 0; k < 27; k++) {
+            const float wgt = w[0][k % 3] * (w[1][(k / 3) % 3] * w[2][k / 9]);
+            const float d[3] = {(float)(k % 3) - off[0], (float)((k / 3) % 3) - off[1], (float)(k / 9) - off[2]};
+            const float wm = wgt * m;
+            const int64_t node = base + (int64_t)(k / 9) * nx * ny + ((k / 3) % 3) * nx + k % 3;
+            grid[0][node] += wm;
+            for (int a = 0; a < 3; a++) {
+                const float at3 = vel[a] + (c[3 * a] * d[0] + (c[3 * a + 1] * d[1] + c[3 * a + 2] * d[2]));
+                const float push = s * d[a];
+                grid[1 + a][node] += wm * at3 + wgt * push;
+            }
+        }
+    }
+}
+
+// Flip3.grid, Flip3.java:267-311 (the plain branch)
+void orc_mpm3_grid(int nx, int ny, int nz, const int32_t params[7], const float* gm, const float* gmu,
+                   const float* gmv, const float* gmw, float* gu, float* gv, float* gw) {
+    const float* mom[3] = {gmu, gmv, gmw};
+    float* out[3] = {gu, gv, gw};
+    const int size[3] = {nx, ny, nz};
+    const float dt = bits_to_float(params[DT3]);
+    const float g[3] = {bits_to_float(params[GX3]), bits_to_float(params[GY3]), bits_to_float(params[GZ3])};
+    const int64_t nodes = (int64_t)nx * ny * nz;
+    for (int64_t node = 0; node < nodes; node++) {
+        const int64_t at[3] = {node % nx, (node / nx) % ny, node / ((int64_t)nx * ny)};
+        const float mass = gm[node];
+        float vel[3] = {0.0f, 0.0f, 0.0f};
+        if (mass > kEmpty) {
+            for (int a = 0; a < 3; a++) vel[a] = mom[a][node] / mass + dt * g[a];
+        }
+        for (int a = 0; a < 3; a++) {
+            if (!(at[a] > 1)) vel[a] = mx(vel[a], 0.0f);
+            if (at[a] > size[a] - 3) vel[a] = mn(vel[a], 0.0f);
+            out[a][node] = vel[a];
+        }
+    }
+}
+
+// Flip3.advect, Flip3.java:329-383 (the plain branch)
+void orc_mpm3_advect(int nx, int ny, int nz, int64_t count, float* const p[17], const int32_t params[7],
+                     const float* gu, const float* gv, const float* gw) {
+    const float* grid[3] = {gu, gv, gw};
+    const int size[3] = {nx, ny, nz};
+    const float dt = bits_to_float(params[DT3]);
+    for (int64_t q = 0; q < count; q++) {
+        if (!(p[6][q] > 0.0f)) continue;
+        int32_t cell[3];
+        float off[3], w[3][3];
+        for (int a = 0; a < 3; a++) axis(p[a][q], size[a], &cell[a], &off[a], w[a]);
+        const int64_t base = ((int64_t)cell[2] * ny + cell[1]) * nx + cell[0];
+        float vel[3] = {0.0f, 0.0f, 0.0f};
+        float sum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < 27; k++) {
+            const float wgt = w[0][k % 3] * (w[1][(k / 3) % 3] * w[2][k / 9]);
+            const float d[3] = {(float)(k % 3) - off[0], (float)((k / 3) % 3) - off[1], (float)(k / 9) - off[2]};
+            const int64_t node = base + (int64_t)(k / 9) * nx * ny + ((k / 3) % 3) * nx + k % 3;
+            for (int a = 0; a < 3; a++) {
+                const float na = wgt * grid[a][node];
+                vel[a] = vel[a] + na;
+                for (int e = 0; e < 3; e++) sum[3 * a + e] = sum[3 * a + e] + na * d[e];
+            }
+        }
+        float c[9];
+        for (int a = 0; a < 9; a++) c[a] = 4.0f * sum[a];
+        for (int a = 0; a < 3; a++) {
+            float pos = p[a][q] + dt * vel[a];
+            keep_inside(&pos, &vel[a], size[a]);
+            p[a][q] = pos;
+            p[3 + a][q] = vel[a];
+        }
+        for (int a = 0; a < 9; a++) p[8 + a][q] = c[a];
+        const float j = p[7][q] * (1.0f + dt * (c[0] + (c[4] + c[8])));
+        p[7][q] = clampf(j, 1.0f / kJBound, kJBound);
+    }
+}
+
+void orc_mpm3_step(int nx, int ny, int nz, int64_t count, float* const p[17], const int32_t params[7],
+                   float* const grid[7]) {
+    orc_mpm3_clear(nx, ny, nz, grid[0], grid[1], grid[2], grid[3]);
+    orc_mpm3_scatter(nx, ny, nz, count, p, params, grid[0], grid[1], grid[2], grid[3]);
+    orc_mpm3_grid(nx, ny, nz, params, grid[0], grid[1], grid[2], grid[3], grid[4], grid[5], grid[6]);
+    orc_mpm3_advect(nx, ny, nz, count, p, params, grid[4], grid[5], grid[6]);
+}
+
+// The 3D sort key: Stencil3's base cell, numbered brick-major (8x8x8-cell bricks, bricks
+// row-major, cells row-major within a brick). A bijection of the base cell, so a stable
+// sort by it groups each cell's particles and keeps a brick's particles together.
+static inline int brick_count(int n) { return (n - 2 + 7) / 8; }
+
+int orc_mpm3_key_bits(int nx, int ny, int nz) {
+    int64_t keys = (int64_t)brick_count(nx) * brick_count(ny) * brick_count(nz) * 512;
+    int bits = 0;
+    while ((1LL << bits) < keys) bits++;
+    return bits;
+}
+
+void orc_mpm3_keys(int nx, int ny, int nz, int64_t count, const float* x, const float* y, const float* z,
+                   int32_t* keys) {
+    const int bx = brick_count(nx), by = brick_count(ny);
+    const float* at[3] = {x, y, z};
+    const int size[3] = {nx, ny, nz};
+    for (int64_t q = 0; q < count; q++) {
+        int32_t cell[3];
+        for (int a = 0; a < 3; a++) {
+            float px = clampf(at[a][q], 0.5f, (float)size[a] - 1.5f);
+            cell[a] = (int32_t)mn(px - 0.5f, (float)(size[a] - 3));
+        }
+        int32_t brick = ((cell[2] >> 3) * by + (cell[1] >> 3)) * bx + (cell[0] >> 3);
+        int32_t local = ((cell[2] & 7) * 8 + (cell[1] & 7)) * 8 + (cell[0] & 7);
+        keys[q] = brick * 512 + local;
+    }
+}
+
+// ---------------------------------------------------------------- 2D
+
+void orc_mpm2_clear(int nx, int ny, float* gm, float* gmu, float* gmv) {
+    size_t n = (size_t)nx * ny;
+    std::fill(gm, gm + n, 0.0f);
+    std::fill(gmu, gmu + n, 0.0f);
+    std::fill(gmv, gmv + n, 0.0f);
+}
+
+// Flip.scatter, Flip.java:269-303, with Scatter.segmentedDeposit's sums (Scatter.java:385-439)
+// taken in particle order: a segmented sum is the same sum regrouped.
+void orc_mpm2_scatter(int nx, int ny, int64_t count, float* const p[10], const int32_t* params, float* gm,
+                      float* gmu, float* gmv) {
+    const float dt = bits_to_float(params[DT2]);
+    const float bulk = bits_to_float(params[BULK2]);
+    const float rho0 = bits_to_float(params[RHO02]);
+    for (int64_t q = 0; q < count; q++) {
+        int32_t col, row;
+        float fx, fy, wx[3], wy[3];
+        axis(p[0][q], nx, &col, &fx, wx);
+        axis(p[1][q], ny, &row, &fy, wy);
+        const int64_t base = (int64_t)row * nx + col;
+        const float m = p[4][q], u = p[2][q], w = p[3][q];
+        const float c00 = p[6][q], c01 = p[7][q], c10 = p[8][q], c11 = p[9][q];
+        const float s = (4.0f * (dt * (m / rho0))) * (bulk * (1.0f - p[5][q]));
+        for (int k = 0; k < 9; k++) {
+            const float wgt = wx[k % 3] * wy[k / 3];
+            const float wm = wgt * m;
+            const float dx = (float)(k % 3) - fx;
+            const float dy = (float)(k / 3) - fy;
+            const float vu = u + (c00 * dx + c01 * dy);
+            const float vv = w + (c10 * dx + c11 * dy);
+            const int64_t node = base + (int64_t)(k / 3) * nx + k % 3;
+            gm[node] += wm;
+            gmu[node] += wm * vu + wgt * (s * dx);
+            gmv[node] += wm * vv + wgt * (s * dy);
+        }
+    }
+}
+
+// Flip.grid, Flip.java:366-412 (the plain branch: weight 1)
+void orc_mpm2_grid(int nx, int ny, const int32_t* params, const float* gm, const float* gmu, const float* gmv,
+                   float* gu, float* gv) {
+    const float dt = bits_to_float(params[DT2]);
+    const float gx = bits_to_float(params[GX2]), gy = bits_to_float(params[GY2]);
+    const int64_t nodes = (int64_t)nx * ny;
+    for (int64_t node = 0; node < nodes; node++) {
+        const int64_t ni = node % nx, nj = node / nx;
+        const float m = gm[node];
+        float u = 0.0f, w = 0.0f;
+        if (m > kEmpty) {
+            u = gmu[node] / m + dt * (gx * 1.0f);
+            w = gmv[node] / m + dt * (gy * 1.0f);
+        }
+        if (!(ni > 1)) u = mx(u, 0.0f);
+        if (ni > nx - 3) u = mn(u, 0.0f);
+        if (!(nj > 1)) w = mx(w, 0.0f);
+        if (nj > ny - 3) w = mn(w, 0.0f);
+        gu[node] = u;
+        gv[node] = w;
+    }
+}
+
+// Flip.advectBody, Flip.java:494-545 (not relaxing)
+void orc_mpm2_advect(int nx, int ny, int64_t count, float* const p[10], const int32_t* params, const float* gu,
+                     const float* gv) {
+    const float dt = bits_to_float(params[DT2]);
+    for (int64_t q = 0; q < count; q++) {
+        int32_t col, row;
+        float fx, fy, wx[3], wy[3];
+        axis(p[0][q], nx, &col, &fx, wx);
+        axis(p[1][q], ny, &row, &fy, wy);
+        const int64_t base = (int64_t)row * nx + col;
+        float u = 0, w = 0, b00 = 0, b01 = 0, b10 = 0, b11 = 0;
+        for (int k = 0; k < 9; k++) {
+            const float wgt = wx[k % 3] * wy[k / 3];
+            const float dx = (float)(k % 3) - fx;
+            const float dy = (float)(k / 3) - fy;
+            const int64_t node = base + (int64_t)(k / 3) * nx + k % 3;
+            const float ui = wgt * gu[node];
+            const float vi = wgt * gv[node];
+            u = u + ui;
+            w = w + vi;
+            b00 = b00 + ui * dx;
+            b01 = b01 + ui * dy;
+            b10 = b10 + vi * dx;
+            b11 = b11 + vi * dy;
+        }
+        const float c00 = 4.0f * b00, c01 = 4.0f * b01, c10 = 4.0f * b10, c11 = 4.0f * b11;
+        float x = p[0][q] + dt * u;
+        float y = p[1][q] + dt * w;
+        keep_inside(&x, &u, nx);
+        keep_inside(&y, &w, ny);
+        p[0][q] = x;
+        p[1][q] = y;
+        p[2][q] = u;
+        p[3][q] = w;
+        p[6][q] = c00;
+        p[7][q] = c01;
+        p[8][q] = c10;
+        p[9][q] = c11;
+        const float j = p[5][q] * (1.0f + dt * (c00 + c11));
+        p[5][q] = clampf(j, 1.0f / kJBound, kJBound);
+    }
+}
+
+void orc_mpm2_step(int nx, int ny, int64_t count, float* const p[10], const int32_t* params, float* const grid[5]) {
+    orc_mpm2_clear(nx, ny, grid[0], grid[1], grid[2]);
+    orc_mpm2_scatter(nx, ny, count, p, params, grid[0], grid[1], grid[2]);
+    orc_mpm2_grid(nx, ny, params, grid[0], grid[1], grid[2], grid[3], grid[4]);
+    orc_mpm2_advect(nx, ny, count, p, params, grid[3], grid[4]);
+}
+
+// Scatter.cell, Scatter.java:473-477
+int32_t orc_mpm2_cell(float x, float y, int nx, int ny) {
+    float cx = clampf(x, 0.0f, (float)nx - 1.0f);
+    float cy = clampf(y, 0.0f, (float)ny - 1.0f);
+    return (int32_t)mn(cy, (float)ny - 2.0f) * (nx - 1) + (int32_t)mn(cx, (float)nx - 2.0f);
+}
+
+void orc_mpm2_keys(int nx, int ny, int64_t count, const float* x, const float* y, int32_t* keys) {
+    for (int64_t q = 0; q < count; q++) keys[q] = orc_mpm2_cell(x[q], y[q], nx, ny);
+}
+
+// Scatter.cellStarts, Scatter.java:484-502
+int64_t orc_mpm2_cell_starts(int nx, int ny, int64_t count, const float* x, const float* y, int32_t* starts) {
+    const int64_t cells = (int64_t)(nx - 1) * (ny - 1);
+    std::fill(starts, starts + cells + 1, 0);
+    int32_t previous = 0;
+    for (int64_t q = 0; q < count; q++) {
+        int32_t c = orc_mpm2_cell(x[q], y[q], nx, ny);
+        if (c < previous) return -1 - q;
+        previous = c;
+        starts[c + 1]++;
+    }
+    for (int64_t c = 0; c < cells; c++) starts[c + 1] += starts[c];
+    return 0;
+}
+
+void orc_stable_order(int64_t count, const int32_t* keys, int32_t* order) {
+    std::iota(order, order + count, 0);
+    std::stable_sort(order, order + count, [&](int32_t a, int32_t b) { return keys[a] < keys[b]; });
+}
+
+void orc_bucket_starts(int64_t count, const int32_t* keys, int32_t cells, int32_t* starts) {
+    std::fill(starts, starts + (int64_t)cells + 1, 0);
+    for (int64_t q = 0; q < count; q++) starts[keys[q] + 1]++;
+    for (int64_t c = 0; c < cells; c++) starts[c + 1] += starts[c];
+}
+
+// FlipStep.sort(): count by Scatter.cell, scan, permute every field, copy back. Stable here.
+void orc_mpm2_sort(int nx, int ny, int64_t count, float* const p[10]) {
+    std::vector<int32_t> keys((size_t)count), order((size_t)count);
+    orc_mpm2_keys(nx, ny, count, p[0], p[1], keys.data());
+    orc_stable_order(count, keys.data(), order.data());
+    std::vector<float> tmp((size_t)count);
+    for (int f = 0; f < 10; f++) {
+        for (int64_t q = 0; q < count; q++) tmp[q] = p[f][order[q]];
+        std::memcpy(p[f], tmp.data(), sizeof(float) * count);
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,468 @@
+"""The reference's particle-step tests and the closed forms of MLS-MPM, written once over a rig.
+
+A rig (test/particle/Rig.java) is: ``write(name, float32 array)``, ``params(int32 words)``,
+``run(passes)`` with passes from ("clear", "scatter", "grid", "advect", "sort") or "step",
+and ``read(name)``. The oracle supplies one (``OracleRig3``/``OracleRig2`` below) and the HIP
+runner another; each is judged on its own against the physics, as the reference does.
+"""
+import math
+
+import numpy as np
+
+import oracle
+
+WALL = 1.0
+FIELDS3 = oracle.FIELDS3
+FIELDS2 = oracle.FIELDS2
+AFFINE3 = FIELDS3[8:]
+AFFINE2 = FIELDS2[6:]
+
+
+class OracleRig3:
+    def __init__(self, nx, ny, nz, count):
+        self.s = oracle.Mpm3(nx, ny, nz, {"x": np.zeros(count, np.float32)}, np.zeros(7, np.int32))
+        self.s.p["j"][:] = 0
+
+    def write(self, name, values):
+        target = self.s.p if name in self.s.p else self.s.g
+        target[name][:] = np.asarray(values, np.float32)
+
+    def params(self, words):
+        self.s.params = np.asarray(words, np.int32).copy()
+
+    def run(self, passes):
+        for p in (("clear", "scatter", "grid", "advect") if passes == "step" else passes):
+            getattr(self.s, p)()
+
+    def read(self, name):
+        return (self.s.p if name in self.s.p else self.s.g)[name].copy()
+
+
+class OracleRig2:
+    def __init__(self, nx, ny, count):
+        self.s = oracle.Mpm2(nx, ny, {"x": np.zeros(count, np.float32)}, np.zeros(28, np.int32))
+        self.s.p["j"][:] = 0
+
+    def write(self, name, values):
+        target = self.s.p if name in self.s.p else self.s.g
+        target[name][:] = np.asarray(values, np.float32)
+
+    def params(self, words):
+        self.s.params = np.asarray(words, np.int32).copy()
+
+    def run(self, passes):
+        for p in (("clear", "scatter", "grid", "advect") if passes == "step" else passes):
+            getattr(self.s, p)()
+
+    def read(self, name):
+        return (self.s.p if name in self.s.p else self.s.g)[name].copy()
+
+
+# ------------------------------------------------------------------ 3D
+
+def case_lone_particle_falls_3d(make3):
+    """Flip3Test.aLoneParticleFallsAsGravitySays (test/particle/Flip3Test.java:25-50)."""
+    dt, g, steps = 0.01, -10.0, 5
+    rig = make3(12, 12, 12, 1)
+    rig.write("x", [5.3])
+    rig.write("y", [8.6])
+    rig.write("z", [6.2])
+    rig.write("m", [0.125])
+    rig.write("j", [1.0])
+    rig.params(oracle.mpm3_params(dt, 0, g, 0, 0, 1))
+    for _ in range(steps):
+        rig.run("step")
+    assert abs(rig.read("v")[0] - steps * g * dt) <= 1e-5
+    assert abs(rig.read("y")[0] - (8.6 + g * dt * dt * steps * (steps + 1) / 2)) <= 1e-4
+    assert abs(rig.read("x")[0] - 5.3) <= 1e-5
+    assert abs(rig.read("z")[0] - 6.2) <= 1e-5
+    assert abs(rig.read("u")[0]) <= 1e-6 and abs(rig.read("w")[0]) <= 1e-6
+    assert abs(rig.read("j")[0] - 1) <= 1e-6
+
+
+def scatter_inputs_3d(n=14, count=150, seed=3):
+    """Flip3Test.theScatterConservesMassAndMomentum's particles (:60-86), same draw order."""
+    d = oracle.JavaRandom(seed).pattern("f" * 17, count)
+    f = {}
+    f["x"] = 1 + d[:, 0] * np.float32(n - 3)
+    f["y"] = 1 + d[:, 1] * np.float32(n - 3)
+    f["z"] = 1 + d[:, 2] * np.float32(n - 3)
+    f["m"] = np.float32(0.1) + d[:, 3] * np.float32(0.3)
+    f["j"] = np.float32(0.8) + d[:, 4] * np.float32(0.4)
+    for a, name in enumerate("uvw"):
+        f[name] = d[:, 5 + a] * np.float32(4) - np.float32(2)
+    for a, name in enumerate(AFFINE3):
+        f[name] = d[:, 8 + a] * np.float32(2) - np.float32(1)
+    return {k: v.astype(np.float32) for k, v in f.items()}
+
+
+def load3(rig, fields):
+    for name, values in fields.items():
+        rig.write(name, values)
+
+
+def case_scatter_conserves_3d(make3):
+    """Flip3Test.theScatterConservesMassAndMomentum (test/particle/Flip3Test.java:57-118)."""
+    n, count = 14, 150
+    f = scatter_inputs_3d(n, count)
+    rig = make3(n, n, n, count)
+    load3(rig, f)
+    rig.params(oracle.mpm3_params(0.01, 0, -10, 0, 1e4, 1))
+    rig.run(["clear", "scatter"])
+    mass = float(np.sum(f["m"].astype(np.float64)))
+    assert abs(float(np.sum(rig.read("gm").astype(np.float64))) - mass) <= 1e-4 * mass
+    for a, (v, g) in enumerate(zip("uvw", ("gmu", "gmv", "gmw"))):
+        mom = float(np.sum(f["m"].astype(np.float64) * f[v]))
+        tot = float(np.sum(rig.read(g).astype(np.float64)))
+        assert abs(mom - tot) <= 1e-3 * max(1, abs(mom)) + 1e-3, (a, mom, tot)
+    # The north star's bar: grid mass equal to the particles' to 1e-6 relative.
+    assert abs(float(np.sum(rig.read("gm").astype(np.float64))) - mass) <= 1e-6 * mass
+
+
+def _affine_particles(dims, ppc, seed, A, b):
+    """Particles jittered over the interior cells, each on the affine field v = A x + b, C = A."""
+    rng = oracle.JavaRandom(seed)
+    lo, hi = 3, [n - 4 for n in dims]
+    cells = [np.arange(lo, h) for h in hi]
+    grid = np.stack(np.meshgrid(*cells, indexing="ij"), -1).reshape(-1, len(dims))
+    grid = np.repeat(grid, ppc, axis=0)
+    jit = rng.pattern("f" * len(dims), len(grid))
+    pos = (grid.astype(np.float32) + jit * np.float32(0.999)).astype(np.float32)
+    vel = (pos.astype(np.float64) @ np.asarray(A).T + np.asarray(b)).astype(np.float32)
+    return pos, vel
+
+
+def case_affine_field_is_reproduced_3d(make3):
+    """APIC's defining property (Flip3.java:40-42): particles carrying v = A x + b with C = A deposit
+    momentum (A x_i + b) m_i on every node, so the grid pass (gravity and pressure off) gives back
+    the affine field at every node the particles reach fully -- a closed form, exact up to
+    rounding, that fixes C's orientation, d's sign and the stencil offsets. Then G2P of that
+    field returns each particle's v = A x_p + b and C = A (quadratic B-spline: D = I/4)."""
+    n = 16
+    A = np.array([[0.3, -0.7, 0.2], [0.5, 0.1, -0.4], [-0.6, 0.8, 0.25]])
+    b = np.array([0.4, -0.3, 0.2])
+    pos, vel = _affine_particles((n, n, n), 4, 11, A, b)
+    count = len(pos)
+    rig = make3(n, n, n, count)
+    for a, name in enumerate("xyz"):
+        rig.write(name, pos[:, a])
+    for a, name in enumerate("uvw"):
+        rig.write(name, vel[:, a])
+    rig.write("m", np.full(count, 0.25, np.float32))
+    rig.write("j", np.ones(count, np.float32))
+    for k, name in enumerate(AFFINE3):
+        rig.write(name, np.full(count, A[k // 3, k % 3], np.float32))
+    rig.params(oracle.mpm3_params(1e-3, 0, 0, 0, 0, 1))
+    rig.run(["clear", "scatter", "grid"])
+    gm = rig.read("gm")
+    gvel = [rig.read(g) for g in ("gu", "gv", "gw")]
+    idx = np.arange(n)
+    X = np.stack(np.meshgrid(idx, idx, idx, indexing="ij"), -1)[..., ::-1].reshape(-1, 3)  # (i,j,k)->x,y,z
+    # node index (k*ny + j)*nx + i: build coordinates in that order
+    kk, jj, ii = np.meshgrid(idx, idx, idx, indexing="ij")
+    coords = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], -1).astype(np.float64)
+    inner = np.all((coords >= 5) & (coords <= n - 6), axis=1)
+    assert inner.sum() > 50 and np.all(gm[inner] > 0.5)
+    expect = coords @ A.T + b
+    for a in range(3):
+        np.testing.assert_allclose(gvel[a][inner], expect[inner, a], rtol=0, atol=2e-5)
+    del X
+    # G2P of the exact affine field on every node.
+    for a, g in enumerate(("gu", "gv", "gw")):
+        rig.write(g, expect[:, a].astype(np.float32))
+    rig.run(["advect"])
+    back = np.stack([rig.read(v) for v in "uvw"], -1).astype(np.float64)
+    inside = np.all((pos >= 4) & (pos <= n - 5), axis=1)
+    vexp = pos.astype(np.float64) @ A.T + b
+    np.testing.assert_allclose(back[inside], vexp[inside], atol=2e-5)
+    for k, name in enumerate(AFFINE3):
+        np.testing.assert_allclose(rig.read(name)[inside], A[k // 3, k % 3], atol=3e-5)
+
+
+def case_compressed_particle_pushes_out_3d(make3):
+    """Pressure as a force (Flip.java:71-82): a lone particle at rest with J < 1 (compressed) pushes
+    every node away from itself, one with J > 1 pulls in, and the impulses sum to zero."""
+    for j, sign in ((0.9, 1.0), (1.1, -1.0)):
+        rig = make3(10, 10, 10, 1)
+        rig.write("x", [4.3])
+        rig.write("y", [5.6])
+        rig.write("z", [4.8])
+        rig.write("m", [0.5])
+        rig.write("j", [j])
+        rig.params(oracle.mpm3_params(1e-3, 0, 0, 0, 100.0, 1))
+        rig.run(["clear", "scatter"])
+        kk, jj, ii = np.meshgrid(np.arange(10), np.arange(10), np.arange(10), indexing="ij")
+        d = np.stack([ii.ravel() - 4.3, jj.ravel() - 5.6, kk.ravel() - 4.8], -1)
+        mom = np.stack([rig.read(g) for g in ("gmu", "gmv", "gmw")], -1).astype(np.float64)
+        touched = rig.read("gm") > 0
+        radial = np.sum(mom[touched] * d[touched], axis=1)
+        assert np.all(sign * radial > 0)
+        assert np.all(np.abs(mom.sum(axis=0)) < 1e-6)
+
+
+def case_walls_hold_3d(make3):
+    """keepInside (Flip3.java:644-655): a particle flung at the walls stops on them with the
+    velocity into the wall dropped and the velocity along it kept."""
+    rig = make3(10, 10, 10, 2)
+    rig.write("x", [1.2, 7.8])
+    rig.write("y", [4.0, 4.0])
+    rig.write("z", [4.0, 7.9])
+    rig.write("u", [-500.0, 500.0])
+    rig.write("v", [3.0, 3.0])
+    rig.write("w", [0.0, 500.0])
+    rig.write("m", [0.1, 0.1])
+    rig.write("j", [1.0, 1.0])
+    rig.params(oracle.mpm3_params(0.01, 0, 0, 0, 0, 1))
+    rig.run("step")
+    x, z, u, w = rig.read("x"), rig.read("z"), rig.read("u"), rig.read("w")
+    assert x[0] == 1.0 and u[0] >= 0
+    assert x[1] == 8.0 and u[1] <= 0 and z[1] == 8.0 and w[1] <= 0
+
+
+SLAB_N, SLAB_PPC, SLAB_W, SLAB_H, SLAB_G = 24, 8, 6, 12, 200.0
+
+
+def slab_inputs():
+    """Flip3Test's slab (test/particle/Flip3Test.java:144-163), Random(71)."""
+    depth = SLAB_N - 3
+    count = SLAB_W * SLAB_H * depth * SLAB_PPC
+    d = oracle.JavaRandom(71).pattern("fff", count)
+    idx = np.arange(count) // SLAB_PPC
+    col = idx % SLAB_W
+    row = (idx // SLAB_W) % SLAB_H
+    layer = idx // (SLAB_W * SLAB_H)
+    j = np.float32(0.999)
+    x = (WALL + col).astype(np.float32) + d[:, 0] * j
+    y = (WALL + row).astype(np.float32) + d[:, 1] * j
+    z = (WALL + layer).astype(np.float32) + d[:, 2] * j
+    return x, y, z, np.full(count, np.float32(1.0 / SLAB_PPC), np.float32)
+
+
+def case_dam_break_slab_3d(make3):
+    """Flip3Test.aDamBreakInASlabStaysWaterInItsBoxAndDoesNotMoveAlongZ (:134-221)."""
+    fall = math.sqrt(2 * SLAB_G * SLAB_H)
+    sound = 5 * fall
+    bulk = sound * sound
+    dt = 0.3 / (math.sqrt(bulk) + fall)
+    early = math.ceil(0.12 / dt)
+    steps = math.ceil(0.4 / dt)
+    x, y, z, m = slab_inputs()
+    count = len(x)
+    rig = make3(SLAB_N, SLAB_N, SLAB_N, count)
+    rig.write("x", x)
+    rig.write("y", y)
+    rig.write("z", z)
+    rig.write("m", m)
+    rig.write("j", np.ones(count, np.float32))
+    rig.params(oracle.mpm3_params(dt, 0, -SLAB_G, 0, bulk, 1))
+    moved = 0.0
+    for s in range(steps):
+        rig.run("step")
+        if s == early - 1:
+            moved = float(rig.read("x").max()) - (WALL + SLAB_W)
+    px, py, pz, pu, pw, pj = (rig.read(f) for f in ("x", "y", "z", "u", "w", "j"))
+    for a in (px, py, pz, pj):
+        assert np.all(np.isfinite(a))
+    for a in (px, py, pz):
+        assert np.all((a >= WALL) & (a <= SLAB_N - 1 - WALL))
+    ritter = 2 * math.sqrt(SLAB_G * SLAB_H) * early * dt
+    sj = np.sort(pj)
+    sy = np.sort(py)
+    j_lo, j_hi = sj[int(0.01 * count)], sj[int(0.99 * count)]
+    height = sy[int(0.95 * count)] - WALL
+    along_z = math.sqrt(np.mean(pw.astype(np.float64) ** 2)) / math.sqrt(np.mean(pu.astype(np.float64) ** 2))
+    assert 1 < moved < ritter * 1.05, (moved, ritter)
+    assert 0.9 < j_lo and j_hi < 1.1, (j_lo, j_hi)
+    assert height < SLAB_H
+    assert along_z < 0.2
+    return moved, ritter, j_lo, j_hi, height, along_z
+
+
+# ------------------------------------------------------------------ 2D
+
+def case_lone_particle_falls_2d(make2):
+    """FlipTest.aLoneParticleFallsAsGravitySays (test/particle/FlipTest.java:27-47)."""
+    dt, g, steps = 0.01, -10.0, 5
+    rig = make2(16, 16, 1)
+    rig.write("x", [7.3])
+    rig.write("y", [12.6])
+    rig.write("m", [0.25])
+    rig.write("j", [1.0])
+    rig.params(oracle.mpm2_params(dt, 0, g, 0, 1))
+    for _ in range(steps):
+        rig.run("step")
+    assert abs(rig.read("v")[0] - steps * g * dt) <= 1e-5
+    assert abs(rig.read("y")[0] - (12.6 + g * dt * dt * steps * (steps + 1) / 2)) <= 1e-4
+    assert abs(rig.read("x")[0] - 7.3) <= 1e-5
+    assert abs(rig.read("u")[0]) <= 1e-6
+    assert abs(rig.read("j")[0] - 1) <= 1e-6
+
+
+def splat_fp64(x, y, u, v, m, nx, ny):
+    """ParticleSplat.grid (core/particle/ParticleSplat.java:21-47), float64, vectorised: the host's
+    own restatement of the quadratic weights and clamp."""
+    px = np.clip(x.astype(np.float64), 0.5, nx - 1.5)
+    py = np.clip(y.astype(np.float64), 0.5, ny - 1.5)
+    col = np.minimum(px - 0.5, nx - 3).astype(np.int64)
+    row = np.minimum(py - 0.5, ny - 3).astype(np.int64)
+
+    def w(f):
+        return np.stack([0.5 * (1.5 - f) ** 2, 0.75 - (f - 1) ** 2, 0.5 * (f - 0.5) ** 2], -1)
+
+    wx, wy = w(px - col), w(py - row)
+    out = np.zeros((3, nx * ny))
+    for k in range(9):
+        node = (row + k // 3) * nx + col + k % 3
+        ww = wx[:, k % 3] * wy[:, k // 3] * m
+        np.add.at(out[0], node, ww)
+        np.add.at(out[1], node, ww * u)
+        np.add.at(out[2], node, ww * v)
+    return out
+
+
+def jittered2(cells, ppc, seed):
+    """ScatterTest.Particles.jittered (test/particle/ScatterTest.java:460-478), Random(seed)."""
+    d = oracle.JavaRandom(seed).pattern("ffggf", cells * cells * ppc)
+    k = np.arange(cells * cells * ppc)
+    col = ((k // ppc) % cells).astype(np.float32)
+    row = ((k // ppc) // cells).astype(np.float32)
+    x = np.minimum(col + d[:, 0], np.nextafter(col + np.float32(1), np.float32(0)))
+    y = np.minimum(row + d[:, 1], np.nextafter(row + np.float32(1), np.float32(0)))
+    return {"x": x.astype(np.float32), "y": y.astype(np.float32), "u": d[:, 2].copy(), "v": d[:, 3].copy(),
+            "m": (np.float32(0.5) + d[:, 4]).astype(np.float32)}
+
+
+def case_scatter_matches_fp64_splat_2d(make2, n=64, ppc=4, seed=41, order_seed=None):
+    """The scatter with C = 0 and J = 1 is ParticleSplat's: per node within 1e-5 max(1, |e|)
+    (the tolerance of test/particle/ScatterTest.java:125-144), mass to 1e-6."""
+    p = jittered2(n - 1, ppc, seed)
+    if order_seed is not None:
+        order = oracle.JavaRandom(order_seed).shuffle_order(len(p["x"]))
+        p = {k: v[order] for k, v in p.items()}
+    count = len(p["x"])
+    rig = make2(n, n, count)
+    for k, v in p.items():
+        rig.write(k, v)
+    rig.write("j", np.ones(count, np.float32))
+    rig.params(oracle.mpm2_params(1e-3, 0, 0, 10.0, 1))
+    rig.run(["clear", "scatter"])
+    e = splat_fp64(p["x"], p["y"], p["u"], p["v"], p["m"], n, n)
+    got = [rig.read(g) for g in ("gm", "gmu", "gmv")]
+    mass = float(np.sum(p["m"].astype(np.float64)))
+    assert abs(float(np.sum(got[0].astype(np.float64))) - mass) <= 1e-6 * mass
+    for f in range(3):
+        err = np.abs(got[f] - e[f])
+        tol = 1e-5 * np.maximum(1, np.abs(e[f]))
+        assert np.all(err <= tol), (f, float(err.max()))
+    return p
+
+
+def case_broken_runs_2d(make2):
+    """ScatterTest.theSegmentedSumKeepsBrokenRunsApart (:88-108) for the quadratic stencil: one
+    stencil's particles in two runs of a wave with another between, 301 particles so the last
+    wave has a tail; the grid must still be the FP64 splat's."""
+    n = 8
+    cells = [(2, 2), (3, 2), (2, 3)]
+    pattern = [0, 0, 1, 0, 2, 2, 2, 0, 1, 1, 0]
+    count = 301
+    d = oracle.JavaRandom(51).pattern("ffggf", count)
+    cx = np.array([cells[pattern[k % len(pattern)]][0] for k in range(count)], np.float32)
+    cy = np.array([cells[pattern[k % len(pattern)]][1] for k in range(count)], np.float32)
+    p = {"x": cx + d[:, 0] * np.float32(0.999), "y": cy + d[:, 1] * np.float32(0.999),
+         "u": d[:, 2].copy(), "v": d[:, 3].copy(), "m": np.float32(0.5) + d[:, 4]}
+    p = {k: v.astype(np.float32) for k, v in p.items()}
+    rig = make2(n, n, count)
+    for k, v in p.items():
+        rig.write(k, v)
+    rig.write("j", np.ones(count, np.float32))
+    rig.params(oracle.mpm2_params(1e-3, 0, 0, 0, 1))
+    rig.run(["clear", "scatter"])
+    e = splat_fp64(p["x"], p["y"], p["u"], p["v"], p["m"], n, n)
+    for f, g in enumerate(("gm", "gmu", "gmv")):
+        got = rig.read(g)
+        assert np.all(np.abs(got - e[f]) <= 1e-5 * np.maximum(1, np.abs(e[f]))), g
+
+
+def case_affine_field_is_reproduced_2d(make2):
+    n = 24
+    A = np.array([[0.3, -0.7], [0.5, 0.1]])
+    b = np.array([0.4, -0.3])
+    pos, vel = _affine_particles((n, n), 4, 13, A, b)
+    count = len(pos)
+    rig = make2(n, n, count)
+    rig.write("x", pos[:, 0])
+    rig.write("y", pos[:, 1])
+    rig.write("u", vel[:, 0])
+    rig.write("v", vel[:, 1])
+    rig.write("m", np.full(count, 0.25, np.float32))
+    rig.write("j", np.ones(count, np.float32))
+    for k, name in enumerate(AFFINE2):
+        rig.write(name, np.full(count, A[k // 2, k % 2], np.float32))
+    rig.params(oracle.mpm2_params(1e-3, 0, 0, 0, 1))
+    rig.run(["clear", "scatter", "grid"])
+    jj, ii = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    coords = np.stack([ii.ravel(), jj.ravel()], -1).astype(np.float64)
+    inner = np.all((coords >= 5) & (coords <= n - 6), axis=1)
+    expect = coords @ A.T + b
+    for a, g in enumerate(("gu", "gv")):
+        np.testing.assert_allclose(rig.read(g)[inner], expect[inner, a], atol=2e-5)
+    for a, g in enumerate(("gu", "gv")):
+        rig.write(g, expect[:, a].astype(np.float32))
+    rig.run(["advect"])
+    inside = np.all((pos >= 4) & (pos <= n - 5), axis=1)
+    back = np.stack([rig.read("u"), rig.read("v")], -1).astype(np.float64)
+    np.testing.assert_allclose(back[inside], (pos.astype(np.float64) @ A.T + b)[inside], atol=2e-5)
+    for k, name in enumerate(AFFINE2):
+        np.testing.assert_allclose(rig.read(name)[inside], A[k // 2, k % 2], atol=3e-5)
+
+
+COL_N, COL_PPC, COL_W, COL_H, COL_G = 32, 4, 8, 16, 200.0
+
+
+def column2():
+    """FlipTest.column(new Random(71)) (test/particle/FlipTest.java:134-150)."""
+    count = COL_W * COL_H * COL_PPC
+    d = oracle.JavaRandom(71).pattern("ff", count)
+    idx = np.arange(count) // COL_PPC
+    col = idx % COL_W
+    row = idx // COL_W
+    j = np.float32(0.999)
+    x = (WALL + col).astype(np.float32) + d[:, 0] * j
+    y = (WALL + row).astype(np.float32) + d[:, 1] * j
+    return x, y, np.full(count, np.float32(1.0 / COL_PPC), np.float32)
+
+
+def case_dam_break_2d(make2):
+    """FlipTest.aDamBreakStaysWaterInItsBox (test/particle/FlipTest.java:72-127), sorted every 10."""
+    fall = math.sqrt(2 * COL_G * COL_H)
+    sound = 5 * fall
+    bulk = sound * sound
+    dt = 0.3 / (math.sqrt(bulk) + fall)
+    early = math.ceil(0.1 / dt)
+    steps = math.ceil(0.6 / dt)
+    x, y, m = column2()
+    count = len(x)
+    rig = make2(COL_N, COL_N, count)
+    rig.write("x", x)
+    rig.write("y", y)
+    rig.write("m", m)
+    rig.write("j", np.ones(count, np.float32))
+    rig.params(oracle.mpm2_params(dt, 0, -COL_G, bulk, 1))
+    moved = 0.0
+    for k in range(steps):
+        if k % 10 == 0:
+            rig.run(["sort"])
+        rig.run("step")
+        if k == early - 1:
+            moved = float(rig.read("x").max()) - (WALL + COL_W)
+    px, py, pj = rig.read("x"), rig.read("y"), rig.read("j")
+    assert np.all(np.isfinite(px)) and np.all(np.isfinite(py)) and np.all(np.isfinite(pj))
+    assert np.all((px >= WALL) & (px <= COL_N - 1 - WALL) & (py >= WALL) & (py <= COL_N - 1 - WALL))
+    ritter = 2 * math.sqrt(COL_G * COL_H) * early * dt
+    sj, sy = np.sort(pj), np.sort(py)
+    j_lo, j_hi = sj[int(0.01 * count)], sj[int(0.99 * count)]
+    height = sy[int(0.95 * count)] - WALL
+    assert 1 < moved < ritter * 1.05, (moved, ritter)
+    assert 0.9 < j_lo and j_hi < 1.1, (j_lo, j_hi)
+    assert height < COL_H
+    return moved, ritter, j_lo, j_hi, height
