@@ -1,0 +1,180 @@
+"""Benchmark of the hot path on one MI355X.
+
+Default workload: the 3D MLS-MPM/APIC substep at 8,388,608 particles in a 256^3 box (two seeded
+cubes of water, 8 particles a cell, dt = 5e-5), sorted-segmented P2G with the particles
+re-sorted by cell every 10 substeps (the sort's cost is inside the timed region). A "step" is
+one substep: clear, P2G, grid update, G2P. Prints one JSON line; beside the headline it carries
+the dominant kernel's roofline from HIP events on the library's own stream, the HLL step on the
+4096^2 shallow-water scene, and the CPU oracle timed on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "vexelray-sim-fluid_amd"))
+
+import numpy as np  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0
+SORT_EVERY = 10
+
+# Algorithmic bytes per unit (particle, node, element) of each kernel: what the pass must move
+# through HBM at least, as derived in the design notes.
+BYTES = {
+    "p2g3_tiled": ("particle", 68), "p2g3_atomic": ("particle", 68),
+    "g2p3_tiled": ("particle", 84), "g2p3": ("particle", 84),
+    "grid3": ("node", 28), "mpm3_clear": ("node", 16),
+    "keys3": ("particle", 20), "radix_hist": ("element", 4), "radix_scatter": ("element", 16),
+    "radix_scan": ("none", 0), "permute": ("particle", 140),
+    "hll_step": ("cell", 24),
+}
+
+
+def kernel_table(sim, names, units):
+    out = {}
+    for k in names:
+        ms, n = sim.kernel_time(k)
+        if n == 0:
+            continue
+        unit, per = BYTES.get(k, ("none", 0))
+        count = units.get(unit, 0)
+        avg_ms = ms / n
+        gbs = per * count / (avg_ms * 1e-3) / 1e9 if per and avg_ms > 0 else None
+        out[k] = {"launches": n, "avg_us": round(avg_ms * 1e3, 3), "total_ms": round(ms, 3),
+                  "algorithmic_bytes": per * count, "achieved_gbs": None if gbs is None else round(gbs, 1)}
+    return out
+
+
+def bench_mpm3(steps, warmup, p2g, g2p, run):
+    from vexhip import particles, scenes
+
+    s = scenes.mpm3_two_cubes()
+    count = len(s["x"])
+    g, bulk, rho0 = scenes.mpm3_units(256)
+    words = particles.params3(5e-5, 0, g, 0, bulk, rho0)
+    sim = particles.ParticleSimulation3(256, 256, 256, count)
+    sim.sim.set_option("p2g", p2g)
+    sim.sim.set_option("g2p", g2p)
+    sim.sim.set_option("run", run)
+    z = np.zeros(count, np.float32)
+    sim.load((s["x"], s["y"], s["z"]), (z, z, z), s["m"], words)
+    sim.advance(warmup, sort_every=SORT_EVERY)
+    sim.finish()
+    sim.sim.timing(True)
+    t0 = time.perf_counter()
+    sim.advance(steps, sort_every=SORT_EVERY)
+    sim.finish()
+    t1 = time.perf_counter()
+    names = ["mpm3_clear", "p2g3_tiled", "p2g3_atomic", "grid3", "g2p3_tiled", "g2p3", "keys3", "radix_hist",
+             "radix_scan", "radix_scatter", "permute"]
+    units = {"particle": count, "node": 256 ** 3, "element": count}
+    table = kernel_table(sim.sim, names, units)
+    sim.sim.timing(False)
+    mass = float(np.sum(sim.grid()[0], dtype=np.float64))
+    sim.close()
+    return count, t1 - t0, table, mass / float(np.sum(s["m"], dtype=np.float64)) - 1.0
+
+
+def bench_hll(steps, n=4096):
+    from vexhip import patch, scenes
+
+    h, hu, hv = scenes.swe_dam_bumps(n, seed=0)
+    sim = patch.PatchSimulation(n, n)
+    sim.load(h, hu, hv, patch.params(9.81, 0.45, 1.0 / n, 1.0 / n))
+    sim.write_budget(2**31 - 1)
+    sim.dispatch(10)
+    sim.sim.finish()
+    sim.sim.timing(True)
+    t0 = time.perf_counter()
+    sim.dispatch(steps)
+    sim.sim.finish()
+    t1 = time.perf_counter()
+    table = kernel_table(sim.sim, ["hll_step"], {"cell": n * n})
+    sim.close()
+    return n * n * steps / (t1 - t0), table
+
+
+def cpu_baseline_mpm3():
+    """The oracle (scalar C++, one thread) on one whole substep of the same 8,388,608-particle
+    workload: about 10-30 s of CPU."""
+    import oracle
+    from vexhip import scenes
+
+    s = scenes.mpm3_two_cubes()
+    g, bulk, rho0 = scenes.mpm3_units(256)
+    words = oracle.mpm3_params(5e-5, 0, g, 0, bulk, rho0)
+    ref = oracle.Mpm3(256, 256, 256, {"x": s["x"], "y": s["y"], "z": s["z"], "m": s["m"]}, words)
+    t0 = time.perf_counter()
+    ref.step(1)
+    t1 = time.perf_counter()
+    return {"value": round(len(s["x"]) / (t1 - t0), 1), "unit": "particle-substeps/s", "cores": 1, "kind": "port",
+            "sample": f"one substep (clear, scatter, grid, advect) of the full {len(s['x'])}-particle 256^3 workload "
+                      f"by the scalar oracle, {t1 - t0:.2f} s", "seconds": round(t1 - t0, 3),
+            "host_cores": os.cpu_count()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--p2g", type=int, default=1)
+    ap.add_argument("--g2p", type=int, default=1)
+    ap.add_argument("--run", type=int, default=4)
+    ap.add_argument("--hll-steps", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-hll", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the path is single-GPU; "
+              "the design notes say how it would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    from vexhip import _native
+    if _native.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+
+    print(f"[bench] 3D MPM: 8,388,608 particles, 256^3, {args.warmup} warmup + {args.steps} steps", file=sys.stderr,
+          flush=True)
+    count, secs, table, drift = bench_mpm3(args.steps, args.warmup, args.p2g, args.g2p, args.run)
+    value = count * args.steps / secs
+    dominant = max(table.items(), key=lambda kv: kv[1]["total_ms"])
+    dk, dv = dominant
+    roofline = {"kernel": dk, "bound": "hbm", "achieved": dv["achieved_gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": None if dv["achieved_gbs"] is None else round(dv["achieved_gbs"] / PEAK_HBM_GBS, 4),
+                "traffic": None, "bytes_per_launch": dv["algorithmic_bytes"], "avg_us": dv["avg_us"]}
+    line = {
+        "metric": "particle-substeps/s (3D MPM) & cell-updates/s (HLL); surrogate vs HLL at ≤1% err",
+        "value": round(value, 1), "unit": "particle-substeps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(secs / args.steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "3D MLS-MPM/APIC substep, 8,388,608 particles (two seeded cubes, 8 per cell), "
+                               "256^3 nodes, dt=5e-5, sorted-segmented P2G, sort every 10 substeps",
+                   "particles": count, "grid": [256, 256, 256], "sort_every": SORT_EVERY,
+                   "p2g": ["atomic", "sorted-segmented"][args.p2g], "g2p": ["gather", "lds-tile"][args.g2p],
+                   "run": args.run},
+        "roofline": roofline, "kernels": table, "grid_mass_drift": drift,
+    }
+    if not args.no_hll:
+        print(f"[bench] HLL 4096^2, {args.hll_steps} steps", file=sys.stderr, flush=True)
+        rate, htable = bench_hll(args.hll_steps)
+        hk = htable["hll_step"]
+        line["hll"] = {"value": round(rate, 1), "unit": "cell-updates/s", "steps": args.hll_steps,
+                       "workload": "HLL 4096^2 circular dam + 32 seeded bumps, walls, CFL 0.45",
+                       "roofline": {"kernel": "hll_step", "bound": "hbm", "achieved": hk["achieved_gbs"],
+                                    "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                    "frac": round(hk["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": None,
+                                    "avg_us": hk["avg_us"]}}
+    if not args.no_cpu:
+        print("[bench] CPU oracle baseline (one substep, one thread)", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline_mpm3()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

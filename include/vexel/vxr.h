@@ -1,0 +1,136 @@
+/*
+ * vxr.h -- the C ABI of libvexelhip.so: the MI355X-native hot path of vexelray-sim-fluid
+ * (the HLL shallow-water step and the MLS-MPM/APIC substep) behind the reference's
+ * simulator API, for a Java FFM (or any C FFI) host.
+ *
+ * Every entry point returns a status code (VXR_OK or an error below) and leaves a message
+ * for vxr_last_error() on failure. Plain pointers, sizes and 32-bit words only.
+ *
+ * Reference paths are shortened: core/ = vexelray-sim-fluid-core/src/main/java/dev/vexelray/sim/fluid/,
+ * gui/ = vexelray-sim-fluid-gui/src/main/java/dev/vexelray/sim/fluid/gui/.
+ */
+#ifndef VEXEL_VXR_H
+#define VEXEL_VXR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. The Java shim maps EINVAL to IllegalArgumentException and ESTATE to
+ * IllegalStateException, as the reference throws them (core/stencil/ShallowWater.java:167-169,
+ * core/particle/FlipStep.java:83-88, gui/ParticleSimulation.java:147-149). */
+#define VXR_OK 0
+#define VXR_EINVAL 1       /* bad shape, name, length or parameter */
+#define VXR_ESTATE 2       /* a call that does not fit the simulation's kind or state */
+#define VXR_EDEVICE 3      /* a HIP error; the message carries hipGetErrorString */
+#define VXR_ENOMEM 4       /* device allocation failed */
+#define VXR_EUNSUPPORTED 5 /* a flag outside the built path (tension, heat, pump, ...) */
+
+/* Simulation kinds. */
+#define VXR_KIND_MPM2 1 /* gui/ParticleSimulation.java over core/particle/FlipStep.java */
+#define VXR_KIND_MPM3 2 /* gui/ParticleSimulation3.java over core/particle/Flip3Step.java */
+#define VXR_KIND_SWE 3  /* gui/PatchSimulation.java over core/stencil/ShallowWater.java */
+
+/* Creation flags. The reference's optional physics (FlipStep's tension, heat, convection,
+ * relax, pump; Flip3Step's tension, plastic) are outside this path: VXR_EUNSUPPORTED. */
+#define VXR_FLAG_TENSION (1u << 0)
+#define VXR_FLAG_HEAT (1u << 1)
+#define VXR_FLAG_CONVECTION (1u << 2)
+#define VXR_FLAG_RELAX (1u << 3)
+#define VXR_FLAG_PUMP (1u << 4)
+#define VXR_FLAG_PLASTIC (1u << 5)
+/* Shallow-water edges (core/stencil/Edges.java): set = OPEN, clear = WALL. */
+#define VXR_FLAG_OPEN_WEST (1u << 16)
+#define VXR_FLAG_OPEN_EAST (1u << 17)
+#define VXR_FLAG_OPEN_SOUTH (1u << 18)
+#define VXR_FLAG_OPEN_NORTH (1u << 19)
+
+/* Pass groups for vxr_run: the step (clear, scatter, grid, advect) and the sort by cell
+ * (keys, radix passes, permutation of every particle field). */
+#define VXR_GROUP_STEP 0
+#define VXR_GROUP_SORT 1
+
+typedef struct vxr_sim vxr_sim;
+
+/* The message of the last failed call on this thread; never NULL. */
+const char* vxr_last_error(void);
+
+/* Number of HIP devices visible; *n = 0 and VXR_OK when there is none. */
+int vxr_device_count(int* n);
+
+/* Makes a simulation on the current HIP device with every buffer allocated and zeroed
+ * (PassRunner.gpu + runner.clear(), gui/ParticleSimulation.java:76-80):
+ *   MPM2: dims = {nx, ny, -}, nx, ny >= 4 (core/particle/FlipStep.java:81-193)
+ *   MPM3: dims = {nx, ny, nz}, each >= 4 (core/particle/Flip3Step.java:53-123)
+ *   SWE:  dims = {nx, ny, -}, each >= 1, particles ignored (gui/PatchSimulation.java:58-76)
+ * Safe on any thread; the handle then belongs to the thread that uses it. */
+int vxr_create(int kind, const int32_t dims[3], int64_t particles, uint32_t flags, vxr_sim** out);
+
+/* Frees every device buffer and the stream (ParticleSimulation.close, :228-231). */
+int vxr_destroy(vxr_sim* sim);
+
+/* Writes `n` 32-bit words into the named buffer, synchronously, from host memory the call
+ * only borrows (PassRunner.write / ResidentBuffer.write). `n` must equal the buffer's
+ * length; "params" also takes a shorter prefix. Names: MPM3 x y z u v w m j c00..c22 gm gmu
+ * gmv gmw gu gv gw params keys; MPM2 x y u v m j c00 c01 c10 c11 gm gmu gmv gu gv params keys
+ * starts; SWE h hu hv params speed budget clamped (the current slot of each rotating set,
+ * as gui/PatchSimulation.java:147-213 resolves it). */
+int vxr_write(vxr_sim* sim, const char* name, const void* words, int64_t n);
+
+/* Reads `n` words of the named buffer after every queued pass has finished (a readback,
+ * as PassRunner.read / ResidentBuffer.read). */
+int vxr_read(vxr_sim* sim, const char* name, void* words, int64_t n);
+
+/* The named buffer's length in words. */
+int vxr_length(vxr_sim* sim, const char* name, int64_t* n);
+
+/* Queues `repeat` runs of a pass group (PassRunner.run(step.step()) / run(step.sort()),
+ * gui/ParticleSimulation.java:175-183, gui/ParticleSimulation3.java:291-296). Asynchronous. */
+int vxr_run(vxr_sim* sim, int group, int repeat);
+
+/* Queues one named pass: "clear", "scatter", "grid", "advect", "sort" (a sub-list of the
+ * step, as test/particle/Flip3Test.java:102 runs step().subList(0, 2)). */
+int vxr_run_pass(vxr_sim* sim, const char* pass, int repeat);
+
+/* SWE only: queues `count` steps with the reference's buffer rotation -- state k%2 -> (k+1)%2,
+ * speed read k%3 / max (k+1)%3 / clear (k+2)%3, budget k%2 -> (k+1)%2
+ * (gui/PatchSimulation.java:197-205, core/stencil/ShallowWater.java:92-94). */
+int vxr_swe_dispatch(vxr_sim* sim, int count);
+
+/* Zeroes every buffer (PassRunner.clear). */
+int vxr_clear(vxr_sim* sim);
+
+/* Blocks until every queued pass has finished (ParticleSimulation3.finish, :400-402). */
+int vxr_finish(vxr_sim* sim);
+
+/* Lends the device address of a named buffer (ParticleSimulation3.vkBuffer, :377-383). Grid
+ * and SWE addresses are valid until vxr_destroy; particle fields move to a new buffer when
+ * a sort permutes them, so a particle address is valid until the next sort. For the SWE
+ * state the address is that of the current slot, which changes with each dispatch. */
+int vxr_device_ptr(vxr_sim* sim, const char* name, void** ptr);
+
+/* The hipStream_t the simulation queues on, for a caller that orders its own work after it. */
+int vxr_stream(vxr_sim* sim, void** stream);
+
+/* Schedule knobs (no reference counterpart; the autotuner sets them):
+ *   "p2g"        MPM: 0 = direct float atomics (the reference's Flip3.scatter schedule),
+ *                1 = sorted-segmented: per-thread runs, an LDS node tile, one atomic per node (default)
+ *   "g2p"        MPM: 0 = per-particle gathers, 1 = LDS node tile (default)
+ *   "run"        MPM: particles a thread folds per run in the segmented passes (1..8, default 4)
+ *   "radix_bits" sort digit width (4..8, default 8)
+ *   "rows"       SWE: rows a wave marches per tile (4..256, default 32) */
+int vxr_set_option(vxr_sim* sim, const char* key, int64_t value);
+int vxr_get_option(vxr_sim* sim, const char* key, int64_t* value);
+
+/* Kernel timing with HIP events on the simulation's own stream. Enabling clears the
+ * totals; vxr_kernel_time synchronizes and returns the summed milliseconds and launch
+ * count of every launch of the named kernel since then. */
+int vxr_timing(vxr_sim* sim, int enable);
+int vxr_kernel_time(vxr_sim* sim, const char* kernel, double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,108 @@
+// Internal declarations shared by the runtime and the kernel files of libvexelhip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+
+namespace vxr {
+
+// ------------------------------------------------------------------ device helpers
+
+// The IR's min/max as selects: a signed zero goes the same way on the device and in the
+// CPU restatement, and no NaN rule of v_min/v_max enters.
+__device__ __forceinline__ float mn(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float mx(float a, float b) { return a < b ? b : a; }
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return mn(mx(x, lo), hi); }
+
+// Hardware float atomic add into global memory (global_atomic_add_f32, no return).
+__device__ __forceinline__ void atomic_add(float* p, float v) { unsafeAtomicAdd(p, v); }
+
+constexpr int kWave = 64;
+
+// ------------------------------------------------------------------ launch plumbing
+
+struct Timer;  // event pairs per kernel name, owned by the runtime
+
+// Every launch goes through here so timing can bracket it with events on `stream`.
+struct Launch {
+    hipStream_t stream;
+    Timer* timer;  // null when timing is off
+};
+
+void timed_begin(const Launch& l, const char* kernel);
+void timed_end(const Launch& l, const char* kernel);
+
+// Returns hipGetLastError() after a launch as a status, with a message.
+int check_launch(const char* kernel);
+
+// ------------------------------------------------------------------ shallow water
+
+struct SweArgs {
+    int nx, ny;
+    int edges[4];  // west, east, south, north: 0 wall, 1 open
+    float* out[3];
+    const float* in[3];
+    const int32_t* params;  // g, courant, dx, dy, dry as f32 bits
+    const int32_t* speed_in;
+    int32_t* speed_out;
+    int32_t* speed_clear;
+    const int32_t* budget_in;
+    int32_t* budget_out;
+    int32_t* clamped;
+};
+int swe_step(const Launch& l, const SweArgs& a, int rows);
+
+// ------------------------------------------------------------------ fill
+
+int fill_zero(const Launch& l, void* p, int64_t words);
+
+// ------------------------------------------------------------------ MPM
+
+// Particle fields: 3D x y z u v w m j c00..c22 (17); 2D x y u v m j c00 c01 c10 c11 (10).
+struct Mpm3Args {
+    int nx, ny, nz;
+    int64_t count;     // real particles
+    int64_t capacity;  // allocated (multiple of the chunk), tail particles have m = 0
+    float* p[17];
+    float* g[7];  // gm gmu gmv gmw gu gv gw
+    const int32_t* params;
+};
+int mpm3_clear(const Launch& l, const Mpm3Args& a);
+int mpm3_scatter_atomic(const Launch& l, const Mpm3Args& a);
+int mpm3_scatter_tiled(const Launch& l, const Mpm3Args& a, int run);
+int mpm3_grid(const Launch& l, const Mpm3Args& a);
+int mpm3_advect(const Launch& l, const Mpm3Args& a);
+int mpm3_advect_tiled(const Launch& l, const Mpm3Args& a, int run);
+int mpm3_keys(const Launch& l, const Mpm3Args& a, int32_t* keys, int32_t* index);
+int mpm3_key_bits(int nx, int ny, int nz);
+
+struct Mpm2Args {
+    int nx, ny;
+    int64_t count, capacity;
+    float* p[10];
+    float* g[5];  // gm gmu gmv gu gv
+    const int32_t* params;
+};
+int mpm2_clear(const Launch& l, const Mpm2Args& a);
+int mpm2_scatter_atomic(const Launch& l, const Mpm2Args& a);
+int mpm2_scatter_runs(const Launch& l, const Mpm2Args& a, int run);
+int mpm2_grid(const Launch& l, const Mpm2Args& a);
+int mpm2_advect(const Launch& l, const Mpm2Args& a);
+int mpm2_keys(const Launch& l, const Mpm2Args& a, int32_t* keys, int32_t* index);
+int mpm2_starts(const Launch& l, const int32_t* sorted_keys, int64_t count, int32_t cells, int32_t* starts);
+
+// ------------------------------------------------------------------ sort
+
+// Stable LSD radix sort of (key, index) pairs, keys in [0, 2^bits). Scratch sized by
+// radix_scratch_words. Leaves the sorted pairs in keys/index.
+int64_t radix_scratch_words(int64_t n, int digit_bits);
+int radix_sort(const Launch& l, int32_t* keys, int32_t* index, int64_t n, int bits, int digit_bits,
+               int32_t* scratch);
+// out[f][i] = in[f][index[i]] for `fields` float fields.
+int permute_fields(const Launch& l, float* const* in, float* const* out, int fields, const int32_t* index,
+                   int64_t n);
+
+}  // namespace vxr

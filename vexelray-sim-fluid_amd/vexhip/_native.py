@@ -1,0 +1,168 @@
+"""ctypes binding of libvexelhip.so's C ABI (include/vexel/vxr.h).
+
+Loads the in-tree library and raises when it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvexelhip.so")
+
+OK, EINVAL, ESTATE, EDEVICE, ENOMEM, EUNSUPPORTED = range(6)
+KIND_MPM2, KIND_MPM3, KIND_SWE = 1, 2, 3
+GROUP_STEP, GROUP_SORT = 0, 1
+FLAG_TENSION, FLAG_HEAT, FLAG_CONVECTION, FLAG_RELAX, FLAG_PUMP, FLAG_PLASTIC = (1 << k for k in range(6))
+FLAG_OPEN_WEST, FLAG_OPEN_EAST, FLAG_OPEN_SOUTH, FLAG_OPEN_NORTH = (1 << k for k in range(16, 20))
+
+
+class VxrError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"vxr error {code}: {message}")
+        self.code = code
+
+
+class InvalidArgument(VxrError, ValueError):
+    """VXR_EINVAL: the Java shim's IllegalArgumentException."""
+
+
+class InvalidState(VxrError):
+    """VXR_ESTATE: the Java shim's IllegalStateException."""
+
+
+class Unsupported(VxrError):
+    """VXR_EUNSUPPORTED: a flag outside the built path."""
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run vexelray-sim-fluid_amd/build.py")
+        l = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64, cs = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_char_p
+        sig = {
+            "vxr_last_error": (cs, []),
+            "vxr_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+            "vxr_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(i32), i64, ctypes.c_uint32,
+                                          ctypes.POINTER(vp)]),
+            "vxr_destroy": (ctypes.c_int, [vp]),
+            "vxr_write": (ctypes.c_int, [vp, cs, vp, i64]),
+            "vxr_read": (ctypes.c_int, [vp, cs, vp, i64]),
+            "vxr_length": (ctypes.c_int, [vp, cs, ctypes.POINTER(i64)]),
+            "vxr_run": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
+            "vxr_run_pass": (ctypes.c_int, [vp, cs, ctypes.c_int]),
+            "vxr_swe_dispatch": (ctypes.c_int, [vp, ctypes.c_int]),
+            "vxr_clear": (ctypes.c_int, [vp]),
+            "vxr_finish": (ctypes.c_int, [vp]),
+            "vxr_device_ptr": (ctypes.c_int, [vp, cs, ctypes.POINTER(vp)]),
+            "vxr_stream": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
+            "vxr_set_option": (ctypes.c_int, [vp, cs, i64]),
+            "vxr_get_option": (ctypes.c_int, [vp, cs, ctypes.POINTER(i64)]),
+            "vxr_timing": (ctypes.c_int, [vp, ctypes.c_int]),
+            "vxr_kernel_time": (ctypes.c_int, [vp, cs, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc == OK:
+        return
+    msg = lib().vxr_last_error().decode()
+    cls = {EINVAL: InvalidArgument, ESTATE: InvalidState, EUNSUPPORTED: Unsupported}.get(rc, VxrError)
+    raise cls(rc, msg)
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    check(lib().vxr_device_count(ctypes.byref(n)))
+    return n.value
+
+
+class Sim:
+    """One vxr_sim handle: named resident buffers and the passes over them."""
+
+    def __init__(self, kind: int, dims, particles: int = 0, flags: int = 0):
+        d = (ctypes.c_int32 * 3)(*(list(dims) + [1, 1, 1])[:3])
+        h = ctypes.c_void_p()
+        check(lib().vxr_create(kind, d, int(particles), int(flags), ctypes.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            lib().vxr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def length(self, name: str) -> int:
+        n = ctypes.c_int64()
+        check(lib().vxr_length(self.h, name.encode(), ctypes.byref(n)))
+        return n.value
+
+    def write(self, name: str, values) -> None:
+        a = np.ascontiguousarray(values)
+        if a.dtype not in (np.float32, np.int32, np.uint32):
+            a = np.ascontiguousarray(a, dtype=np.float32 if a.dtype.kind == "f" else np.int32)
+        check(lib().vxr_write(self.h, name.encode(), a.ctypes.data, a.size))
+
+    def read_words(self, name: str, n: int | None = None) -> np.ndarray:
+        n = self.length(name) if n is None else n
+        out = np.empty(n, np.int32)
+        check(lib().vxr_read(self.h, name.encode(), out.ctypes.data, n))
+        return out
+
+    def read(self, name: str) -> np.ndarray:
+        return self.read_words(name).view(np.float32)
+
+    def run(self, group: int, repeat: int = 1) -> None:
+        check(lib().vxr_run(self.h, group, repeat))
+
+    def run_pass(self, name: str, repeat: int = 1) -> None:
+        check(lib().vxr_run_pass(self.h, name.encode(), repeat))
+
+    def swe_dispatch(self, count: int) -> None:
+        check(lib().vxr_swe_dispatch(self.h, count))
+
+    def clear(self) -> None:
+        check(lib().vxr_clear(self.h))
+
+    def finish(self) -> None:
+        check(lib().vxr_finish(self.h))
+
+    def device_ptr(self, name: str) -> int:
+        p = ctypes.c_void_p()
+        check(lib().vxr_device_ptr(self.h, name.encode(), ctypes.byref(p)))
+        return p.value or 0
+
+    def set_option(self, key: str, value: int) -> None:
+        check(lib().vxr_set_option(self.h, key.encode(), int(value)))
+
+    def option(self, key: str) -> int:
+        v = ctypes.c_int64()
+        check(lib().vxr_get_option(self.h, key.encode(), ctypes.byref(v)))
+        return v.value
+
+    def timing(self, enable: bool) -> None:
+        check(lib().vxr_timing(self.h, 1 if enable else 0))
+
+    def kernel_time(self, kernel: str):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        check(lib().vxr_kernel_time(self.h, kernel.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value

@@ -1,0 +1,205 @@
+"""ParticleSimulation (2D) and ParticleSimulation3 (3D): MLS-MPM on resident device buffers --
+gui/ParticleSimulation.java and gui/ParticleSimulation3.java's API over libvexelhip.so.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _native
+
+AFFINE2 = ("c00", "c01", "c10", "c11")
+AFFINE3 = ("c00", "c01", "c02", "c10", "c11", "c12", "c20", "c21", "c22")
+
+
+def params2(dt, gx, gy, bulk, rho0, sigma=0.0) -> np.ndarray:
+    """Flip.params (core/particle/Flip.java:145-211): 28 words, the optional physics off."""
+    if not (dt > 0) or not (bulk >= 0) or not (rho0 > 0):
+        raise ValueError(f"dt > 0, bulk >= 0 and rho0 > 0, got dt {dt}, bulk {bulk}, rho0 {rho0}")
+    w = np.zeros(28, np.int32)
+    w[:6] = np.array([dt, gx, gy, bulk, rho0, sigma], np.float64).astype(np.float32).view(np.int32)
+    return w
+
+
+def params3(dt, gx, gy, gz, bulk, rho0, sigma=0.0) -> np.ndarray:
+    """Flip3.params (core/particle/Flip3.java:89-103)."""
+    if not (dt > 0) or not (bulk >= 0) or not (rho0 > 0):
+        raise ValueError(f"dt > 0, bulk >= 0 and rho0 > 0, got dt {dt}, bulk {bulk}, rho0 {rho0}")
+    return np.array([dt, gx, gy, gz, bulk, rho0, sigma], np.float64).astype(np.float32).view(np.int32)
+
+
+def stable_step(bulk, rho0, speed, courant) -> float:
+    """Flip.stableStep (:214-216): C / (sqrt(B/rho0) + |v|)."""
+    return courant / (np.sqrt(bulk / rho0) + speed)
+
+
+def _flags(tension=False, heat=False, convection=False, relax=False, pump=False, plastic=False) -> int:
+    f = 0
+    for on, bit in ((tension, _native.FLAG_TENSION), (heat, _native.FLAG_HEAT),
+                    (convection, _native.FLAG_CONVECTION), (relax, _native.FLAG_RELAX),
+                    (pump, _native.FLAG_PUMP), (plastic, _native.FLAG_PLASTIC)):
+        if on:
+            f |= bit
+    return f
+
+
+class _Runner:
+    def _write(self, name, values):
+        self.sim.write(name, np.asarray(values, np.float32))
+
+    def _read(self, name):
+        return self.sim.read(name)
+
+    def on_gpu(self) -> bool:
+        return True
+
+    def params(self, words) -> None:
+        self.sim.write("params", np.asarray(words, np.int32))
+
+    def steps(self) -> int:
+        return self._steps
+
+    def finish(self) -> None:
+        self.sim.finish()
+
+    def device_ptr(self, name: str) -> int:
+        return self.sim.device_ptr(name)
+
+    def close(self) -> None:
+        self.sim.close()
+
+    def _advance(self, count: int, sort_every: int | None) -> None:
+        k = 0
+        while k < count:
+            if sort_every and self._steps % sort_every == 0:
+                self.sim.run(_native.GROUP_SORT)
+            run = count - k
+            if sort_every:
+                run = min(run, sort_every - self._steps % sort_every)
+            self.sim.run(_native.GROUP_STEP, run)
+            self._steps += run
+            k += run
+
+
+class ParticleSimulation(_Runner):
+    """gui/ParticleSimulation.java:40-239, over core/particle/FlipStep.java's passes."""
+
+    def __init__(self, nx, ny, particles, tension=False, heat=False, convection=False, relax=False, pump=False):
+        if convection and (not heat or tension):
+            raise ValueError("convection needs heat and does not run with tension")
+        self.sim = _native.Sim(_native.KIND_MPM2, (nx, ny, 1), particles,
+                               _flags(tension, heat, convection, relax, pump))
+        self._nx, self._ny, self._particles = nx, ny, particles
+        self._steps = 0
+
+    def nx(self):
+        return self._nx
+
+    def ny(self):
+        return self._ny
+
+    def particles(self):
+        return self._particles
+
+    def load(self, x, y, u, v, m, words) -> None:
+        """Replaces the particles and the parameters; J = 1, C = 0 (:129-143)."""
+        n = self._particles
+        self._write("x", x)
+        self._write("y", y)
+        self._write("u", u)
+        self._write("v", v)
+        self._write("m", m)
+        self._write("j", np.ones(n, np.float32))
+        for c in AFFINE2:
+            self._write(c, np.zeros(n, np.float32))
+        self.params(words)
+        self._steps = 0
+
+    def advance(self, count: int, sort_every: int) -> None:
+        """count steps, sorting first on every step whose number is a multiple of sort_every (:175-183)."""
+        self._advance(count, sort_every)
+
+    def grid(self):
+        return [self._read("gm"), self._read("gmu"), self._read("gmv")]
+
+    def masses(self):
+        return self._read("m")
+
+    def velocities(self):
+        return [self._read("u"), self._read("v")]
+
+    def positions(self):
+        return [self._read("x"), self._read("y")]
+
+    def compression(self, j=None):
+        if j is None:
+            return self._read("j")
+        self._write("j", j)
+        return None
+
+
+class ParticleSimulation3(_Runner):
+    """gui/ParticleSimulation3.java:110-408, over core/particle/Flip3Step.java's passes."""
+
+    def __init__(self, nx, ny, nz, particles, tension=False, plastic=False):
+        self.sim = _native.Sim(_native.KIND_MPM3, (nx, ny, nz), particles, _flags(tension=tension, plastic=plastic))
+        self._n = (nx, ny, nz)
+        self._particles = particles
+        self._steps = 0
+        self._nominal = np.float32(np.finfo(np.float32).max)
+
+    def nx(self):
+        return self._n[0]
+
+    def ny(self):
+        return self._n[1]
+
+    def nz(self):
+        return self._n[2]
+
+    def particles(self):
+        return self._particles
+
+    def load(self, at, velocity, mass, words, kind=None) -> None:
+        """Replaces the particles (node units) and the parameters; J = 1, C = 0 (:227-275)."""
+        if kind is not None:
+            raise ValueError("a simulation that is not plastic has no kinds of particle")
+        n = self._particles
+        for name, v in zip("xyz", at):
+            self._write(name, v)
+        for name, v in zip("uvw", velocity):
+            self._write(name, v)
+        m = np.asarray(mass, np.float32)
+        self._write("m", m)
+        live = m[m > 0]
+        self._nominal = live.min() if live.size else np.float32(np.finfo(np.float32).max)
+        self._write("j", np.ones(n, np.float32))
+        for c in AFFINE3:
+            self._write(c, np.zeros(n, np.float32))
+        self.params(words)
+        self._steps = 0
+
+    def advance(self, count: int, sort_every: int | None = None) -> None:
+        """count steps (:291-296); with sort_every, a sort by cell first on every step whose number
+        is a multiple of it (the 2D runner's cadence, which the 3D reference step lacks)."""
+        self._advance(count, sort_every)
+
+    def sort(self) -> None:
+        self.sim.run(_native.GROUP_SORT)
+
+    def grid(self):
+        return [self._read(g) for g in ("gm", "gmu", "gmv", "gmw")]
+
+    def compression(self):
+        """J of the particles holding at least half the nominal mass (:312-333)."""
+        j, m = self._read("j"), self._read("m")
+        keep = m >= np.float32(0.5) * self._nominal
+        return j if keep.all() else j[keep]
+
+    def velocities(self):
+        return [self._read(v) for v in "uvw"]
+
+    def positions(self):
+        return [self._read(v) for v in "xyz"]
+
+    def active(self) -> int:
+        return int(np.count_nonzero(self._read("m") > 0))
