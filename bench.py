@@ -25,8 +25,8 @@ SORT_EVERY = 10
 # Algorithmic bytes per unit (particle, node, element) of each kernel: what the pass must move
 # through HBM at least, as derived in the design notes.
 BYTES = {
-    "p2g3_tiled": ("particle", 68), "p2g3_atomic": ("particle", 68),
-    "g2p3_tiled": ("particle", 84), "g2p3": ("particle", 84),
+    "p2g3_seg": ("particle", 68), "p2g3_atomic": ("particle", 68),
+    "g2p3_fast": ("particle", 84), "g2p3_tiled": ("particle", 84), "g2p3": ("particle", 84),
     "grid3": ("node", 28), "mpm3_clear": ("node", 16),
     "keys3": ("particle", 20), "radix_hist": ("element", 4), "radix_scatter": ("element", 16),
     "radix_scan": ("none", 0), "permute": ("particle", 140),
@@ -69,8 +69,8 @@ def bench_mpm3(steps, warmup, p2g, g2p, run):
     sim.advance(steps, sort_every=SORT_EVERY)
     sim.finish()
     t1 = time.perf_counter()
-    names = ["mpm3_clear", "p2g3_tiled", "p2g3_atomic", "grid3", "g2p3_tiled", "g2p3", "keys3", "radix_hist",
-             "radix_scan", "radix_scatter", "permute"]
+    names = ["mpm3_clear", "p2g3_seg", "p2g3_atomic", "grid3", "g2p3_fast", "g2p3_tiled", "g2p3", "keys3",
+             "radix_hist", "radix_scan", "radix_scatter", "permute"]
     units = {"particle": count, "node": 256 ** 3, "element": count}
     table = kernel_table(sim.sim, names, units)
     sim.sim.timing(False)
@@ -123,7 +123,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--p2g", type=int, default=1)
-    ap.add_argument("--g2p", type=int, default=1)
+    ap.add_argument("--g2p", type=int, default=2)
     ap.add_argument("--run", type=int, default=4)
     ap.add_argument("--hll-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
@@ -156,7 +156,7 @@ def main():
         "config": {"workload": "3D MLS-MPM/APIC substep, 8,388,608 particles (two seeded cubes, 8 per cell), "
                                "256^3 nodes, dt=5e-5, sorted-segmented P2G, sort every 10 substeps",
                    "particles": count, "grid": [256, 256, 256], "sort_every": SORT_EVERY,
-                   "p2g": ["atomic", "sorted-segmented"][args.p2g], "g2p": ["gather", "lds-tile"][args.g2p],
+                   "p2g": ["atomic", "sorted-segmented"][args.p2g], "g2p": ["gather", "lds-tile", "lds-tile-contracted"][args.g2p],
                    "run": args.run},
         "roofline": roofline, "kernels": table, "grid_mass_drift": drift,
     }

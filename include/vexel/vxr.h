@@ -116,9 +116,13 @@ int vxr_stream(vxr_sim* sim, void** stream);
 
 /* Schedule knobs (no reference counterpart; the autotuner sets them):
  *   "p2g"        MPM: 0 = direct float atomics (the reference's Flip3.scatter schedule),
- *                1 = sorted-segmented: per-thread runs, an LDS node tile, one atomic per node (default)
- *   "g2p"        MPM: 0 = per-particle gathers, 1 = LDS node tile (default)
- *   "run"        MPM: particles a thread folds per run in the segmented passes (1..8, default 4)
+ *                1 = sorted-segmented (default): per-thread runs of consecutive particles, an FP64
+ *                LDS node tile per block of 256 * run particles, one atomic per node and field
+ *   "g2p"        MPM3: 0 = per-particle gathers and 1 = an LDS node tile, both in the reference's
+ *                arithmetic order (the CPU restatement's bits); 2 = the LDS tile with the sums
+ *                contracted axis by axis in fused multiply-adds (default, within a few ulps)
+ *   "run"        MPM: consecutive particles a thread takes in the segmented passes: 1, 2 or 4
+ *                (default 4)
  *   "radix_bits" sort digit width (4..8, default 8)
  *   "rows"       SWE: rows a wave marches per tile (4..256, default 32) */
 int vxr_set_option(vxr_sim* sim, const char* key, int64_t value);

@@ -11,8 +11,9 @@ from vexhip import _native, particles, scenes
 
 pytestmark = pytest.mark.gpu
 
-SCHEDULES = [dict(p2g=0, g2p=0), dict(p2g=1, g2p=1, run=4), dict(p2g=1, g2p=1, run=1), dict(p2g=1, g2p=1, run=8)]
-IDS = ["atomic", "tiled-run4", "tiled-run1", "tiled-run8"]
+SCHEDULES = [dict(p2g=0, g2p=0), dict(p2g=1, g2p=1, run=4), dict(p2g=1, g2p=2, run=4), dict(p2g=1, g2p=2, run=1),
+             dict(p2g=1, g2p=2, run=2)]
+IDS = ["atomic", "seg-exact", "seg-fast", "seg-run1", "seg-run2"]
 
 
 @pytest.mark.parametrize("opts", SCHEDULES, ids=IDS)
@@ -38,7 +39,7 @@ def test_walls_hold_3d(gpu):
     cases.case_walls_hold_3d(maker3())
 
 
-@pytest.mark.parametrize("opts", SCHEDULES[:2], ids=IDS[:2])
+@pytest.mark.parametrize("opts", SCHEDULES[:3], ids=IDS[:3])
 def test_dam_break_slab_3d(gpu, opts):
     r = cases.case_dam_break_slab_3d(maker3(**opts))
     print(f"[flip3] GPU {opts}: moved {r[0]:.2f} of {r[1]:.2f}, J {r[2]:.3f}..{r[3]:.3f}, "
@@ -50,13 +51,13 @@ def test_lone_particle_falls_2d(gpu, opts):
     cases.case_lone_particle_falls_2d(maker2(**opts))
 
 
-@pytest.mark.parametrize("opts", [dict(p2g=0), dict(p2g=1, run=4), dict(p2g=1, run=8)], ids=["atomic", "run4", "run8"])
+@pytest.mark.parametrize("opts", [dict(p2g=0), dict(p2g=1, run=4), dict(p2g=1, run=2)], ids=["atomic", "run4", "run2"])
 def test_scatter_matches_fp64_splat_2d(gpu, opts):
     cases.case_scatter_matches_fp64_splat_2d(maker2(**opts))
     cases.case_scatter_matches_fp64_splat_2d(maker2(**opts), order_seed=42)
 
 
-@pytest.mark.parametrize("opts", [dict(p2g=0), dict(p2g=1, run=4), dict(p2g=1, run=8)], ids=["atomic", "run4", "run8"])
+@pytest.mark.parametrize("opts", [dict(p2g=0), dict(p2g=1, run=4), dict(p2g=1, run=2)], ids=["atomic", "run4", "run2"])
 def test_broken_runs_2d(gpu, opts):
     cases.case_broken_runs_2d(maker2(**opts))
 
@@ -131,10 +132,12 @@ def test_grid_and_advect_bit_exact_3d(gpu):
                                           err_msg=f"g2p={g2p} field {name}")
 
 
-@pytest.mark.parametrize("opts", SCHEDULES[:2], ids=IDS[:2])
-def test_steps_track_the_oracle_3d(gpu, opts):
+@pytest.mark.parametrize("sort", [False, True], ids=["unsorted", "sorted"])
+@pytest.mark.parametrize("opts", SCHEDULES[:3], ids=IDS[:3])
+def test_steps_track_the_oracle_3d(gpu, opts, sort):
     """Twenty whole steps of a falling, compressing block: positions within 1e-5 relative of the
-    oracle's, grid mass equal to particle mass to 1e-6."""
+    oracle's, grid mass equal to particle mass to 1e-6. Sorted, the segmented P2G and the tiled
+    G2P take one brick of particles a block (the ranges the sort leaves)."""
     n = 32
     x, y, z = scenes.jittered_block3((3, 3, 3), (10, 14, 9), 8, scenes.JavaRandomFloats(5))
     count = len(x)
@@ -145,7 +148,10 @@ def test_steps_track_the_oracle_3d(gpu, opts):
     for name, v in (("x", x), ("y", y), ("z", z), ("m", m), ("j", np.ones(count, np.float32))):
         rig.write(name, v)
     rig.params(params)
-    for _ in range(20):
+    for k in range(20):
+        if sort and k % 10 == 0:
+            ref.sort()
+            rig.run(["sort"])
         ref.step()
         rig.run("step")
     for name in ("x", "y", "z"):
@@ -153,6 +159,26 @@ def test_steps_track_the_oracle_3d(gpu, opts):
     np.testing.assert_allclose(rig.read("j"), ref.p["j"], rtol=1e-4, atol=1e-5)
     mass = float(np.sum(m, dtype=np.float64))
     assert abs(float(np.sum(rig.read("gm"), dtype=np.float64)) - mass) <= 1e-6 * mass
+
+
+def test_fast_advect_close_3d(gpu):
+    """The contracted G2P (g2p=2) from the oracle's own grid: every particle field within
+    1e-5 max(1, |e|) of the reference-order result."""
+    n, count = 32, 30_000
+    f = _state3(n, count, 9)
+    ref = oracle.Mpm3(n, n, n, f, oracle.mpm3_params(2e-3, 0.5, -50, 0.25, 400.0, 1))
+    ref.clear()
+    ref.scatter()
+    ref.grid()
+    rig = maker3(p2g=1, g2p=2)(n, n, n, count)
+    cases.load3(rig, f)
+    rig.params(ref.params)
+    for g in oracle.GRID3:
+        rig.write(g, ref.g[g])
+    rig.run(["advect"])
+    ref.advect()
+    for name in oracle.FIELDS3:
+        _close(rig.read(name), ref.p[name], name)
 
 
 # ------------------------------------------------------------------ sort

@@ -72,10 +72,11 @@ struct Mpm3Args {
 };
 int mpm3_clear(const Launch& l, const Mpm3Args& a);
 int mpm3_scatter_atomic(const Launch& l, const Mpm3Args& a);
+// Chunks of 256 * run particles, run in {1, 2, 4}.
 int mpm3_scatter_tiled(const Launch& l, const Mpm3Args& a, int run);
 int mpm3_grid(const Launch& l, const Mpm3Args& a);
 int mpm3_advect(const Launch& l, const Mpm3Args& a);
-int mpm3_advect_tiled(const Launch& l, const Mpm3Args& a, int run);
+int mpm3_advect_tiled(const Launch& l, const Mpm3Args& a, int run, bool fast);
 int mpm3_keys(const Launch& l, const Mpm3Args& a, int32_t* keys, int32_t* index);
 int mpm3_key_bits(int nx, int ny, int nz);
 

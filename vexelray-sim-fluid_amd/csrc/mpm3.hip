@@ -27,7 +27,6 @@ constexpr float kWall = 1.0f;                                // Flip.WALL
 constexpr float kEmpty = 1e-9f;                              // Flip3.EMPTY
 constexpr float kJBound = 10.0f;                             // Flip3.J_BOUND
 constexpr int kBlock = 256;
-constexpr int kTileNodes = 2560;  // LDS tile: 4 x 2560 x 4 B = 40 KB for P2G, 30 KB for G2P
 
 // One axis of Flip3.Stencil3.of, Flip3.java:670-676 and :682-688.
 struct Axis {
@@ -95,6 +94,7 @@ __device__ __forceinline__ void load_particle(const Mpm3Args& a, int64_t q, floa
 
 // ------------------------------------------------------------------ clear
 
+
 __global__ void clear4_kernel(float* __restrict__ g0, float* __restrict__ g1, float* __restrict__ g2,
                               float* __restrict__ g3, int64_t n4) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -109,6 +109,28 @@ __global__ void clear4_kernel(float* __restrict__ g0, float* __restrict__ g1, fl
 
 // ------------------------------------------------------------------ P2G, the reference schedule
 
+// One particle's 108 amounts straight into the grid by float atomics, in the reference's
+// arithmetic (Flip3.java:225-243).
+__device__ __forceinline__ void atomic_particle(const Mpm3Args& a, const Axis (&ax)[3], float m, const float (&vel)[3],
+                                                const float (&c)[9], float s) {
+    const int64_t base = ((int64_t)ax[2].cell * a.ny + ax[1].cell) * a.nx + ax[0].cell;
+    const int64_t plane = (int64_t)a.nx * a.ny;
+#pragma unroll
+    for (int k = 0; k < 27; ++k) {
+        const float wgt = ax[0].w[k % 3] * (ax[1].w[(k / 3) % 3] * ax[2].w[k / 9]);
+        const float d[3] = {(float)(k % 3) - ax[0].off, (float)((k / 3) % 3) - ax[1].off, (float)(k / 9) - ax[2].off};
+        const float wm = wgt * m;
+        const int64_t node = base + (k / 9) * plane + ((k / 3) % 3) * a.nx + k % 3;
+        atomic_add(a.g[0] + node, wm);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const float at3 = vel[i] + (c[3 * i] * d[0] + (c[3 * i + 1] * d[1] + c[3 * i + 2] * d[2]));
+            const float push = s * d[i];
+            atomic_add(a.g[1 + i] + node, wm * at3 + wgt * push);
+        }
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void p2g3_atomic_kernel(Mpm3Args a) {
     const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (q >= a.count) return;
@@ -116,23 +138,7 @@ __global__ __launch_bounds__(kBlock) void p2g3_atomic_kernel(Mpm3Args a) {
     const float dt = bits_f(a.params[DT]), bulk = bits_f(a.params[BULK]), rho0 = bits_f(a.params[RHO0]);
     Particle p;
     load_particle(a, q, dt, bulk, rho0, p);
-    const int64_t base = ((int64_t)p.ax[2].cell * a.ny + p.ax[1].cell) * a.nx + p.ax[0].cell;
-    const int64_t plane = (int64_t)a.nx * a.ny;
-#pragma unroll
-    for (int k = 0; k < 27; ++k) {
-        const float wgt = p.ax[0].w[k % 3] * (p.ax[1].w[(k / 3) % 3] * p.ax[2].w[k / 9]);
-        const float d[3] = {(float)(k % 3) - p.ax[0].off, (float)((k / 3) % 3) - p.ax[1].off,
-                            (float)(k / 9) - p.ax[2].off};
-        const float wm = wgt * p.m;
-        const int64_t node = base + (k / 9) * plane + ((k / 3) % 3) * a.nx + k % 3;
-        atomic_add(a.g[0] + node, wm);
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            const float at3 = p.vel[ax] + (p.c[3 * ax] * d[0] + (p.c[3 * ax + 1] * d[1] + p.c[3 * ax + 2] * d[2]));
-            const float push = p.s * d[ax];
-            atomic_add(a.g[1 + ax] + node, wm * at3 + wgt * push);
-        }
-    }
+    atomic_particle(a, p.ax, p.m, p.vel, p.c, p.s);
 }
 
 // ------------------------------------------------------------------ block bounding box
@@ -143,21 +149,9 @@ struct Box {
     bool empty;
 };
 
-// The box of base cells of the live particles q in [first, first + n), over the block.
-// Every thread returns the same box. Uses red[24] of LDS.
-__device__ Box block_box(const Mpm3Args& a, int64_t first, int n, int stride, int* red) {
-    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-    const int sizes[3] = {a.nx, a.ny, a.nz};
-    for (int r = 0; r < n; ++r) {
-        const int64_t q = first + (int64_t)r * stride;
-        if (q >= a.capacity || !(a.p[6][q] > 0.0f)) continue;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            const int c = base_cell(a.p[i][q], sizes[i]);
-            lo[i] = min(lo[i], c);
-            hi[i] = max(hi[i], c);
-        }
-    }
+// The block-wide box of base cells from each thread's own lo/hi (empty: lo > hi). Every
+// thread returns the same box; nodes = INT_MAX when the box's nodes exceed `cap`. Uses red[24].
+__device__ Box reduce_box(int (&lo)[3], int (&hi)[3], int* red, int cap) {
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) {
 #pragma unroll
@@ -193,81 +187,306 @@ __device__ Box block_box(const Mpm3Args& a, int64_t first, int n, int stride, in
     b.ey = b.empty ? 0 : b.hi[1] - b.lo[1] + 3;
     b.ez = b.empty ? 0 : b.hi[2] - b.lo[2] + 3;
     const int64_t nodes = (int64_t)b.ex * b.ey * b.ez;
-    b.nodes = b.empty ? 0 : (nodes > kTileNodes ? INT_MAX : (int)nodes);
+    b.nodes = b.empty ? 0 : (nodes > cap ? INT_MAX : (int)nodes);
     return b;
+}
+
+// R consecutive floats from i (i a multiple of R, the arrays 256-byte aligned): one load.
+template <int R>
+__device__ __forceinline__ void load_run(const float* __restrict__ p, int64_t i, float (&v)[R]) {
+    if constexpr (R == 4) {
+        const float4 t = *reinterpret_cast<const float4*>(p + i);
+        v[0] = t.x;
+        v[1] = t.y;
+        v[2] = t.z;
+        v[3] = t.w;
+    } else if constexpr (R == 2) {
+        const float2 t = *reinterpret_cast<const float2*>(p + i);
+        v[0] = t.x;
+        v[1] = t.y;
+    } else {
+        v[0] = p[i];
+    }
 }
 
 // ------------------------------------------------------------------ P2G, sorted-segmented
 
-__global__ __launch_bounds__(kBlock) void p2g3_tiled_kernel(Mpm3Args a, int run) {
-    __shared__ float tile[4 * kTileNodes];
-    __shared__ int red[6 * (kBlock / kWave)];
-    const int64_t first = (int64_t)blockIdx.x * kBlock * run + (int64_t)threadIdx.x * run;
-    const Box box = block_box(a, first, run, 1, red);
-    if (box.empty) return;  // the same for every thread of the block
-    const bool use_tile = box.nodes <= kTileNodes;
-    if (use_tile) {
-        for (int t = threadIdx.x; t < 4 * box.nodes; t += kBlock) tile[t] = 0.0f;
+// Node (i, j, l) of a particle's stencil, k = i + 3j + 9l, receives mass w m and momentum
+// w (E + B n) with n = (i, j, l), B_ab = m C_ab + s [a = b], E_a = m v_a - B_a . off: the
+// reference's (w m)(v + C d) + w s d (Flip3.java:225-243) with d = n - off, regrouped so the
+// per-node work is one add and one fused multiply-add per field. The weights are the
+// reference's products exactly; the regrouping moves each amount by a few ulps.
+__device__ __forceinline__ void p2g_fold(const Axis (&ax)[3], float m, const float (&vel)[3], const float (&c)[9],
+                                         float s, float (&acc)[108]) {
+    float B[3][3], E[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) B[a][b] = a == b ? fmaf(m, c[3 * a + b], s) : m * c[3 * a + b];
+        E[a] = fmaf(m, vel[a], -fmaf(B[a][0], ax[0].off, fmaf(B[a][1], ax[1].off, B[a][2] * ax[2].off)));
     }
-    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const float wyz = ax[1].w[j] * ax[2].w[l];
+            float T[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) T[a] = E[a] + (float)j * B[a][1] + (float)l * B[a][2];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const int k = i + 3 * j + 9 * l;
+                const float wgt = ax[0].w[i] * wyz;
+                acc[4 * k] = fmaf(wgt, m, acc[4 * k]);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) acc[4 * k + 1 + a] = fmaf(wgt, T[a] + (float)i * B[a][0], acc[4 * k + 1 + a]);
+            }
+        }
+    }
+}
+
+// Lanes of one wave exchange values through LDS below without atomics. The hardware keeps a
+// wave's LDS operations in program order; this keeps the compiler from moving, merging or
+// forwarding LDS accesses across the point (a lane must re-read what other lanes wrote).
+__device__ __forceinline__ void lds_order() { asm volatile("" ::: "memory"); }
+
+// The wave's box of base cells from each lane's lo/hi (empty: lo > hi); the same in every lane.
+__device__ __forceinline__ Box wave_box(int (&lo)[3], int (&hi)[3], int cap) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            lo[i] = min(lo[i], __shfl_xor(lo[i], off));
+            hi[i] = max(hi[i], __shfl_xor(hi[i], off));
+        }
+    }
+    Box b;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        b.lo[i] = lo[i];
+        b.hi[i] = hi[i];
+    }
+    b.empty = b.lo[0] > b.hi[0];
+    b.ex = b.empty ? 0 : b.hi[0] - b.lo[0] + 3;
+    b.ey = b.empty ? 0 : b.hi[1] - b.lo[1] + 3;
+    b.ez = b.empty ? 0 : b.hi[2] - b.lo[2] + 3;
+    const int64_t nodes = (int64_t)b.ex * b.ey * b.ez;
+    b.nodes = b.empty ? 0 : (nodes > cap ? INT_MAX : (int)nodes);
+    return b;
+}
+
+constexpr int kWaveTile = 1024;  // nodes of a wave's LDS tile: 1024 x (16 B + 4 B claim) = 20 KB
+constexpr int kMaxPasses = 16;   // slabs a wave's box may take through the tile
+
+// A slab of a wave's box: base cells [slo, shi] of the split axis; its nodes lo .. lo + dims - 1.
+struct Slab {
+    int lo[3], dims[3], slo, shi, nodes;  // slo..shi: the split axis's base cells
+};
+
+// v[i] without indexing a local array at run time (that would put it in scratch memory)
+__device__ __forceinline__ int pick(const int (&v)[3], int i) { return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]); }
+
+__device__ __forceinline__ Slab slab(const Box& box, const int (&nodes_of)[3], int split, int width, int pass,
+                                     bool sliced) {
+    Slab sl;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        sl.lo[i] = box.lo[i];
+        sl.dims[i] = nodes_of[i];
+    }
+    sl.slo = pick(box.lo, split);
+    sl.shi = pick(box.hi, split);
+    if (sliced) {
+        sl.slo += pass * width;
+        sl.shi = min(sl.slo + width - 1, sl.shi);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            if (i == split) {
+                sl.lo[i] = sl.slo;
+                sl.dims[i] = sl.shi - sl.slo + 3;
+            }
+        }
+    }
+    sl.nodes = sl.dims[0] * sl.dims[1] * sl.dims[2];
+    return sl;
+}
+
+__device__ __forceinline__ void clear_tile(float4* tile, const Slab& sl) {
+    lds_order();  // whatever the wave read of the tile before is done with
+    for (int t = threadIdx.x; t < sl.nodes; t += kWave) tile[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// Called by the whole wave; `mine` says whether this lane has a run at tile node `run` to flush.
+// Lanes flushing the same node take turns: each writes its lane id into claim[run] and the lane
+// it reads back wins the round, so within a round no two lanes touch one node at one offset.
+__device__ __forceinline__ void flush_run(float4* tile, int* claim, const Slab& sl, int run, bool mine,
+                                          float (&acc)[108]) {
+    const int lane = threadIdx.x;
+    const int b = mine ? run : 0;  // lanes with nothing to flush never index with their run
+    const int ex = sl.dims[0], ey = sl.dims[1];
+    bool pending = mine;
+    while (__ballot(pending)) {
+        if (pending) claim[b] = lane;
+        lds_order();
+        const bool won = pending && claim[b] == lane;
+        if (won) {
+#pragma unroll
+            for (int k = 0; k < 27; ++k) {
+                const int node = b + ((k / 9) * ey + (k / 3) % 3) * ex + k % 3;
+                float4 t = tile[node];
+                t.x += acc[4 * k];
+                t.y += acc[4 * k + 1];
+                t.z += acc[4 * k + 2];
+                t.w += acc[4 * k + 3];
+                tile[node] = t;
+                lds_order();
+            }
+        }
+        pending = pending && !won;
+    }
+    if (mine) {
+#pragma unroll
+        for (int i = 0; i < 108; ++i) acc[i] = 0.0f;
+    }
+}
+
+// One particle into this lane's run: its stencil, a flush when its base changes, its amounts.
+__device__ __forceinline__ void take(const Mpm3Args& a, float4* tile, int* claim, const Slab& sl, int split, float x,
+                                     float y, float z, float m, float j, const float (&vel)[3], const float (&c)[9],
+                                     int& run_base, float (&acc)[108]) {
+    Axis ax[3] = {axis(x, a.nx), axis(y, a.ny), axis(z, a.nz)};
+    const int cs = split == 0 ? ax[0].cell : (split == 1 ? ax[1].cell : ax[2].cell);
+    const bool live = m > 0.0f && cs >= sl.slo && cs <= sl.shi;
+    const int base = ((ax[2].cell - sl.lo[2]) * sl.dims[1] + (ax[1].cell - sl.lo[1])) * sl.dims[0] +
+                     (ax[0].cell - sl.lo[0]);
+    const bool change = live && base != run_base;
+    flush_run(tile, claim, sl, run_base, change && run_base >= 0, acc);
+    if (change) run_base = base;
+    if (live) {
+        const float dt = bits_f(a.params[DT]), bulk = bits_f(a.params[BULK]), rho0 = bits_f(a.params[RHO0]);
+        const float s = (4.0f * (dt * (m / rho0))) * (bulk * (1.0f - j));
+        p2g_fold(ax, m, vel, c, s, acc);
+    }
+}
+
+// The tile's nodes into the grid: one global float atomic per touched node and field.
+__device__ __forceinline__ void leave_tile(const Mpm3Args& a, const float4* tile, const Slab& sl) {
+    lds_order();
+    const int ex = sl.dims[0], ey = sl.dims[1];
+    for (int t = threadIdx.x; t < sl.nodes; t += kWave) {
+        const float4 v = tile[t];
+        if (v.x == 0.0f) continue;  // a node no live particle weighs on holds nothing
+        const int lx = t % ex, ly = (t / ex) % ey, lz = t / (ex * ey);
+        const int64_t node = ((int64_t)(sl.lo[2] + lz) * a.ny + (sl.lo[1] + ly)) * a.nx + (sl.lo[0] + lx);
+        atomic_add(a.g[0] + node, v.x);
+        atomic_add(a.g[1] + node, v.y);
+        atomic_add(a.g[2] + node, v.z);
+        atomic_add(a.g[3] + node, v.w);
+    }
+}
+
+// A wave takes a chunk of 64 * R particles, lane l the R consecutive ones from l * R, each field
+// read with one R-wide load so the wave's loads are whole cache lines, all in flight at once.
+// Runs of particles sharing a stencil base fold into 108 register sums. A run leaves through the
+// wave's own node tile in LDS, a float4 (m, mu, mv, mw) per node, by plain read-modify-write
+// (LDS float atomics run at a small fraction of LDS bandwidth on gfx950): lanes flushing the same
+// base take turns -- each writes its lane id into claim[base], the lane it reads back wins this
+// round -- so within a round no two lanes touch one node at one stencil offset, and the rounds,
+// the offsets and the wave's LDS operations are in program order. The tile then leaves with one
+// global float atomic per touched node and field.
+template <int R>
+__global__ __launch_bounds__(kWave) void p2g3_wave_kernel(Mpm3Args a) {
+    __shared__ float4 tile[kWaveTile];
+    __shared__ int claim[kWaveTile];
+    const int lane = threadIdx.x;
+    const int64_t first = ((int64_t)blockIdx.x * kWave + lane) * R;  // first + R <= capacity
+    float f[17][R];
+#pragma unroll
+    for (int k = 0; k < 17; ++k) load_run<R>(a.p[k], first, f[k]);
+    const int size[3] = {a.nx, a.ny, a.nz};
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (!(f[6][r] > 0.0f)) continue;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int c = base_cell(f[i][r], size[i]);
+            lo[i] = min(lo[i], c);
+            hi[i] = max(hi[i], c);
+        }
+    }
+    const Box box = wave_box(lo, hi, kWaveTile);
+    if (box.empty) return;
+    // A box larger than the tile goes through it in slabs across its longest axis, each slab
+    // taking the particles whose base cell lies in it; past kMaxPasses slabs the runs go to
+    // global memory directly.
+    int cells[3], nodes_of[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        cells[i] = box.hi[i] - box.lo[i] + 1;
+        nodes_of[i] = cells[i] + 2;
+    }
+    const int split = cells[0] >= cells[1] && cells[0] >= cells[2] ? 0 : (cells[1] >= cells[2] ? 1 : 2);
+    const int cross = nodes_of[0] * nodes_of[1] * nodes_of[2] / pick(nodes_of, split);
+    const int width = kWaveTile / cross - 2;  // cells of the split axis a slab takes
+    const int passes = width >= 1 ? (pick(cells, split) + width - 1) / width : INT_MAX;
+    const bool use_tile = passes <= kMaxPasses;
 
     const float dt = bits_f(a.params[DT]), bulk = bits_f(a.params[BULK]), rho0 = bits_f(a.params[RHO0]);
-    const int64_t plane = (int64_t)a.nx * a.ny;
+    if (!use_tile) {
+        // particles spread too thin for the tile (unsorted input): the reference's direct atomics
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const float m = f[6][r];
+            if (!(m > 0.0f)) continue;
+            Axis ax[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) ax[i] = axis(f[i][r], size[i]);
+            const float vel[3] = {f[3][r], f[4][r], f[5][r]};
+            float c[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) c[i] = f[8 + i][r];
+            const float s = (4.0f * (dt * (m / rho0))) * (bulk * (1.0f - f[7][r]));
+            atomic_particle(a, ax, m, vel, c, s);
+        }
+        return;
+    }
     float acc[108];
 #pragma unroll
     for (int i = 0; i < 108; ++i) acc[i] = 0.0f;
-    int64_t run_base = -1;
-
-    auto flush = [&](int64_t base) {
-        if (use_tile) {
-            const int b = (int)base;
+    if (passes == 1) {
+        // the common case: one slab, the particles from the registers they were loaded into
+        const Slab sl = slab(box, nodes_of, split, width, 0, false);
+        clear_tile(tile, sl);
+        int run_base = -1;
 #pragma unroll
-            for (int k = 0; k < 27; ++k) {
-                const int node = b + ((k / 9) * box.ey + (k / 3) % 3) * box.ex + k % 3;
+        for (int r = 0; r < R; ++r) {
+            const float vel[3] = {f[3][r], f[4][r], f[5][r]};
+            float c[9];
 #pragma unroll
-                for (int f = 0; f < 4; ++f) atomicAdd(&tile[f * box.nodes + node], acc[4 * k + f]);
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < 27; ++k) {
-                const int64_t node = base + (k / 9) * plane + ((k / 3) % 3) * a.nx + k % 3;
-#pragma unroll
-                for (int f = 0; f < 4; ++f) atomic_add(a.g[f] + node, acc[4 * k + f]);
-            }
+            for (int i = 0; i < 9; ++i) c[i] = f[8 + i][r];
+            take(a, tile, claim, sl, split, f[0][r], f[1][r], f[2][r], f[6][r], f[7][r], vel, c, run_base, acc);
         }
-#pragma unroll
-        for (int i = 0; i < 108; ++i) acc[i] = 0.0f;
-    };
-
-    for (int r = 0; r < run; ++r) {
-        const int64_t q = first + r;
-        if (q >= a.capacity || !(a.p[6][q] > 0.0f)) continue;
-        Particle p;
-        load_particle(a, q, dt, bulk, rho0, p);
-        const int64_t base = use_tile ? (int64_t)(((p.ax[2].cell - box.lo[2]) * box.ey + (p.ax[1].cell - box.lo[1])) *
-                                                      box.ex +
-                                                  (p.ax[0].cell - box.lo[0]))
-                                      : ((int64_t)p.ax[2].cell * a.ny + p.ax[1].cell) * a.nx + p.ax[0].cell;
-        if (base != run_base) {
-            if (run_base >= 0) flush(run_base);
-            run_base = base;
-        }
-        p2g_amounts(p.ax, p.m, p.vel, p.c, p.s, acc);
+        flush_run(tile, claim, sl, run_base, run_base >= 0, acc);
+        leave_tile(a, tile, sl);
+        return;
     }
-    if (run_base >= 0) flush(run_base);
-
-    if (use_tile) {
-        __syncthreads();
-        for (int t = threadIdx.x; t < box.nodes; t += kBlock) {
-            const float m = tile[t];
-            if (m == 0.0f) continue;  // a node no live particle weighs on holds nothing
-            const int lx = t % box.ex, ly = (t / box.ex) % box.ey, lz = t / (box.ex * box.ey);
-            const int64_t node = ((int64_t)(box.lo[2] + lz) * a.ny + (box.lo[1] + ly)) * a.nx + (box.lo[0] + lx);
-            atomic_add(a.g[0] + node, m);
-            atomic_add(a.g[1] + node, tile[box.nodes + t]);
-            atomic_add(a.g[2] + node, tile[2 * box.nodes + t]);
-            atomic_add(a.g[3] + node, tile[3 * box.nodes + t]);
+    // several slabs: each pass re-reads its particles (from cache), nothing held across passes
+    for (int pass = 0; pass < passes; ++pass) {
+        const Slab sl = slab(box, nodes_of, split, width, pass, true);
+        clear_tile(tile, sl);
+        int run_base = -1;
+        for (int r = 0; r < R; ++r) {
+            const int64_t q = first + r;
+            const float vel[3] = {a.p[3][q], a.p[4][q], a.p[5][q]};
+            float c[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) c[i] = a.p[8 + i][q];
+            take(a, tile, claim, sl, split, a.p[0][q], a.p[1][q], a.p[2][q], a.p[6][q], a.p[7][q], vel, c, run_base,
+                 acc);
         }
+        flush_run(tile, claim, sl, run_base, run_base >= 0, acc);
+        leave_tile(a, tile, sl);
     }
 }
 
@@ -310,10 +529,31 @@ __device__ __forceinline__ void keep_inside(float& at, float& vel, int size) {
     }
 }
 
-// One particle's G2P, node velocities read through `node_vel(k, axis)`.
+// The end of Flip3.advect (:361-377): C = 4 sum, the move, the walls, J.
+__device__ __forceinline__ void g2p_finish(const Mpm3Args& a, int64_t q, const float (&pos0)[3], float (&vel)[3],
+                                           const float (&sum)[9], float dt) {
+    float c[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) c[i] = 4.0f * sum[i];
+    const int size[3] = {a.nx, a.ny, a.nz};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float pos = pos0[i] + dt * vel[i];
+        keep_inside(pos, vel[i], size[i]);
+        a.p[i][q] = pos;
+        a.p[3 + i][q] = vel[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) a.p[8 + i][q] = c[i];
+    const float j = a.p[7][q] * (1.0f + dt * (c[0] + (c[4] + c[8])));
+    a.p[7][q] = clampf(j, 1.0f / kJBound, kJBound);
+}
+
+// One particle's G2P in the reference's order (Flip3.java:349-360), node velocities read
+// through `node_vel(k, axis)`: the CPU restatement's bits.
 template <typename NodeVel>
-__device__ __forceinline__ void g2p_one(const Mpm3Args& a, int64_t q, const Axis (&ax)[3], float dt,
-                                        NodeVel node_vel) {
+__device__ __forceinline__ void g2p_one(const Mpm3Args& a, int64_t q, const float (&pos0)[3], const Axis (&ax)[3],
+                                        float dt, NodeVel node_vel) {
     float vel[3] = {0.0f, 0.0f, 0.0f};
     float sum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -328,43 +568,96 @@ __device__ __forceinline__ void g2p_one(const Mpm3Args& a, int64_t q, const Axis
             for (int e = 0; e < 3; ++e) sum[3 * i + e] = sum[3 * i + e] + na * d[e];
         }
     }
-    float c[9];
+    g2p_finish(a, q, pos0, vel, sum, dt);
+}
+
+// The same sums contracted one axis at a time with fused multiply-adds: x first (the weight
+// and the weight times the offset), then y, then z -- a third of the operations, each sum
+// within a few ulps of the reference's.
+template <typename NodeVel>
+__device__ __forceinline__ void g2p_fast(const Mpm3Args& a, int64_t q, const float (&pos0)[3], const Axis (&ax)[3],
+                                         float dt, NodeVel node_vel) {
+    float wd[3][3];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) c[i] = 4.0f * sum[i];
-    const int size[3] = {a.nx, a.ny, a.nz};
+    for (int e = 0; e < 3; ++e)
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        float pos = a.p[i][q] + dt * vel[i];
-        keep_inside(pos, vel[i], size[i]);
-        a.p[i][q] = pos;
-        a.p[3 + i][q] = vel[i];
+        for (int i = 0; i < 3; ++i) wd[e][i] = ax[e].w[i] * ((float)i - ax[e].off);
+    float vel[3], sum[9];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float v = 0, s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            float R = 0, S = 0, U = 0;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                float P = 0, Q = 0;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const float g = node_vel(i + 3 * j + 9 * l, c);
+                    P = fmaf(ax[0].w[i], g, P);
+                    Q = fmaf(wd[0][i], g, Q);
+                }
+                R = fmaf(ax[1].w[j], P, R);
+                S = fmaf(ax[1].w[j], Q, S);
+                U = fmaf(wd[1][j], P, U);
+            }
+            v = fmaf(ax[2].w[l], R, v);
+            s0 = fmaf(ax[2].w[l], S, s0);
+            s1 = fmaf(ax[2].w[l], U, s1);
+            s2 = fmaf(wd[2][l], R, s2);
+        }
+        vel[c] = v;
+        sum[3 * c] = s0;
+        sum[3 * c + 1] = s1;
+        sum[3 * c + 2] = s2;
     }
-#pragma unroll
-    for (int i = 0; i < 9; ++i) a.p[8 + i][q] = c[i];
-    const float j = a.p[7][q] * (1.0f + dt * (c[0] + (c[4] + c[8])));
-    a.p[7][q] = clampf(j, 1.0f / kJBound, kJBound);
+    g2p_finish(a, q, pos0, vel, sum, dt);
 }
 
 __global__ __launch_bounds__(kBlock) void g2p3_kernel(Mpm3Args a) {
     const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (q >= a.count || !(a.p[6][q] > 0.0f)) return;
-    Axis ax[3] = {axis(a.p[0][q], a.nx), axis(a.p[1][q], a.ny), axis(a.p[2][q], a.nz)};
+    const float pos[3] = {a.p[0][q], a.p[1][q], a.p[2][q]};
+    Axis ax[3] = {axis(pos[0], a.nx), axis(pos[1], a.ny), axis(pos[2], a.nz)};
     const int64_t base = ((int64_t)ax[2].cell * a.ny + ax[1].cell) * a.nx + ax[0].cell;
     const int64_t plane = (int64_t)a.nx * a.ny;
     const float dt = bits_f(a.params[DT]);
-    g2p_one(a, q, ax, dt, [&](int k, int i) {
+    g2p_one(a, q, pos, ax, dt, [&](int k, int i) {
         return a.g[4 + i][base + (k / 9) * plane + ((k / 3) % 3) * a.nx + k % 3];
     });
 }
 
-// The chunk's node velocities staged in LDS; particles of the chunk are taken lane-consecutive.
-__global__ __launch_bounds__(kBlock) void g2p3_tiled_kernel(Mpm3Args a, int run) {
-    __shared__ float tile[3 * kTileNodes];
+constexpr int kG2PTile = 2560;  // nodes of the f32 LDS tile: 3 x 2560 x 4 B = 30 KB
+
+// A block takes a chunk of kBlock * R particles, lane-consecutive, with the chunk's node
+// velocities staged in LDS; every position is loaded once, up front, for the box and the move.
+template <int R, bool kFast>
+__global__ __launch_bounds__(kBlock) void g2p3_tiled_kernel(Mpm3Args a) {
+    __shared__ float tile[3 * kG2PTile];
     __shared__ int red[6 * (kBlock / kWave)];
-    const int64_t chunk = (int64_t)blockIdx.x * kBlock * run;
-    const Box box = block_box(a, chunk + threadIdx.x, run, kBlock, red);
+    const int64_t chunk = (int64_t)blockIdx.x * kBlock * R;
+    float pos[R][3];
+    bool live[R];
+    const int size[3] = {a.nx, a.ny, a.nz};
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int64_t q = chunk + (int64_t)r * kBlock + threadIdx.x;  // < capacity
+#pragma unroll
+        for (int i = 0; i < 3; ++i) pos[r][i] = a.p[i][q];
+        live[r] = q < a.count && a.p[6][q] > 0.0f;
+        if (!live[r]) continue;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int c = base_cell(pos[r][i], size[i]);
+            lo[i] = min(lo[i], c);
+            hi[i] = max(hi[i], c);
+        }
+    }
+    const Box box = reduce_box(lo, hi, red, kG2PTile);
     if (box.empty) return;
-    const bool use_tile = box.nodes <= kTileNodes;
+    const bool use_tile = box.nodes <= kG2PTile;
     if (use_tile) {
         for (int t = threadIdx.x; t < box.nodes; t += kBlock) {
             const int lx = t % box.ex, ly = (t / box.ex) % box.ey, lz = t / (box.ex * box.ey);
@@ -377,28 +670,31 @@ __global__ __launch_bounds__(kBlock) void g2p3_tiled_kernel(Mpm3Args a, int run)
     __syncthreads();
     const float dt = bits_f(a.params[DT]);
     const int64_t plane = (int64_t)a.nx * a.ny;
-    for (int r = 0; r < run; ++r) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (!live[r]) continue;
         const int64_t q = chunk + (int64_t)r * kBlock + threadIdx.x;
-        if (q >= a.count || !(a.p[6][q] > 0.0f)) continue;
-        Axis ax[3] = {axis(a.p[0][q], a.nx), axis(a.p[1][q], a.ny), axis(a.p[2][q], a.nz)};
+        Axis ax[3] = {axis(pos[r][0], a.nx), axis(pos[r][1], a.ny), axis(pos[r][2], a.nz)};
         if (use_tile) {
             const int base = ((ax[2].cell - box.lo[2]) * box.ey + (ax[1].cell - box.lo[1])) * box.ex +
                              (ax[0].cell - box.lo[0]);
-            g2p_one(a, q, ax, dt, [&](int k, int i) {
+            auto at = [&](int k, int i) {
                 return tile[i * box.nodes + base + ((k / 9) * box.ey + (k / 3) % 3) * box.ex + k % 3];
-            });
+            };
+            if (kFast) g2p_fast(a, q, pos[r], ax, dt, at);
+            else g2p_one(a, q, pos[r], ax, dt, at);
         } else {
             const int64_t base = ((int64_t)ax[2].cell * a.ny + ax[1].cell) * a.nx + ax[0].cell;
-            g2p_one(a, q, ax, dt, [&](int k, int i) {
-                return a.g[4 + i][base + (k / 9) * plane + ((k / 3) % 3) * a.nx + k % 3];
-            });
+            auto at = [&](int k, int i) { return a.g[4 + i][base + (k / 9) * plane + ((k / 3) % 3) * a.nx + k % 3]; };
+            if (kFast) g2p_fast(a, q, pos[r], ax, dt, at);
+            else g2p_one(a, q, pos[r], ax, dt, at);
         }
     }
 }
 
 // ------------------------------------------------------------------ sort keys
 
-__device__ __forceinline__ int brick_count(int n) { return (n - 2 + 7) / 8; }
+__host__ __device__ __forceinline__ int brick_count(int n) { return (n - 2 + 7) / 8; }
 
 // Stencil3's base cell numbered brick-major: 8x8x8-cell bricks row-major, cells row-major
 // inside a brick. Padding particles take the largest key so a stable sort leaves them last.
@@ -449,13 +745,19 @@ int mpm3_scatter_atomic(const Launch& l, const Mpm3Args& a) {
     return check_launch("p2g3_atomic");
 }
 
+template <int R>
+static void launch_p2g(const Launch& l, const Mpm3Args& a) {
+    hipLaunchKernelGGL(p2g3_wave_kernel<R>, dim3(blocks_for(a.count, kWave * R)), dim3(kWave), 0, l.stream, a);
+}
+
 int mpm3_scatter_tiled(const Launch& l, const Mpm3Args& a, int run) {
     if (a.count == 0) return 0;
-    timed_begin(l, "p2g3_tiled");
-    hipLaunchKernelGGL(p2g3_tiled_kernel, dim3(blocks_for(a.count, kBlock * run)), dim3(kBlock), 0, l.stream, a,
-                       run);
-    timed_end(l, "p2g3_tiled");
-    return check_launch("p2g3_tiled");
+    timed_begin(l, "p2g3_seg");
+    if (run == 4) launch_p2g<4>(l, a);
+    else if (run == 2) launch_p2g<2>(l, a);
+    else launch_p2g<1>(l, a);
+    timed_end(l, "p2g3_seg");
+    return check_launch("p2g3_seg");
 }
 
 int mpm3_grid(const Launch& l, const Mpm3Args& a) {
@@ -474,13 +776,22 @@ int mpm3_advect(const Launch& l, const Mpm3Args& a) {
     return check_launch("g2p3");
 }
 
-int mpm3_advect_tiled(const Launch& l, const Mpm3Args& a, int run) {
+template <int R>
+static void launch_g2p(const Launch& l, const Mpm3Args& a, bool fast) {
+    const unsigned blocks = blocks_for(a.count, kBlock * R);
+    if (fast) hipLaunchKernelGGL((g2p3_tiled_kernel<R, true>), dim3(blocks), dim3(kBlock), 0, l.stream, a);
+    else hipLaunchKernelGGL((g2p3_tiled_kernel<R, false>), dim3(blocks), dim3(kBlock), 0, l.stream, a);
+}
+
+int mpm3_advect_tiled(const Launch& l, const Mpm3Args& a, int run, bool fast) {
     if (a.count == 0) return 0;
-    timed_begin(l, "g2p3_tiled");
-    hipLaunchKernelGGL(g2p3_tiled_kernel, dim3(blocks_for(a.count, kBlock * run)), dim3(kBlock), 0, l.stream, a,
-                       run);
-    timed_end(l, "g2p3_tiled");
-    return check_launch("g2p3_tiled");
+    const char* name = fast ? "g2p3_fast" : "g2p3_tiled";
+    timed_begin(l, name);
+    if (run == 4) launch_g2p<4>(l, a, fast);
+    else if (run == 2) launch_g2p<2>(l, a, fast);
+    else launch_g2p<1>(l, a, fast);
+    timed_end(l, name);
+    return check_launch(name);
 }
 
 int mpm3_keys(const Launch& l, const Mpm3Args& a, int32_t* keys, int32_t* index) {

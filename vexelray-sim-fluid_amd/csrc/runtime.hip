@@ -273,8 +273,10 @@ int run_pass(vxr_sim* s, const std::string& pass) {
         if (pass == "scatter")
             return opt(s, "p2g") == 0 ? vxr::mpm3_scatter_atomic(l, a) : vxr::mpm3_scatter_tiled(l, a, (int)opt(s, "run"));
         if (pass == "grid") return vxr::mpm3_grid(l, a);
-        if (pass == "advect")
-            return opt(s, "g2p") == 0 ? vxr::mpm3_advect(l, a) : vxr::mpm3_advect_tiled(l, a, (int)opt(s, "run"));
+        if (pass == "advect") {
+            const int64_t g2p = opt(s, "g2p");
+            return g2p == 0 ? vxr::mpm3_advect(l, a) : vxr::mpm3_advect_tiled(l, a, (int)opt(s, "run"), g2p == 2);
+        }
     } else if (s->kind == VXR_KIND_MPM2) {
         const vxr::Mpm2Args a = mpm2_args(s);
         if (pass == "clear") return vxr::mpm2_clear(l, a);
@@ -327,7 +329,7 @@ struct OptionSpec {
     int64_t lo, hi, def;
 };
 const OptionSpec kOptions[] = {
-    {"p2g", 0, 1, 1}, {"g2p", 0, 1, 1}, {"run", 1, 8, 4}, {"radix_bits", 4, 8, 8}, {"rows", 4, 256, 32},
+    {"p2g", 0, 1, 1}, {"g2p", 0, 2, 2}, {"run", 1, 4, 4}, {"radix_bits", 4, 8, 8}, {"rows", 4, 256, 32},
 };
 
 }  // namespace
@@ -571,6 +573,8 @@ int vxr_set_option(vxr_sim* s, const char* key, int64_t value) {
             if (value < o.lo || value > o.hi)
                 return fail(VXR_EINVAL, "option '%s' takes %lld..%lld, got %lld", key, (long long)o.lo, (long long)o.hi,
                             (long long)value);
+            if (std::string(key) == "run" && value == 3)
+                return fail(VXR_EINVAL, "option 'run' takes 1, 2 or 4, got 3");
             s->options[key] = value;
             return VXR_OK;
         }
