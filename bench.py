@@ -117,6 +117,33 @@ def cpu_baseline_mpm3():
             "host_cores": os.cpu_count()}
 
 
+def copy_calibration(mib):
+    """A plain device copy of a known size, timed under the same HBM counters (FETCH_SIZE,
+    WRITE_SIZE) as the kernels, to scale those counters to bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 3}), flush=True)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the counter summary kept beside this file
+    (profiles/traffic.json, written from two rocprofv3 --pmc passes and the copy calibration);
+    None when the kernel has no entry."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f).get("kernels", {})
+    entry = table.get(kernel)
+    return None if entry is None else entry.get("hbm_bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,7 +155,12 @@ def main():
     ap.add_argument("--hll-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hll", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB mebibytes (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the path is single-GPU; "
               "the design notes say how it would shard)", file=sys.stderr)
@@ -147,7 +179,8 @@ def main():
     dk, dv = dominant
     roofline = {"kernel": dk, "bound": "hbm", "achieved": dv["achieved_gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": None if dv["achieved_gbs"] is None else round(dv["achieved_gbs"] / PEAK_HBM_GBS, 4),
-                "traffic": None, "bytes_per_launch": dv["algorithmic_bytes"], "avg_us": dv["avg_us"]}
+                "traffic": measured_traffic(dk), "bytes_per_launch": dv["algorithmic_bytes"],
+                "avg_us": dv["avg_us"]}
     line = {
         "metric": "particle-substeps/s (3D MPM) & cell-updates/s (HLL); surrogate vs HLL at ≤1% err",
         "value": round(value, 1), "unit": "particle-substeps/s", "n_gpus": 1, "steps": args.steps,
@@ -168,7 +201,8 @@ def main():
                        "workload": "HLL 4096^2 circular dam + 32 seeded bumps, walls, CFL 0.45",
                        "roofline": {"kernel": "hll_step", "bound": "hbm", "achieved": hk["achieved_gbs"],
                                     "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                                    "frac": round(hk["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": None,
+                                    "frac": round(hk["achieved_gbs"] / PEAK_HBM_GBS, 4),
+                                    "traffic": measured_traffic("hll_step"),
                                     "avg_us": hk["avg_us"]}}
     if not args.no_cpu:
         print("[bench] CPU oracle baseline (one substep, one thread)", file=sys.stderr, flush=True)

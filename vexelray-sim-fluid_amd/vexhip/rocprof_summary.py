@@ -1,0 +1,91 @@
+"""Condenses rocprofv3 CSV output into the short summaries kept under profiles/.
+
+    python -m vexhip.rocprof_summary stats  <rocprof-dir> <out.txt> "<command that made it>"
+    python -m vexhip.rocprof_summary traffic <fetch-dir> <write-dir> <cal-fetch-dir> <cal-write-dir> \
+        <copy-bytes> <out.json> "<commands>"
+
+``stats`` reads run_kernel_stats.csv (rocprofv3 --kernel-trace --stats --output-format csv) and
+writes one line per kernel: launches, average/min/max duration and share of GPU time.
+
+``traffic`` reads the FETCH_SIZE and WRITE_SIZE passes (rocprofv3 --pmc <counter> --kernel-trace
+--output-format csv, KiB per dispatch) of the workload and of a plain device copy of
+``copy-bytes`` each way, scales each counter by (copy bytes / what the counter read for the copy)
+and writes HBM bytes per launch for every kernel, the scale factors beside them.
+"""
+import collections
+import csv
+import json
+import os
+import sys
+
+_SHORT = {
+    "p2g3_wave_kernel": "p2g3_seg", "p2g3_atomic_kernel": "p2g3_atomic", "g2p3_tiled_kernel<4, true>": "g2p3_fast",
+    "g2p3_tiled_kernel": "g2p3_tiled", "g2p3_kernel": "g2p3", "grid3_kernel": "grid3", "clear4_kernel": "mpm3_clear",
+    "keys3_kernel": "keys3", "hist_kernel": "radix_hist", "scan_rows_kernel": "radix_scan",
+    "scatter_kernel": "radix_scatter", "permute_kernel": "permute", "hll_step_kernel": "hll_step",
+}
+
+
+def short_name(kernel: str) -> str:
+    """The library's timer name for a demangled kernel name (the names bench.py reports)."""
+    for key, name in _SHORT.items():
+        if key in kernel:
+            return name
+    return kernel
+
+
+def _counters(directory):
+    per = collections.defaultdict(list)
+    with open(os.path.join(directory, "run_counter_collection.csv")) as f:
+        for r in csv.DictReader(f):
+            per[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in per.items()}, {k: len(v) for k, v in per.items()}
+
+
+def stats(directory, out, command):
+    with open(os.path.join(directory, "run_kernel_stats.csv")) as f:
+        rows = list(csv.DictReader(f))
+    lines = [f"# rocprofv3 kernel statistics", f"# command: {command}",
+             "# name | launches | avg us | min us | max us | share of GPU time %"]
+    for r in rows:
+        lines.append(f"{short_name(r['Name']):14s} | {r['Name'][:90]} | {r['Calls']} | "
+                     f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['MinNs']) / 1e3:.1f} | "
+                     f"{float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.2f}")
+    with open(out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def traffic(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, copy_bytes, out, command):
+    copy_bytes = float(copy_bytes)
+    cf, _ = _counters(cal_fetch_dir)
+    cw, _ = _counters(cal_write_dir)
+    copy = [k for k in cf if "copyBuffer" in k or "copy" in k.lower()]
+    if not copy:
+        raise SystemExit("no copy kernel in the calibration passes")
+    fetch_scale = copy_bytes / (cf[copy[0]] * 1024.0)
+    write_scale = copy_bytes / (cw[copy[0]] * 1024.0)
+    f, nf = _counters(fetch_dir)
+    w, _ = _counters(write_dir)
+    kernels = {}
+    for k in f:
+        if k not in w or "rocclr" in k:
+            continue
+        fb = f[k] * 1024.0 * fetch_scale
+        wb = w[k] * 1024.0 * write_scale
+        kernels[short_name(k)] = {"kernel": k, "dispatches": nf[k], "fetch_kib_raw": round(f[k], 1),
+                                  "write_kib_raw": round(w[k], 1), "fetch_bytes": round(fb),
+                                  "write_bytes": round(wb), "hbm_bytes_per_launch": round(fb + wb)}
+    doc = {"command": command, "copy_bytes_each_way": copy_bytes, "fetch_scale": round(fetch_scale, 4),
+           "write_scale": round(write_scale, 4), "kernels": kernels}
+    with open(out, "w") as fh:
+        json.dump(doc, fh, indent=1)
+        fh.write("\n")
+
+
+if __name__ == "__main__":
+    if sys.argv[1] == "stats":
+        stats(*sys.argv[2:5])
+    elif sys.argv[1] == "traffic":
+        traffic(*sys.argv[2:9])
+    else:
+        raise SystemExit(__doc__)
