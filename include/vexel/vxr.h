@@ -32,6 +32,7 @@ extern "C" {
 #define VXR_KIND_MPM2 1 /* gui/ParticleSimulation.java over core/particle/FlipStep.java */
 #define VXR_KIND_MPM3 2 /* gui/ParticleSimulation3.java over core/particle/Flip3Step.java */
 #define VXR_KIND_SWE 3  /* gui/PatchSimulation.java over core/stencil/ShallowWater.java */
+#define VXR_KIND_MPM2_BATCH 4 /* many ParticleSimulation scenes in one set of buffers (vxr_create_batch) */
 
 /* Creation flags. The reference's optional physics (FlipStep's tension, heat, convection,
  * relax, pump; Flip3Step's tension, plastic) are outside this path: VXR_EUNSUPPORTED. */
@@ -68,6 +69,22 @@ int vxr_device_count(int* n);
  * Safe on any thread; the handle then belongs to the thread that uses it. */
 int vxr_create(int kind, const int32_t dims[3], int64_t particles, uint32_t flags, vxr_sim** out);
 
+/* Makes a batch of `scenes` independent 2D scenes (no reference counterpart: the reference runs
+ * one gui/ParticleSimulation per scene). Scene k has an nx[k] x ny[k] grid and particles[k]
+ * particles. Every pass runs all scenes in one launch, and each scene's results are those of a
+ * single-scene VXR_KIND_MPM2 run with option p2g = 2 (the fixed-point P2G), bit for bit. The
+ * buffers have the 2D names; each holds every scene at the offsets vxr_batch_layout gives
+ * (particle ranges padded with m = 0 particles), and "params" holds 28 words per scene. */
+int vxr_create_batch(int scenes, const int32_t* nx, const int32_t* ny, const int64_t* particles, uint32_t flags,
+                     vxr_sim** out);
+
+/* Scene k's first particle, node and start in the batch's buffers; each array takes scenes + 1
+ * entries, the last being the buffer's length. */
+int vxr_batch_layout(vxr_sim* sim, int64_t* particle_offsets, int64_t* node_offsets, int64_t* start_offsets);
+
+/* Scenes in a batch (1 for any other simulation). */
+int vxr_scene_count(vxr_sim* sim, int* n);
+
 /* Frees every device buffer and the stream (ParticleSimulation.close, :228-231). */
 int vxr_destroy(vxr_sim* sim);
 
@@ -89,6 +106,12 @@ int vxr_length(vxr_sim* sim, const char* name, int64_t* n);
 /* Queues `repeat` runs of a pass group (PassRunner.run(step.step()) / run(step.sort()),
  * gui/ParticleSimulation.java:175-183, gui/ParticleSimulation3.java:291-296). Asynchronous. */
 int vxr_run(vxr_sim* sim, int group, int repeat);
+
+/* ParticleSimulation.advance (gui/ParticleSimulation.java:175-183): `count` steps, numbered from
+ * `first_step` (the caller's step counter), with a sort first on each step whose number is a
+ * multiple of `sort_every` (0: never). With option "graph" = 1 and timing off, each run of a sort
+ * and the steps up to the next one is captured once as a hipGraph and replayed. Asynchronous. */
+int vxr_advance(vxr_sim* sim, int count, int sort_every, int64_t first_step);
 
 /* Queues one named pass: "clear", "scatter", "grid", "advect", "sort" (a sub-list of the
  * step, as test/particle/Flip3Test.java:102 runs step().subList(0, 2)). */
@@ -114,19 +137,30 @@ int vxr_device_ptr(vxr_sim* sim, const char* name, void** ptr);
 /* The hipStream_t the simulation queues on, for a caller that orders its own work after it. */
 int vxr_stream(vxr_sim* sim, void** stream);
 
-/* Schedule knobs (no reference counterpart; the autotuner sets them):
+/* Schedule knobs (no reference counterpart; vxr_autotune sets them):
  *   "p2g"        MPM: 0 = direct float atomics (the reference's Flip3.scatter schedule),
- *                1 = sorted-segmented (default): per-thread runs of consecutive particles, an FP64
- *                LDS node tile per block of 256 * run particles, one atomic per node and field
+ *                1 = sorted-segmented (default): per-thread runs of consecutive particles folded in
+ *                registers (3D: then a wave-private LDS node tile), one atomic per node and field;
+ *                2 = 2D only, and always for a batch: fixed point, each amount rounded to a 64-bit
+ *                integer (mass 2^-40, momentum 2^-32) and summed by integer atomics -- the same
+ *                bits in any order, equal to the CPU restatement's
  *   "g2p"        MPM3: 0 = per-particle gathers and 1 = an LDS node tile, both in the reference's
  *                arithmetic order (the CPU restatement's bits); 2 = the LDS tile with the sums
  *                contracted axis by axis in fused multiply-adds (default, within a few ulps)
  *   "run"        MPM: consecutive particles a thread takes in the segmented passes: 1, 2 or 4
  *                (default 4)
  *   "radix_bits" sort digit width (4..8, default 8)
- *   "rows"       SWE: rows a wave marches per tile (4..256, default 32) */
+ *   "rows"       SWE: rows a wave marches per tile (4..256, default 32)
+ *   "blocks"     batch passes: blocks per launch, each striding over chunks (0 = one per chunk)
+ *   "graph"      1 = vxr_advance replays captured hipGraphs (default 0) */
 int vxr_set_option(vxr_sim* sim, const char* key, int64_t value);
 int vxr_get_option(vxr_sim* sim, const char* key, int64_t* value);
+
+/* Times the choices of "run", "blocks" and "radix_bits" (MPM) or "rows" (SWE) on this
+ * simulation's own data, `reps` runs each, and keeps the fastest. Only passes that leave the
+ * state as it was are timed: clear, scatter and grid, the sort's keys and radix passes without
+ * the permutation, and the shallow-water step into the spare slot. Synchronous. */
+int vxr_autotune(vxr_sim* sim, int reps);
 
 /* Kernel timing with HIP events on the simulation's own stream. Enabling clears the
  * totals; vxr_kernel_time synchronizes and returns the summed milliseconds and launch

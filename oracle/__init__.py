@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _c_i = ctypes.c_int
 _c_i64 = ctypes.c_int64
@@ -64,6 +65,8 @@ def _load() -> ctypes.CDLL:
         "orc_mpm2_grid": (None, [_c_i, _c_i, _i32p, _f32p, _f32p, _f32p, _f32p, _f32p]),
         "orc_mpm2_advect": (None, [_c_i, _c_i, _c_i64, _vp, _i32p, _f32p, _f32p]),
         "orc_mpm2_step": (None, [_c_i, _c_i, _c_i64, _vp, _i32p, _vp]),
+        "orc_mpm2_scatter_fixed": (None, [_c_i, _c_i, _c_i64, _vp, _i32p, _i64p, _i64p, _i64p]),
+        "orc_mpm2_decode_fixed": (None, [_c_i64, _i64p, _i64p, _i64p, _f32p, _f32p, _f32p]),
         "orc_mpm2_cell": (ctypes.c_int32, [ctypes.c_float, ctypes.c_float, _c_i, _c_i]),
         "orc_mpm2_keys": (None, [_c_i, _c_i, _c_i64, _f32p, _f32p, _i32p]),
         "orc_mpm2_cell_starts": (_c_i64, [_c_i, _c_i, _c_i64, _f32p, _f32p, _i32p]),
@@ -302,9 +305,12 @@ def mpm3_key_bits(nx, ny, nz) -> int:
 
 
 class Mpm2:
-    """A 2D particle state on the CPU, stepped as FlipStep lists its passes."""
+    """A 2D particle state on the CPU, stepped as FlipStep lists its passes. With ``fixed`` the
+    scatter is the deterministic fixed-point one (orc_mpm2_scatter_fixed) and its sums are
+    decoded into gm, gmu, gmv before the grid update."""
 
-    def __init__(self, nx, ny, fields: dict, params):
+    def __init__(self, nx, ny, fields: dict, params, fixed=False):
+        self.fixed = fixed
         self.n = (nx, ny)
         count = len(fields["x"])
         self.p = {f: _c(fields.get(f, np.zeros(count)), np.float32).copy() for f in FIELDS2}
@@ -322,6 +328,13 @@ class Mpm2:
         lib.orc_mpm2_clear(*self.n, self.g["gm"], self.g["gmu"], self.g["gmv"])
 
     def scatter(self):
+        if self.fixed:
+            nodes = self.n[0] * self.n[1]
+            q = [np.zeros(nodes, np.int64) for _ in range(3)]
+            lib.orc_mpm2_scatter_fixed(*self.n, self.count, self._pp(), self.params, *q)
+            self.q = q
+            lib.orc_mpm2_decode_fixed(nodes, *q, self.g["gm"], self.g["gmu"], self.g["gmv"])
+            return
         lib.orc_mpm2_scatter(*self.n, self.count, self._pp(), self.params,
                              self.g["gm"], self.g["gmu"], self.g["gmv"])
 
@@ -332,6 +345,13 @@ class Mpm2:
         lib.orc_mpm2_advect(*self.n, self.count, self._pp(), self.params, self.g["gu"], self.g["gv"])
 
     def step(self, n=1):
+        if self.fixed:
+            for _ in range(n):
+                self.clear()
+                self.scatter()
+                self.grid()
+                self.advect()
+            return
         grid = _ptrs([self.g[f] for f in GRID2])
         for _ in range(n):
             lib.orc_mpm2_step(*self.n, self.count, self._pp(), self.params, grid)

@@ -5,6 +5,7 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -247,6 +248,55 @@ void orc_mpm2_scatter(int nx, int ny, int64_t count, float* const p[10], const i
 }
 
 // Flip.grid, Flip.java:366-412 (the plain branch: weight 1)
+// Fixed-point amounts: a * 2^bits is exact in binary32 (a power-of-two scaling), and llrintf
+// rounds it to the nearest integer, ties to even, under the default rounding mode.
+static inline int64_t quantize(float a, int bits) { return (int64_t)llrintf(std::ldexp(a, bits)); }
+
+void orc_mpm2_scatter_fixed(int nx, int ny, int64_t count, float* const p[10], const int32_t* params,
+                            int64_t* qm, int64_t* qmu, int64_t* qmv) {
+    const float dt = bits_to_float(params[DT2]);
+    const float bulk = bits_to_float(params[BULK2]);
+    const float rho0 = bits_to_float(params[RHO02]);
+    const int64_t nodes = (int64_t)nx * ny;
+    std::fill(qm, qm + nodes, 0);
+    std::fill(qmu, qmu + nodes, 0);
+    std::fill(qmv, qmv + nodes, 0);
+    for (int64_t q = 0; q < count; q++) {
+        int32_t col, row;
+        float fx, fy, wx[3], wy[3];
+        axis(p[0][q], nx, &col, &fx, wx);
+        axis(p[1][q], ny, &row, &fy, wy);
+        const int64_t base = (int64_t)row * nx + col;
+        const float m = p[4][q], u = p[2][q], w = p[3][q];
+        const float c00 = p[6][q], c01 = p[7][q], c10 = p[8][q], c11 = p[9][q];
+        const float s = (4.0f * (dt * (m / rho0))) * (bulk * (1.0f - p[5][q]));
+        for (int k = 0; k < 9; k++) {
+            const float wgt = wx[k % 3] * wy[k / 3];
+            const float wm = wgt * m;
+            const float dx = (float)(k % 3) - fx;
+            const float dy = (float)(k / 3) - fy;
+            const float vu = u + (c00 * dx + c01 * dy);
+            const float vv = w + (c10 * dx + c11 * dy);
+            const int64_t node = base + (int64_t)(k / 3) * nx + k % 3;
+            // two's-complement wrap-around, as the device's 64-bit atomics
+            qm[node] = (int64_t)((uint64_t)qm[node] + (uint64_t)quantize(wm, ORC_FIXED_MASS_BITS));
+            qmu[node] = (int64_t)((uint64_t)qmu[node] +
+                                  (uint64_t)quantize(wm * vu + wgt * (s * dx), ORC_FIXED_MOMENTUM_BITS));
+            qmv[node] = (int64_t)((uint64_t)qmv[node] +
+                                  (uint64_t)quantize(wm * vv + wgt * (s * dy), ORC_FIXED_MOMENTUM_BITS));
+        }
+    }
+}
+
+void orc_mpm2_decode_fixed(int64_t nodes, const int64_t* qm, const int64_t* qmu, const int64_t* qmv, float* gm,
+                           float* gmu, float* gmv) {
+    for (int64_t i = 0; i < nodes; i++) {
+        gm[i] = (float)std::ldexp((double)qm[i], -ORC_FIXED_MASS_BITS);
+        gmu[i] = (float)std::ldexp((double)qmu[i], -ORC_FIXED_MOMENTUM_BITS);
+        gmv[i] = (float)std::ldexp((double)qmv[i], -ORC_FIXED_MOMENTUM_BITS);
+    }
+}
+
 void orc_mpm2_grid(int nx, int ny, const int32_t* params, const float* gm, const float* gmu, const float* gmv,
                    float* gu, float* gv) {
     const float dt = bits_to_float(params[DT2]);

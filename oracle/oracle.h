@@ -94,6 +94,18 @@ void orc_mpm2_advect(int nx, int ny, int64_t count, float* const p[10], const in
                      const float* gu, const float* gv);
 void orc_mpm2_step(int nx, int ny, int64_t count, float* const p[10], const int32_t* params,
                    float* const grid[5]);
+/* The deterministic P2G: each of a particle's 27 amounts (computed as orc_mpm2_scatter computes
+ * them) is rounded to a fixed-point integer -- mass in units of 2^-40, momentum in 2^-32, round to
+ * nearest even -- and summed in 64-bit integers, which are associative, so every order and
+ * grouping of the sums gives the same grid (the principle of core/particle/FixedScatter.java:24-31,
+ * applied to the quadratic APIC amounts). orc_mpm2_decode_fixed turns the sums back into the
+ * float accumulators the grid update reads: (float)((double)q * 2^-bits). */
+#define ORC_FIXED_MASS_BITS 40
+#define ORC_FIXED_MOMENTUM_BITS 32
+void orc_mpm2_scatter_fixed(int nx, int ny, int64_t count, float* const p[10], const int32_t* params,
+                            int64_t* qm, int64_t* qmu, int64_t* qmv);
+void orc_mpm2_decode_fixed(int64_t nodes, const int64_t* qm, const int64_t* qmu, const int64_t* qmv,
+                           float* gm, float* gmu, float* gmv);
 /* core/particle/Scatter.java:473-477 */
 int32_t orc_mpm2_cell(float x, float y, int nx, int ny);
 void orc_mpm2_keys(int nx, int ny, int64_t count, const float* x, const float* y, int32_t* keys);

@@ -54,6 +54,17 @@ def test_arguments_refused_before_the_device():
     assert lib.vxr_run(None, 0, 1) == _native.EINVAL
     n = ctypes.c_int(-1)
     assert lib.vxr_device_count(ctypes.byref(n)) == _native.OK and n.value >= 0
+    # a batch of scenes is refused on its shapes before any device call
+    nx = (ctypes.c_int32 * 2)(8, 3)
+    ny = (ctypes.c_int32 * 2)(8, 8)
+    cnt = (ctypes.c_int64 * 2)(10, 10)
+    assert lib.vxr_create_batch(0, nx, ny, cnt, 0, ctypes.byref(h)) == _native.EINVAL
+    assert lib.vxr_create_batch(2, nx, ny, cnt, 0, ctypes.byref(h)) == _native.EINVAL
+    assert b"scene 1" in lib.vxr_last_error()
+    assert lib.vxr_create_batch(1, nx, ny, cnt, _native.FLAG_HEAT, ctypes.byref(h)) == _native.EUNSUPPORTED
+    assert lib.vxr_create(_native.KIND_MPM2_BATCH, dims, 10, 0, ctypes.byref(h)) == _native.EINVAL
+    assert lib.vxr_advance(None, 1, 10, 0) == _native.EINVAL
+    assert lib.vxr_autotune(None, 1) == _native.EINVAL
 
 
 def test_params_words():

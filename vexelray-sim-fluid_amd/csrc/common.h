@@ -95,6 +95,36 @@ int mpm2_advect(const Launch& l, const Mpm2Args& a);
 int mpm2_keys(const Launch& l, const Mpm2Args& a, int32_t* keys, int32_t* index);
 int mpm2_starts(const Launch& l, const int32_t* sorted_keys, int64_t count, int32_t cells, int32_t* starts);
 
+// Many 2D scenes in one set of buffers, stepped by one launch per pass (a single scene with the
+// fixed-point P2G is a batch of one). Scene s: particles [poff, poff + count) padded to
+// kBatchAlign, nodes [goff, goff + nx*ny) padded to 256, starts [soff, soff + cells + 1) padded
+// to 256, 28 params at params[28 s].
+constexpr int kBatchAlign = 1024;
+struct Scene2 {
+    int nx, ny;
+    int32_t cells, unused;
+    int64_t poff, count, goff, soff;
+};
+struct Batch2Args {
+    const Scene2* scenes;
+    const int32_t* pscene;  // scene of each kBatchAlign particles
+    const int32_t* gscene;  // scene of each 256 nodes
+    const int32_t* sscene;  // scene of each 256 starts
+    int64_t pchunks, gchunks, schunks;
+    float* p[10];
+    float* g[5];
+    unsigned long long* q[3];  // fixed-point sums: mass (2^-40), momentum u, v (2^-32)
+    const int32_t* params;
+    int cell_bits;
+};
+// `blocks` > 0 caps the grid (each block then strides over chunks); 0 is a block per chunk.
+int batch2_clear(const Launch& l, const Batch2Args& a, int blocks);
+int batch2_scatter(const Launch& l, const Batch2Args& a, int run, int blocks);
+int batch2_grid(const Launch& l, const Batch2Args& a, int blocks);
+int batch2_advect(const Launch& l, const Batch2Args& a, int blocks);
+int batch2_keys(const Launch& l, const Batch2Args& a, int32_t* keys, int32_t* index, int blocks);
+int batch2_starts(const Launch& l, const Batch2Args& a, const int32_t* keys, int32_t* starts, int blocks);
+
 // ------------------------------------------------------------------ sort
 
 // Stable LSD radix sort of (key, index) pairs, keys in [0, 2^bits). Scratch sized by

@@ -3,12 +3,18 @@
 // Layout: 10 particle fields as f32 arrays (x y u v m j c00 c01 c10 c11), padded with m = 0
 // to whole chunks; the grid as 5 f32 node arrays, row-major (gm gmu gmv gu gv).
 //
-// P2G: the reference sums each run of particles sharing a stencil base across a subgroup's
+// P2G (float, p2g 0 and 1): the reference sums each run of particles sharing a stencil base across a subgroup's
 // lanes (Scatter.segmentedDeposit, Scatter.java:385-439) and the run's last lane takes the
 // atomics. Here a thread walks a run of consecutive particles and folds those sharing a base
 // into 27 register sums, with one atomic per node and field for each run it holds: the same
 // sums regrouped, right in any order and cheap in cell order. The grid update and G2P keep
 // the reference's arithmetic order, so from the same grid they are its results bit for bit.
+//
+// P2G in fixed point (p2g 2, and every batch of scenes): each amount rounded to a 64-bit
+// integer (mass in 2^-40, momentum in 2^-32) and summed by integer atomics, which associate
+// exactly -- core/particle/FixedScatter.java:24-31's reason for integers, applied to these
+// amounts -- so the grid is the same bits in any order, on any launch shape, and equals the
+// CPU restatement's (orc_mpm2_scatter_fixed) bit for bit.
 #include "common.h"
 
 namespace vxr {
@@ -43,12 +49,14 @@ __device__ __forceinline__ Axis axis(float pos, int n) {
     return s;
 }
 
-// The 27 amounts of one particle (Flip.java:283-300), node k's three at 3k .. 3k+2.
-__device__ __forceinline__ void amounts(const Mpm2Args& a, int64_t q, const Axis& ax, const Axis& ay, float dt,
-                                        float bulk, float rho0, float (&acc)[27]) {
-    const float m = a.p[4][q], u = a.p[2][q], w = a.p[3][q];
-    const float c00 = a.p[6][q], c01 = a.p[7][q], c10 = a.p[8][q], c11 = a.p[9][q];
-    const float s = (4.0f * (dt * (m / rho0))) * (bulk * (1.0f - a.p[5][q]));
+// The 27 amounts of one particle (Flip.java:283-300), node k's three at 3k .. 3k+2, handed to
+// `take(k, mass, momentum_u, momentum_v)`.
+template <class Take>
+__device__ __forceinline__ void amounts(float* const* p, int64_t q, const Axis& ax, const Axis& ay, float dt,
+                                        float bulk, float rho0, Take take) {
+    const float m = p[4][q], u = p[2][q], w = p[3][q];
+    const float c00 = p[6][q], c01 = p[7][q], c10 = p[8][q], c11 = p[9][q];
+    const float s = (4.0f * (dt * (m / rho0))) * (bulk * (1.0f - p[5][q]));
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const float wgt = ax.w[k % 3] * ay.w[k / 3];
@@ -57,10 +65,26 @@ __device__ __forceinline__ void amounts(const Mpm2Args& a, int64_t q, const Axis
         const float dy = (float)(k / 3) - ay.off;
         const float vu = u + (c00 * dx + c01 * dy);
         const float vv = w + (c10 * dx + c11 * dy);
-        acc[3 * k] += wm;
-        acc[3 * k + 1] += wm * vu + wgt * (s * dx);
-        acc[3 * k + 2] += wm * vv + wgt * (s * dy);
+        take(k, wm, wm * vu + wgt * (s * dx), wm * vv + wgt * (s * dy));
     }
+}
+
+__device__ __forceinline__ void amounts(float* const* p, int64_t q, const Axis& ax, const Axis& ay, float dt,
+                                        float bulk, float rho0, float (&acc)[27]) {
+    amounts(p, q, ax, ay, dt, bulk, rho0, [&](int k, float a0, float a1, float a2) {
+        acc[3 * k] += a0;
+        acc[3 * k + 1] += a1;
+        acc[3 * k + 2] += a2;
+    });
+}
+
+// Fixed point: an amount in units of 2^-bits, rounded to the nearest integer (ties to even).
+// Scaling by a power of two is exact, so this is the one rounding, the same on the CPU.
+constexpr int kMassBits = 40;
+constexpr int kMomentumBits = 32;
+__device__ __forceinline__ long long quantize(float a, float scale) { return __float2ll_rn(a * scale); }
+__device__ __forceinline__ float dequantize(unsigned long long q, int bits) {
+    return (float)ldexp((double)(long long)q, -bits);
 }
 
 __global__ void clear3_kernel(float* __restrict__ g0, float* __restrict__ g1, float* __restrict__ g2, int64_t n) {
@@ -80,7 +104,7 @@ __global__ __launch_bounds__(kBlock) void p2g2_atomic_kernel(Mpm2Args a) {
     float acc[27];
 #pragma unroll
     for (int i = 0; i < 27; ++i) acc[i] = 0.0f;
-    amounts(a, q, ax, ay, dt, bulk, rho0, acc);
+    amounts(a.p, q, ax, ay, dt, bulk, rho0, acc);
     const int64_t base = (int64_t)ay.cell * a.nx + ax.cell;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
@@ -116,29 +140,33 @@ __global__ __launch_bounds__(kBlock) void p2g2_runs_kernel(Mpm2Args a, int run) 
             if (run_base >= 0) flush(run_base);
             run_base = base;
         }
-        amounts(a, q, ax, ay, dt, bulk, rho0, acc);
+        amounts(a.p, q, ax, ay, dt, bulk, rho0, acc);
     }
     if (run_base >= 0) flush(run_base);
 }
 
-// Flip.grid, Flip.java:366-412 (weight 1: no buoyancy)
+// Flip.grid, Flip.java:366-412 (weight 1: no buoyancy), for node `node` of an nx x ny grid.
+__device__ __forceinline__ void grid2_node(int64_t node, int nx, const int32_t* params, float m, float mu, float mv,
+                                           int ny, float* gu, float* gv) {
+    const int ni = (int)(node % nx), nj = (int)(node / nx);
+    float u = 0.0f, w = 0.0f;
+    if (m > kEmpty) {
+        const float dt = bits_f(params[DT]);
+        u = mu / m + dt * (bits_f(params[GX]) * 1.0f);
+        w = mv / m + dt * (bits_f(params[GY]) * 1.0f);
+    }
+    if (!(ni > 1)) u = mx(u, 0.0f);
+    if (ni > nx - 3) u = mn(u, 0.0f);
+    if (!(nj > 1)) w = mx(w, 0.0f);
+    if (nj > ny - 3) w = mn(w, 0.0f);
+    gu[node] = u;
+    gv[node] = w;
+}
+
 __global__ __launch_bounds__(kBlock) void grid2_kernel(Mpm2Args a) {
     const int64_t node = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (node >= (int64_t)a.nx * a.ny) return;
-    const int ni = (int)(node % a.nx), nj = (int)(node / a.nx);
-    const float m = a.g[0][node];
-    float u = 0.0f, w = 0.0f;
-    if (m > kEmpty) {
-        const float dt = bits_f(a.params[DT]);
-        u = a.g[1][node] / m + dt * (bits_f(a.params[GX]) * 1.0f);
-        w = a.g[2][node] / m + dt * (bits_f(a.params[GY]) * 1.0f);
-    }
-    if (!(ni > 1)) u = mx(u, 0.0f);
-    if (ni > a.nx - 3) u = mn(u, 0.0f);
-    if (!(nj > 1)) w = mx(w, 0.0f);
-    if (nj > a.ny - 3) w = mn(w, 0.0f);
-    a.g[3][node] = u;
-    a.g[4][node] = w;
+    grid2_node(node, a.nx, a.params, a.g[0][node], a.g[1][node], a.g[2][node], a.ny, a.g[3], a.g[4]);
 }
 
 __device__ __forceinline__ void keep_inside(float& at, float& vel, int size) {
@@ -154,21 +182,20 @@ __device__ __forceinline__ void keep_inside(float& at, float& vel, int size) {
     }
 }
 
-// Flip.advectBody, Flip.java:494-545 (not relaxing)
-__global__ __launch_bounds__(kBlock) void g2p2_kernel(Mpm2Args a) {
-    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (q >= a.count) return;
-    const Axis ax = axis(a.p[0][q], a.nx), ay = axis(a.p[1][q], a.ny);
-    const int64_t base = (int64_t)ay.cell * a.nx + ax.cell;
+// Flip.advectBody, Flip.java:494-545 (not relaxing), for particle q of one scene.
+__device__ __forceinline__ void g2p2_particle(float* const* p, int64_t q, int nx, int ny, const float* gu,
+                                              const float* gv, const int32_t* params) {
+    const Axis ax = axis(p[0][q], nx), ay = axis(p[1][q], ny);
+    const int64_t base = (int64_t)ay.cell * nx + ax.cell;
     float u = 0, w = 0, b00 = 0, b01 = 0, b10 = 0, b11 = 0;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const float wgt = ax.w[k % 3] * ay.w[k / 3];
         const float dx = (float)(k % 3) - ax.off;
         const float dy = (float)(k / 3) - ay.off;
-        const int64_t node = base + (k / 3) * a.nx + k % 3;
-        const float ui = wgt * a.g[3][node];
-        const float vi = wgt * a.g[4][node];
+        const int64_t node = base + (k / 3) * nx + k % 3;
+        const float ui = wgt * gu[node];
+        const float vi = wgt * gv[node];
         u = u + ui;
         w = w + vi;
         b00 = b00 + ui * dx;
@@ -177,21 +204,34 @@ __global__ __launch_bounds__(kBlock) void g2p2_kernel(Mpm2Args a) {
         b11 = b11 + vi * dy;
     }
     const float c00 = 4.0f * b00, c01 = 4.0f * b01, c10 = 4.0f * b10, c11 = 4.0f * b11;
-    const float dt = bits_f(a.params[DT]);
-    float x = a.p[0][q] + dt * u;
-    float y = a.p[1][q] + dt * w;
-    keep_inside(x, u, a.nx);
-    keep_inside(y, w, a.ny);
-    a.p[0][q] = x;
-    a.p[1][q] = y;
-    a.p[2][q] = u;
-    a.p[3][q] = w;
-    a.p[6][q] = c00;
-    a.p[7][q] = c01;
-    a.p[8][q] = c10;
-    a.p[9][q] = c11;
-    const float j = a.p[5][q] * (1.0f + dt * (c00 + c11));
-    a.p[5][q] = clampf(j, 1.0f / kJBound, kJBound);
+    const float dt = bits_f(params[DT]);
+    float x = p[0][q] + dt * u;
+    float y = p[1][q] + dt * w;
+    keep_inside(x, u, nx);
+    keep_inside(y, w, ny);
+    p[0][q] = x;
+    p[1][q] = y;
+    p[2][q] = u;
+    p[3][q] = w;
+    p[6][q] = c00;
+    p[7][q] = c01;
+    p[8][q] = c10;
+    p[9][q] = c11;
+    const float j = p[5][q] * (1.0f + dt * (c00 + c11));
+    p[5][q] = clampf(j, 1.0f / kJBound, kJBound);
+}
+
+__global__ __launch_bounds__(kBlock) void g2p2_kernel(Mpm2Args a) {
+    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (q >= a.count) return;
+    g2p2_particle(a.p, q, a.nx, a.ny, a.g[3], a.g[4], a.params);
+}
+
+// Scatter.cell as a function (Scatter.java:473-477).
+__device__ __forceinline__ int32_t cell2(float x, float y, int nx, int ny) {
+    const float cx = clampf(x, 0.0f, (float)nx - 1.0f);
+    const float cy = clampf(y, 0.0f, (float)ny - 1.0f);
+    return (int32_t)mn(cy, (float)ny - 2.0f) * (nx - 1) + (int32_t)mn(cx, (float)nx - 2.0f);
 }
 
 // Scatter.cell, Scatter.java:473-477
@@ -199,12 +239,7 @@ __global__ __launch_bounds__(kBlock) void keys2_kernel(Mpm2Args a, int32_t* __re
                                                        int32_t* __restrict__ index, int32_t pad_key) {
     const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (q >= a.capacity) return;
-    int32_t key = pad_key;
-    if (q < a.count) {
-        const float cx = clampf(a.p[0][q], 0.0f, (float)a.nx - 1.0f);
-        const float cy = clampf(a.p[1][q], 0.0f, (float)a.ny - 1.0f);
-        key = (int32_t)mn(cy, (float)a.ny - 2.0f) * (a.nx - 1) + (int32_t)mn(cx, (float)a.nx - 2.0f);
-    }
+    const int32_t key = q < a.count ? cell2(a.p[0][q], a.p[1][q], a.nx, a.ny) : pad_key;
     keys[q] = key;
     index[q] = (int32_t)q;
 }
@@ -221,6 +256,144 @@ __global__ __launch_bounds__(kBlock) void starts_kernel(const int32_t* __restric
         else hi = mid;
     }
     starts[c] = (int32_t)lo;
+}
+
+
+// ------------------------------------------------------------------ scenes in one launch
+//
+// A batch holds many 2D scenes in one set of buffers: scene s's particles at
+// [poff, poff + count) of every field (its range padded to kBatchAlign with m = 0), its nodes
+// at [goff, goff + nx*ny) of every grid field, its 28 params at params[28 s], its starts at
+// [soff, soff + cells + 1). Each kernel walks fixed chunks -- kBatchAlign particles, 256 nodes,
+// 256 starts -- and finds a chunk's scene in a table, so one launch steps every scene, and
+// each scene gets exactly the arithmetic a single-scene run gives it: the P2G is the
+// fixed-point one, whose integer sums do not depend on the order or grouping of the atomics.
+
+__device__ __forceinline__ void add_fixed(unsigned long long* p, long long v) {
+    atomicAdd(p, (unsigned long long)v);
+}
+
+// P2G in fixed point. A block takes a chunk of kBatchAlign particles of one scene; a thread
+// folds `run` consecutive particles sharing a stencil base into 27 integer sums and adds each
+// to the grid with one 64-bit atomic.
+__global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a, int run) {
+    const float mscale = 0x1p40f, pscale = 0x1p32f;
+    for (int64_t c = blockIdx.x; c < a.pchunks; c += gridDim.x) {
+        const Scene2 sc = a.scenes[a.pscene[c]];
+        const int32_t* params = a.params + 28 * (int64_t)a.pscene[c];
+        const float dt = bits_f(params[DT]), bulk = bits_f(params[BULK]), rho0 = bits_f(params[RHO0]);
+        unsigned long long* q0 = a.q[0] + sc.goff;
+        unsigned long long* q1 = a.q[1] + sc.goff;
+        unsigned long long* q2 = a.q[2] + sc.goff;
+        const int passes = kBatchAlign / (kBlock * run);
+        for (int pass = 0; pass < passes; ++pass) {
+            const int64_t first = c * kBatchAlign + ((int64_t)pass * kBlock + threadIdx.x) * run;
+            long long acc[27];
+#pragma unroll
+            for (int i = 0; i < 27; ++i) acc[i] = 0;
+            int64_t run_base = -1;
+            auto flush = [&](int64_t base) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    const int64_t node = base + (k / 3) * sc.nx + k % 3;
+                    add_fixed(q0 + node, acc[3 * k]);
+                    add_fixed(q1 + node, acc[3 * k + 1]);
+                    add_fixed(q2 + node, acc[3 * k + 2]);
+                }
+#pragma unroll
+                for (int i = 0; i < 27; ++i) acc[i] = 0;
+            };
+            for (int r = 0; r < run; ++r) {
+                const int64_t q = first + r;
+                if (q - sc.poff >= sc.count) break;
+                const Axis ax = axis(a.p[0][q], sc.nx), ay = axis(a.p[1][q], sc.ny);
+                const int64_t base = (int64_t)ay.cell * sc.nx + ax.cell;
+                if (base != run_base) {
+                    if (run_base >= 0) flush(run_base);
+                    run_base = base;
+                }
+                amounts(a.p, q, ax, ay, dt, bulk, rho0, [&](int k, float m, float mu, float mv) {
+                    acc[3 * k] += quantize(m, mscale);
+                    acc[3 * k + 1] += quantize(mu, pscale);
+                    acc[3 * k + 2] += quantize(mv, pscale);
+                });
+            }
+            if (run_base >= 0) flush(run_base);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void batch2_clear_kernel(Batch2Args a) {
+    const int64_t n = a.gchunks * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        a.q[0][i] = 0ull;
+        a.q[1][i] = 0ull;
+        a.q[2][i] = 0ull;
+    }
+}
+
+// The sums decoded into gm gmu gmv, then Flip.grid on them.
+__global__ __launch_bounds__(kBlock) void batch2_grid_kernel(Batch2Args a) {
+    for (int64_t c = blockIdx.x; c < a.gchunks; c += gridDim.x) {
+        const int s = a.gscene[c];
+        const Scene2 sc = a.scenes[s];
+        const int64_t node = c * kBlock + threadIdx.x - sc.goff;
+        if (node >= (int64_t)sc.nx * sc.ny) continue;
+        const int64_t at = sc.goff + node;
+        const float m = dequantize(a.q[0][at], kMassBits);
+        const float mu = dequantize(a.q[1][at], kMomentumBits);
+        const float mv = dequantize(a.q[2][at], kMomentumBits);
+        a.g[0][at] = m;
+        a.g[1][at] = mu;
+        a.g[2][at] = mv;
+        grid2_node(node, sc.nx, a.params + 28 * (int64_t)s, m, mu, mv, sc.ny, a.g[3] + sc.goff, a.g[4] + sc.goff);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void batch2_g2p_kernel(Batch2Args a) {
+    const int64_t chunks = a.pchunks * (kBatchAlign / kBlock);
+    for (int64_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        const int64_t q = c * kBlock + threadIdx.x;
+        const int s = a.pscene[q / kBatchAlign];
+        const Scene2 sc = a.scenes[s];
+        if (q - sc.poff >= sc.count) continue;
+        g2p2_particle(a.p, q, sc.nx, sc.ny, a.g[3] + sc.goff, a.g[4] + sc.goff, a.params + 28 * (int64_t)s);
+    }
+}
+
+// key = scene << cell_bits | Scatter.cell; a pad particle takes its scene's cell count, so it
+// sorts after the scene's particles and before the next scene's.
+__global__ __launch_bounds__(kBlock) void batch2_keys_kernel(Batch2Args a, int32_t* __restrict__ keys,
+                                                             int32_t* __restrict__ index) {
+    const int64_t chunks = a.pchunks * (kBatchAlign / kBlock);
+    for (int64_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        const int64_t q = c * kBlock + threadIdx.x;
+        const int s = a.pscene[q / kBatchAlign];
+        const Scene2 sc = a.scenes[s];
+        const int32_t cell = q - sc.poff < sc.count ? cell2(a.p[0][q], a.p[1][q], sc.nx, sc.ny) : sc.cells;
+        keys[q] = (int32_t)(((uint32_t)s << a.cell_bits) | (uint32_t)cell);
+        index[q] = (int32_t)q;
+    }
+}
+
+// starts[soff + c] = the first of the scene's sorted particles in cell c or after it, counted
+// from the scene's first particle (Scatter.cellStarts, Scatter.java:484-502).
+__global__ __launch_bounds__(kBlock) void batch2_starts_kernel(Batch2Args a, const int32_t* __restrict__ keys,
+                                                               int32_t* __restrict__ starts) {
+    for (int64_t c = blockIdx.x; c < a.schunks; c += gridDim.x) {
+        const int s = a.sscene[c];
+        const Scene2 sc = a.scenes[s];
+        const int64_t cell = c * kBlock + threadIdx.x - sc.soff;
+        if (cell > sc.cells) continue;
+        const int32_t target = (int32_t)(((uint32_t)s << a.cell_bits) | (uint32_t)cell);
+        int64_t lo = sc.poff, hi = sc.poff + sc.count;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (keys[mid] < target) lo = mid + 1;
+            else hi = mid;
+        }
+        starts[sc.soff + cell] = (int32_t)(lo - sc.poff);
+    }
 }
 
 inline unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
@@ -284,6 +457,69 @@ int mpm2_starts(const Launch& l, const int32_t* sorted_keys, int64_t count, int3
                        sorted_keys, count, cells, starts);
     timed_end(l, "starts2");
     return check_launch("starts2");
+}
+
+// ------------------------------------------------------------------ batch launchers
+
+namespace {
+unsigned grid_of(int64_t chunks, int blocks) {
+    if (chunks <= 0) return 0;
+    return (unsigned)(blocks > 0 && blocks < chunks ? blocks : chunks);
+}
+}  // namespace
+
+int batch2_clear(const Launch& l, const Batch2Args& a, int blocks) {
+    const unsigned g = grid_of(a.gchunks, blocks);
+    if (!g) return 0;
+    timed_begin(l, "batch2_clear");
+    hipLaunchKernelGGL(batch2_clear_kernel, dim3(g), dim3(kBlock), 0, l.stream, a);
+    timed_end(l, "batch2_clear");
+    return check_launch("batch2_clear");
+}
+
+int batch2_scatter(const Launch& l, const Batch2Args& a, int run, int blocks) {
+    const unsigned g = grid_of(a.pchunks, blocks);
+    if (!g) return 0;
+    timed_begin(l, "batch2_p2g");
+    hipLaunchKernelGGL(batch2_p2g_kernel, dim3(g), dim3(kBlock), 0, l.stream, a, run);
+    timed_end(l, "batch2_p2g");
+    return check_launch("batch2_p2g");
+}
+
+int batch2_grid(const Launch& l, const Batch2Args& a, int blocks) {
+    const unsigned g = grid_of(a.gchunks, blocks);
+    if (!g) return 0;
+    timed_begin(l, "batch2_grid");
+    hipLaunchKernelGGL(batch2_grid_kernel, dim3(g), dim3(kBlock), 0, l.stream, a);
+    timed_end(l, "batch2_grid");
+    return check_launch("batch2_grid");
+}
+
+int batch2_advect(const Launch& l, const Batch2Args& a, int blocks) {
+    const unsigned g = grid_of(a.pchunks * (kBatchAlign / kBlock), blocks);
+    if (!g) return 0;
+    timed_begin(l, "batch2_g2p");
+    hipLaunchKernelGGL(batch2_g2p_kernel, dim3(g), dim3(kBlock), 0, l.stream, a);
+    timed_end(l, "batch2_g2p");
+    return check_launch("batch2_g2p");
+}
+
+int batch2_keys(const Launch& l, const Batch2Args& a, int32_t* keys, int32_t* index, int blocks) {
+    const unsigned g = grid_of(a.pchunks * (kBatchAlign / kBlock), blocks);
+    if (!g) return 0;
+    timed_begin(l, "batch2_keys");
+    hipLaunchKernelGGL(batch2_keys_kernel, dim3(g), dim3(kBlock), 0, l.stream, a, keys, index);
+    timed_end(l, "batch2_keys");
+    return check_launch("batch2_keys");
+}
+
+int batch2_starts(const Launch& l, const Batch2Args& a, const int32_t* keys, int32_t* starts, int blocks) {
+    const unsigned g = grid_of(a.schunks, blocks);
+    if (!g) return 0;
+    timed_begin(l, "batch2_starts");
+    hipLaunchKernelGGL(batch2_starts_kernel, dim3(g), dim3(kBlock), 0, l.stream, a, keys, starts);
+    timed_end(l, "batch2_starts");
+    return check_launch("batch2_starts");
 }
 
 }  // namespace vxr

@@ -9,8 +9,10 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <map>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -50,6 +52,12 @@ const char* kGrid2[5] = {"gm", "gmu", "gmv", "gu", "gv"};
 constexpr int64_t kChunk = 2048;  // particle arrays are padded to whole chunks
 
 int64_t round_up(int64_t n, int64_t m) { return (n + m - 1) / m * m; }
+
+int bits_for(int64_t values) {  // bits to hold 0 .. values-1
+    int b = 0;
+    while ((1LL << b) < values) b++;
+    return b;
+}
 
 }  // namespace
 
@@ -147,6 +155,20 @@ struct vxr_sim {
     int32_t* clamped = nullptr;
     int64_t dispatched = 0;
 
+    // 2D scenes as a batch (VXR_KIND_MPM2_BATCH, or one scene whose P2G is the fixed-point one)
+    std::vector<vxr::Scene2> scenes;
+    vxr::Scene2* d_scenes = nullptr;
+    int32_t *pscene = nullptr, *gscene = nullptr, *sscene = nullptr;
+    int64_t pchunks = 0, gchunks = 0, schunks = 0;
+    unsigned long long* q[3] = {};
+    int cell_bits = 0, scene_bits = 0;
+    int64_t starts_len = 0;
+
+    // ParticleSimulation.advance's step counter, and the graphs it replays
+    int64_t steps = 0;
+    bool flipped = false;  // whether p holds the second set of particle buffers (each sort swaps)
+    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;  // (sort first, steps, flipped)
+
     std::map<std::string, int64_t> options;
 
     vxr::Launch launch() const { return vxr::Launch{stream, timer}; }
@@ -169,6 +191,25 @@ int alloc(vxr_sim* s, int64_t words, void** out) {
 
 // Resolves a buffer name to its current address and logical length.
 bool lookup(vxr_sim* s, const std::string& name, Buf* b) {
+    if (s->kind == VXR_KIND_MPM2_BATCH) {
+        for (int f = 0; f < 10; ++f) {
+            if (name == kFields2[f]) {
+                *b = Buf{s->p[f], s->capacity, s->capacity};
+                return true;
+            }
+        }
+        for (int f = 0; f < 5; ++f) {
+            if (name == kGrid2[f]) {
+                *b = Buf{s->g[f], s->nodes, s->nodes};
+                return true;
+            }
+        }
+        if (name == "params") *b = Buf{s->params, s->param_words, s->param_words};
+        else if (name == "keys") *b = Buf{s->keys, s->capacity, s->capacity};
+        else if (name == "starts") *b = Buf{s->starts, s->starts_len, s->starts_len};
+        else return false;
+        return true;
+    }
     if (s->kind == VXR_KIND_MPM3 || s->kind == VXR_KIND_MPM2) {
         const bool three = s->kind == VXR_KIND_MPM3;
         const char** fields = three ? kFields3 : kFields2;
@@ -241,7 +282,43 @@ vxr::Mpm2Args mpm2_args(vxr_sim* s) {
 
 int64_t opt(vxr_sim* s, const char* key) { return s->options[key]; }
 
+vxr::Batch2Args batch2_args(vxr_sim* s) {
+    vxr::Batch2Args a;
+    a.scenes = s->d_scenes;
+    a.pscene = s->pscene;
+    a.gscene = s->gscene;
+    a.sscene = s->sscene;
+    a.pchunks = s->pchunks;
+    a.gchunks = s->gchunks;
+    a.schunks = s->schunks;
+    for (int f = 0; f < 10; ++f) a.p[f] = s->p[f];
+    for (int f = 0; f < 5; ++f) a.g[f] = s->g[f];
+    for (int f = 0; f < 3; ++f) a.q[f] = s->q[f];
+    a.params = s->params;
+    a.cell_bits = s->cell_bits;
+    return a;
+}
+
+bool fixed_p2g(vxr_sim* s) {
+    return s->kind == VXR_KIND_MPM2_BATCH || (s->kind == VXR_KIND_MPM2 && s->options["p2g"] == 2);
+}
+
+int batch_sort_pass(vxr_sim* s) {
+    const vxr::Launch l = s->launch();
+    const vxr::Batch2Args a = batch2_args(s);
+    const int blocks = (int)opt(s, "blocks");
+    int rc = vxr::batch2_keys(l, a, s->keys, s->index, blocks);
+    if (!rc) rc = vxr::radix_sort(l, s->keys, s->index, s->capacity, s->cell_bits + s->scene_bits,
+                                  (int)opt(s, "radix_bits"), s->scratch);
+    if (!rc) rc = vxr::permute_fields(l, s->p.data(), s->p_alt.data(), 10, s->index, s->capacity);
+    if (rc) return rc;
+    std::swap(s->p, s->p_alt);
+    s->flipped = !s->flipped;
+    return vxr::batch2_starts(l, batch2_args(s), s->keys, s->starts, blocks);
+}
+
 int sort_pass(vxr_sim* s) {
+    if (s->kind == VXR_KIND_MPM2_BATCH) return batch_sort_pass(s);
     const vxr::Launch l = s->launch();
     const bool three = s->kind == VXR_KIND_MPM3;
     int rc = three ? vxr::mpm3_keys(l, mpm3_args(s), s->keys, s->index) : vxr::mpm2_keys(l, mpm2_args(s), s->keys, s->index);
@@ -257,6 +334,7 @@ int sort_pass(vxr_sim* s) {
     const int nf = three ? 17 : 10;
     if ((rc = vxr::permute_fields(l, s->p.data(), s->p_alt.data(), nf, s->index, s->count))) return rc;
     std::swap(s->p, s->p_alt);
+    s->flipped = !s->flipped;
     if (!three) {
         const int32_t cells = (s->nx - 1) * (s->ny - 1);
         if ((rc = vxr::mpm2_starts(l, s->keys, s->count, cells, s->starts))) return rc;
@@ -277,6 +355,14 @@ int run_pass(vxr_sim* s, const std::string& pass) {
             const int64_t g2p = opt(s, "g2p");
             return g2p == 0 ? vxr::mpm3_advect(l, a) : vxr::mpm3_advect_tiled(l, a, (int)opt(s, "run"), g2p == 2);
         }
+    } else if (fixed_p2g(s)) {
+        const vxr::Batch2Args b = batch2_args(s);
+        const int blocks = (int)opt(s, "blocks");
+        if (pass == "clear") return vxr::batch2_clear(l, b, blocks);
+        if (pass == "scatter") return vxr::batch2_scatter(l, b, (int)opt(s, "run"), blocks);
+        if (pass == "grid") return vxr::batch2_grid(l, b, blocks);
+        if (pass == "advect")
+            return s->kind == VXR_KIND_MPM2 ? vxr::mpm2_advect(l, mpm2_args(s)) : vxr::batch2_advect(l, b, blocks);
     } else if (s->kind == VXR_KIND_MPM2) {
         const vxr::Mpm2Args a = mpm2_args(s);
         if (pass == "clear") return vxr::mpm2_clear(l, a);
@@ -286,6 +372,177 @@ int run_pass(vxr_sim* s, const std::string& pass) {
         if (pass == "advect") return vxr::mpm2_advect(l, a);
     }
     return fail(VXR_EINVAL, "no pass named '%s' for this simulation", pass.c_str());
+}
+
+// ------------------------------------------------------------------ advance, graphs
+
+const char* const kStep[4] = {"clear", "scatter", "grid", "advect"};
+constexpr int kMaxSegment = 64;  // steps one captured graph holds at most
+
+// A segment: a sort when `sort`, then `n` steps.
+int run_segment(vxr_sim* s, bool sort, int n) {
+    if (sort)
+        if (int rc = sort_pass(s)) return rc;
+    for (int i = 0; i < n; ++i)
+        for (const char* p : kStep)
+            if (int rc = run_pass(s, p)) return rc;
+    return 0;
+}
+
+// With the "graph" option (and timing off) a segment is captured once into a hipGraph per
+// (sort, n, which set of particle buffers is current) and replayed from then on: the same
+// kernels with the same arguments, one launch for the lot. Every pass clears with kernels, so
+// no memset node is recorded.
+int replay_segment(vxr_sim* s, bool sort, int n) {
+    if (!opt(s, "graph") || s->timer) return run_segment(s, sort, n);
+    const auto key = std::make_tuple((int)sort, n, (int)s->flipped);
+    auto it = s->graphs.find(key);
+    if (it == s->graphs.end()) {
+        const std::vector<float*> p0 = s->p, alt0 = s->p_alt;
+        const bool flipped0 = s->flipped;
+        HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = run_segment(s, sort, n);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s->stream, &g);
+        // capture ran nothing: put back the buffer sets the sort swapped while recording
+        s->p = p0;
+        s->p_alt = alt0;
+        s->flipped = flipped0;
+        if (rc) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (ie != hipSuccess) return hip_fail(ie, "hipGraphInstantiate");
+        it = s->graphs.emplace(key, exec).first;
+    }
+    HIP_TRY(hipGraphLaunch(it->second, s->stream));
+    if (sort) {
+        std::swap(s->p, s->p_alt);
+        s->flipped = !s->flipped;
+    }
+    return 0;
+}
+
+void drop_graphs(vxr_sim* s) {
+    if (s->stream) hipStreamSynchronize(s->stream);
+    for (auto& g : s->graphs) hipGraphExecDestroy(g.second);
+    s->graphs.clear();
+}
+
+// ------------------------------------------------------------------ autotuning
+
+// Milliseconds of `reps` runs of `body`, best of two rounds, on the simulation's stream.
+template <class Body>
+int time_ms(vxr_sim* s, int reps, Body body, float* out) {
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    float best = 3.0e38f;
+    int rc = 0;
+    for (int round = 0; round < 2 && !rc; ++round) {
+        if (hipEventRecord(a, s->stream) != hipSuccess) rc = fail(VXR_EDEVICE, "hipEventRecord");
+        for (int r = 0; r < reps && !rc; ++r) rc = body();
+        if (!rc && hipEventRecord(b, s->stream) != hipSuccess) rc = fail(VXR_EDEVICE, "hipEventRecord");
+        if (!rc && hipEventSynchronize(b) != hipSuccess) rc = fail(VXR_EDEVICE, "hipEventSynchronize");
+        float ms = 0;
+        if (!rc) hipEventElapsedTime(&ms, a, b);
+        if (!rc && ms < best) best = ms;
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    *out = best;
+    return rc;
+}
+
+// Picks the fastest value of `key` among `values` for `body`, leaving the option set to it.
+template <class Body>
+int tune_option(vxr_sim* s, const char* key, std::initializer_list<int64_t> values, int reps, Body body) {
+    int64_t best_v = opt(s, key);
+    float best_ms = 3.0e38f;
+    for (int64_t v : values) {
+        s->options[key] = v;
+        float ms = 0;
+        if (int rc = time_ms(s, reps, body, &ms)) return rc;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best_v = v;
+        }
+    }
+    s->options[key] = best_v;
+    return 0;
+}
+
+// Times passes that leave the simulation as it was: clear + scatter + grid (they rewrite the
+// grid from the particles), the sort's keys and radix passes on the key scratch (the particles
+// keep their order), and for a patch the step into the spare slot, its clamp count restored.
+int autotune(vxr_sim* s, int reps) {
+    const vxr::Launch l = s->launch();
+    auto scatter = [&]() -> int {
+        for (const char* p : {"clear", "scatter", "grid"})
+            if (int rc = run_pass(s, p)) return rc;
+        return 0;
+    };
+    auto keys_and_sort = [&]() -> int {
+        int rc = 0;
+        int bits = 0;
+        int64_t n = s->count;
+        if (s->kind == VXR_KIND_MPM2_BATCH) {
+            rc = vxr::batch2_keys(l, batch2_args(s), s->keys, s->index, (int)opt(s, "blocks"));
+            bits = s->cell_bits + s->scene_bits;
+            n = s->capacity;
+        } else if (s->kind == VXR_KIND_MPM3) {
+            rc = vxr::mpm3_keys(l, mpm3_args(s), s->keys, s->index);
+            bits = vxr::mpm3_key_bits(s->nx, s->ny, s->nz);
+        } else {
+            rc = vxr::mpm2_keys(l, mpm2_args(s), s->keys, s->index);
+            bits = bits_for((int64_t)(s->nx - 1) * (s->ny - 1) + 1);
+        }
+        if (rc) return rc;
+        return vxr::radix_sort(l, s->keys, s->index, n, bits, (int)opt(s, "radix_bits"), s->scratch);
+    };
+    if (s->kind == VXR_KIND_SWE) {
+        int32_t clamped = 0;
+        HIP_TRY(hipMemcpy(&clamped, s->clamped, 4, hipMemcpyDeviceToHost));
+        vxr::SweArgs a;
+        a.nx = s->nx;
+        a.ny = s->ny;
+        a.edges[0] = (s->flags & VXR_FLAG_OPEN_WEST) ? 1 : 0;
+        a.edges[1] = (s->flags & VXR_FLAG_OPEN_EAST) ? 1 : 0;
+        a.edges[2] = (s->flags & VXR_FLAG_OPEN_SOUTH) ? 1 : 0;
+        a.edges[3] = (s->flags & VXR_FLAG_OPEN_NORTH) ? 1 : 0;
+        a.params = s->params;
+        a.clamped = s->clamped;
+        const int64_t k = s->dispatched;
+        for (int f = 0; f < 3; ++f) {
+            a.in[f] = s->state[k % 2][f];
+            a.out[f] = s->state[(k + 1) % 2][f];
+        }
+        a.speed_in = s->speed[k % 3];
+        a.speed_out = s->speed[(k + 1) % 3];
+        a.speed_clear = s->speed[(k + 2) % 3];
+        a.budget_in = s->budget[k % 2];
+        a.budget_out = s->budget[(k + 1) % 2];
+        int rc = tune_option(s, "rows", {8, 16, 32, 64, 128}, reps,
+                             [&]() { return vxr::swe_step(l, a, (int)opt(s, "rows")); });
+        HIP_TRY(hipMemcpy(s->clamped, &clamped, 4, hipMemcpyHostToDevice));
+        return rc;
+    }
+    int rc = 0;
+    if (opt(s, "p2g") >= 1) rc = tune_option(s, "run", {1, 2, 4}, reps, scatter);
+    if (!rc && fixed_p2g(s))
+        rc = tune_option(s, "blocks", {0, 1024, 2048, 4096, 8192}, reps, [&]() {
+            for (const char* p : kStep)
+                if (p != std::string("advect"))
+                    if (int r = run_pass(s, p)) return r;
+            return 0;
+        });
+    if (!rc) rc = tune_option(s, "radix_bits", {4, 5, 6, 7, 8}, reps, keys_and_sort);
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return rc;
 }
 
 int clear_all(vxr_sim* s) {
@@ -306,10 +563,9 @@ int clear_all(vxr_sim* s) {
         HIP_TRY(hipMemsetAsync(s->params, 0, s->param_words * 4, s->stream));
         HIP_TRY(hipMemsetAsync(s->keys, 0, s->capacity * 4, s->stream));
         HIP_TRY(hipMemsetAsync(s->index, 0, s->capacity * 4, s->stream));
-        if (s->starts) {
-            const int64_t cells = (int64_t)(s->nx - 1) * (s->ny - 1);
-            HIP_TRY(hipMemsetAsync(s->starts, 0, (cells + 1) * 4, s->stream));
-        }
+        if (s->starts) HIP_TRY(hipMemsetAsync(s->starts, 0, s->starts_len * 4, s->stream));
+        for (unsigned long long* qf : s->q)
+            if (qf) HIP_TRY(hipMemsetAsync(qf, 0, s->gchunks * 256 * 8, s->stream));
     }
     HIP_TRY(hipStreamSynchronize(s->stream));
     return 0;
@@ -318,6 +574,7 @@ int clear_all(vxr_sim* s) {
 void destroy(vxr_sim* s) {
     if (!s) return;
     if (s->stream) hipStreamSynchronize(s->stream);
+    for (auto& g : s->graphs) hipGraphExecDestroy(g.second);
     delete s->timer;
     for (void* ptr : s->allocations) hipFree(ptr);
     if (s->stream) hipStreamDestroy(s->stream);
@@ -329,8 +586,66 @@ struct OptionSpec {
     int64_t lo, hi, def;
 };
 const OptionSpec kOptions[] = {
-    {"p2g", 0, 1, 1}, {"g2p", 0, 2, 2}, {"run", 1, 4, 4}, {"radix_bits", 4, 8, 8}, {"rows", 4, 256, 32},
+    {"p2g", 0, 2, 1},   {"g2p", 0, 2, 2},        {"run", 1, 4, 4},   {"radix_bits", 4, 8, 8},
+    {"rows", 4, 256, 32}, {"blocks", 0, 1 << 20, 0}, {"graph", 0, 1, 0},
 };
+
+
+
+// Lays scenes out (particles padded to kBatchAlign, nodes and starts to 256) and fills the
+// chunk counts; the device tables come later, in alloc_batch.
+void layout_batch(vxr_sim* s) {
+    int64_t po = 0, go = 0, so = 0;
+    int64_t max_cells = 0;
+    for (vxr::Scene2& sc : s->scenes) {
+        sc.cells = (sc.nx - 1) * (sc.ny - 1);
+        sc.poff = po;
+        sc.goff = go;
+        sc.soff = so;
+        po += round_up(sc.count, vxr::kBatchAlign);
+        go += round_up((int64_t)sc.nx * sc.ny, 256);
+        so += round_up((int64_t)sc.cells + 1, 256);
+        max_cells = std::max<int64_t>(max_cells, sc.cells);
+    }
+    s->pchunks = po / vxr::kBatchAlign;
+    s->gchunks = go / 256;
+    s->schunks = so / 256;
+    s->cell_bits = bits_for(max_cells + 1);
+    s->scene_bits = bits_for((int64_t)s->scenes.size());
+}
+
+int alloc_batch(vxr_sim* s) {
+    void* ptr = nullptr;
+    const int64_t nscenes = (int64_t)s->scenes.size();
+    int rc = alloc(s, nscenes * (int64_t)(sizeof(vxr::Scene2) / 4), &ptr);
+    if (rc) return rc;
+    s->d_scenes = (vxr::Scene2*)ptr;
+    std::vector<int32_t> pt(s->pchunks), gt(s->gchunks), st(s->schunks);
+    for (int64_t k = 0; k < nscenes; ++k) {
+        const vxr::Scene2& sc = s->scenes[k];
+        const int64_t p1 = k + 1 < nscenes ? s->scenes[k + 1].poff : s->pchunks * vxr::kBatchAlign;
+        const int64_t g1 = k + 1 < nscenes ? s->scenes[k + 1].goff : s->gchunks * 256;
+        const int64_t s1 = k + 1 < nscenes ? s->scenes[k + 1].soff : s->schunks * 256;
+        for (int64_t c = sc.poff / vxr::kBatchAlign; c < p1 / vxr::kBatchAlign; ++c) pt[c] = (int32_t)k;
+        for (int64_t c = sc.goff / 256; c < g1 / 256; ++c) gt[c] = (int32_t)k;
+        for (int64_t c = sc.soff / 256; c < s1 / 256; ++c) st[c] = (int32_t)k;
+    }
+    if ((rc = alloc(s, std::max<int64_t>(s->pchunks, 1), &ptr))) return rc;
+    s->pscene = (int32_t*)ptr;
+    if ((rc = alloc(s, std::max<int64_t>(s->gchunks, 1), &ptr))) return rc;
+    s->gscene = (int32_t*)ptr;
+    if ((rc = alloc(s, std::max<int64_t>(s->schunks, 1), &ptr))) return rc;
+    s->sscene = (int32_t*)ptr;
+    for (int f = 0; f < 3; ++f) {
+        if ((rc = alloc(s, s->gchunks * 256 * 2, &ptr))) return rc;
+        s->q[f] = (unsigned long long*)ptr;
+    }
+    HIP_TRY(hipMemcpy(s->d_scenes, s->scenes.data(), nscenes * sizeof(vxr::Scene2), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->pscene, pt.data(), pt.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->gscene, gt.data(), gt.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->sscene, st.data(), st.size() * 4, hipMemcpyHostToDevice));
+    return 0;
+}
 
 }  // namespace
 
@@ -361,6 +676,10 @@ int vxr_create(int kind, const int32_t dims[3], int64_t particles, uint32_t flag
     s->flags = flags;
     for (const OptionSpec& o : kOptions) s->options[o.key] = o.def;
     int rc = 0;
+    if (kind == VXR_KIND_MPM2_BATCH) {
+        delete s;
+        return fail(VXR_EINVAL, "a batch of scenes is made by vxr_create_batch");
+    }
     if (kind == VXR_KIND_SWE) {
         if (dims[0] < 1 || dims[1] < 1) {
             delete s;
@@ -437,7 +756,19 @@ int vxr_create(int kind, const int32_t dims[3], int64_t particles, uint32_t flag
         if (!rc) rc = alloc(s, s->capacity, &ptr), s->keys = (int32_t*)ptr;
         if (!rc) rc = alloc(s, s->capacity, &ptr), s->index = (int32_t*)ptr;
         if (!rc) rc = alloc(s, vxr::radix_scratch_words(s->capacity, 8), &ptr), s->scratch = (int32_t*)ptr;
-        if (!rc && !three) rc = alloc(s, (int64_t)(s->nx - 1) * (s->ny - 1) + 1, &ptr), s->starts = (int32_t*)ptr;
+        if (!rc && !three) {
+            s->starts_len = (int64_t)(s->nx - 1) * (s->ny - 1) + 1;
+            rc = alloc(s, s->starts_len, &ptr), s->starts = (int32_t*)ptr;
+        }
+        if (!rc && !three) {
+            vxr::Scene2 one{};
+            one.nx = s->nx;
+            one.ny = s->ny;
+            one.count = s->count;
+            s->scenes.push_back(one);
+            layout_batch(s);
+            rc = alloc_batch(s);
+        }
     }
     if (!rc) rc = clear_all(s);
     if (rc) {
@@ -445,6 +776,100 @@ int vxr_create(int kind, const int32_t dims[3], int64_t particles, uint32_t flag
         return rc;
     }
     *out = s;
+    return VXR_OK;
+}
+
+int vxr_create_batch(int nscenes, const int32_t* nx, const int32_t* ny, const int64_t* particles, uint32_t flags,
+                     vxr_sim** out) {
+    if (!out || !nx || !ny || !particles) return fail(VXR_EINVAL, "null argument");
+    *out = nullptr;
+    if (nscenes < 1) return fail(VXR_EINVAL, "a batch needs at least one scene, got %d", nscenes);
+    const uint32_t physics = VXR_FLAG_TENSION | VXR_FLAG_HEAT | VXR_FLAG_CONVECTION | VXR_FLAG_RELAX | VXR_FLAG_PUMP |
+                             VXR_FLAG_PLASTIC;
+    if (flags & physics)
+        return fail(VXR_EUNSUPPORTED, "tension, heat, convection, relax, pump and plastic are not built");
+    if (flags) return fail(VXR_EINVAL, "unknown flags 0x%x", flags);
+    vxr_sim* s = new vxr_sim;
+    s->kind = VXR_KIND_MPM2_BATCH;
+    for (const OptionSpec& o : kOptions) s->options[o.key] = o.def;
+    s->options["p2g"] = 2;
+    for (int k = 0; k < nscenes; ++k) {
+        if (nx[k] < 4 || ny[k] < 4 || particles[k] < 1) {
+            delete s;
+            return fail(VXR_EINVAL, "scene %d: a grid needs at least four nodes each way and a step particles, got "
+                        "%d x %d and %lld", k, nx[k], ny[k], (long long)particles[k]);
+        }
+        if ((int64_t)nx[k] * ny[k] > (1LL << 28)) {
+            delete s;
+            return fail(VXR_EINVAL, "scene %d: grid of %d x %d is too large", k, nx[k], ny[k]);
+        }
+        vxr::Scene2 sc{};
+        sc.nx = nx[k];
+        sc.ny = ny[k];
+        sc.count = particles[k];
+        s->scenes.push_back(sc);
+    }
+    layout_batch(s);
+    s->capacity = s->pchunks * vxr::kBatchAlign;
+    for (const vxr::Scene2& sc : s->scenes) s->count += sc.count;
+    s->nodes = s->gchunks * 256;
+    s->starts_len = s->schunks * 256;
+    s->param_words = 28LL * nscenes;
+    if (s->capacity > (int64_t)INT32_MAX - vxr::kBatchAlign || s->cell_bits + s->scene_bits > 31) {
+        const int64_t cap = s->capacity;
+        const int bits = s->cell_bits + s->scene_bits;
+        delete s;
+        return fail(VXR_EINVAL, "batch too large: %lld particle slots, %d key bits", (long long)cap, bits);
+    }
+    hipError_t e = hipStreamCreate(&s->stream);
+    if (e != hipSuccess) {
+        delete s;
+        return hip_fail(e, "hipStreamCreate");
+    }
+    int rc = 0;
+    void* ptr = nullptr;
+    for (int f = 0; f < 10 && !rc; ++f) {
+        rc = alloc(s, s->capacity, &ptr);
+        s->p.push_back((float*)ptr);
+        if (!rc) rc = alloc(s, s->capacity, &ptr), s->p_alt.push_back((float*)ptr);
+    }
+    for (int f = 0; f < 5 && !rc; ++f) {
+        rc = alloc(s, s->nodes, &ptr);
+        s->g.push_back((float*)ptr);
+    }
+    if (!rc) rc = alloc(s, s->param_words, &ptr), s->params = (int32_t*)ptr;
+    if (!rc) rc = alloc(s, s->capacity, &ptr), s->keys = (int32_t*)ptr;
+    if (!rc) rc = alloc(s, s->capacity, &ptr), s->index = (int32_t*)ptr;
+    if (!rc) rc = alloc(s, vxr::radix_scratch_words(s->capacity, 8), &ptr), s->scratch = (int32_t*)ptr;
+    if (!rc) rc = alloc(s, s->starts_len, &ptr), s->starts = (int32_t*)ptr;
+    if (!rc) rc = alloc_batch(s);
+    if (!rc) rc = clear_all(s);
+    if (rc) {
+        destroy(s);
+        return rc;
+    }
+    *out = s;
+    return VXR_OK;
+}
+
+int vxr_batch_layout(vxr_sim* s, int64_t* particle_offsets, int64_t* node_offsets, int64_t* start_offsets) {
+    if (!s || !particle_offsets || !node_offsets || !start_offsets) return fail(VXR_EINVAL, "null argument");
+    if (s->kind != VXR_KIND_MPM2_BATCH) return fail(VXR_ESTATE, "vxr_batch_layout is for a batch of scenes");
+    const size_t n = s->scenes.size();
+    for (size_t k = 0; k < n; ++k) {
+        particle_offsets[k] = s->scenes[k].poff;
+        node_offsets[k] = s->scenes[k].goff;
+        start_offsets[k] = s->scenes[k].soff;
+    }
+    particle_offsets[n] = s->pchunks * vxr::kBatchAlign;
+    node_offsets[n] = s->gchunks * 256;
+    start_offsets[n] = s->schunks * 256;
+    return VXR_OK;
+}
+
+int vxr_scene_count(vxr_sim* s, int* n) {
+    if (!s || !n) return fail(VXR_EINVAL, "null argument");
+    *n = s->kind == VXR_KIND_MPM2_BATCH ? (int)s->scenes.size() : 1;
     return VXR_OK;
 }
 
@@ -508,6 +933,33 @@ int vxr_run_pass(vxr_sim* s, const char* pass, int repeat) {
     for (int r = 0; r < repeat; ++r)
         if (int rc = run_pass(s, pass)) return rc;
     return VXR_OK;
+}
+
+int vxr_advance(vxr_sim* s, int count, int sort_every, int64_t first_step) {
+    if (!s) return fail(VXR_EINVAL, "null simulation");
+    if (s->kind == VXR_KIND_SWE) return fail(VXR_ESTATE, "a shallow-water patch steps by vxr_swe_dispatch");
+    if (count < 0 || sort_every < 0 || first_step < 0)
+        return fail(VXR_EINVAL, "count, sort_every and first_step must not be negative");
+    int64_t step = first_step;
+    int left = count;
+    while (left > 0) {
+        const bool sort = sort_every > 0 && step % sort_every == 0;
+        int64_t n = left;
+        if (sort_every > 0) n = std::min<int64_t>(n, sort_every - step % sort_every);
+        n = std::min<int64_t>(n, kMaxSegment);
+        if (int rc = replay_segment(s, sort, (int)n)) return rc;
+        step += n;
+        left -= (int)n;
+    }
+    return VXR_OK;
+}
+
+int vxr_autotune(vxr_sim* s, int reps) {
+    if (!s) return fail(VXR_EINVAL, "null simulation");
+    if (reps < 1) return fail(VXR_EINVAL, "reps must be positive, got %d", reps);
+    if (s->timer) return fail(VXR_ESTATE, "turn timing off before tuning");
+    drop_graphs(s);
+    return autotune(s, reps);
 }
 
 int vxr_swe_dispatch(vxr_sim* s, int count) {
@@ -575,6 +1027,11 @@ int vxr_set_option(vxr_sim* s, const char* key, int64_t value) {
                             (long long)value);
             if (std::string(key) == "run" && value == 3)
                 return fail(VXR_EINVAL, "option 'run' takes 1, 2 or 4, got 3");
+            if (std::string(key) == "p2g" && value == 2 && s->kind != VXR_KIND_MPM2 && s->kind != VXR_KIND_MPM2_BATCH)
+                return fail(VXR_EINVAL, "p2g 2 (fixed point) is built for 2D scenes");
+            if (std::string(key) == "p2g" && value != 2 && s->kind == VXR_KIND_MPM2_BATCH)
+                return fail(VXR_EINVAL, "a batch of scenes always scatters in fixed point (p2g 2)");
+            drop_graphs(s);
             s->options[key] = value;
             return VXR_OK;
         }

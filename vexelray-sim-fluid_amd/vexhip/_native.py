@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libvexelhip.so")
 
 OK, EINVAL, ESTATE, EDEVICE, ENOMEM, EUNSUPPORTED = range(6)
-KIND_MPM2, KIND_MPM3, KIND_SWE = 1, 2, 3
+KIND_MPM2, KIND_MPM3, KIND_SWE, KIND_MPM2_BATCH = 1, 2, 3, 4
 GROUP_STEP, GROUP_SORT = 0, 1
 FLAG_TENSION, FLAG_HEAT, FLAG_CONVECTION, FLAG_RELAX, FLAG_PUMP, FLAG_PLASTIC = (1 << k for k in range(6))
 FLAG_OPEN_WEST, FLAG_OPEN_EAST, FLAG_OPEN_SOUTH, FLAG_OPEN_NORTH = (1 << k for k in range(16, 20))
@@ -67,6 +67,12 @@ def lib() -> ctypes.CDLL:
             "vxr_get_option": (ctypes.c_int, [vp, cs, ctypes.POINTER(i64)]),
             "vxr_timing": (ctypes.c_int, [vp, ctypes.c_int]),
             "vxr_kernel_time": (ctypes.c_int, [vp, cs, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]),
+            "vxr_create_batch": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                                ctypes.POINTER(i64), ctypes.c_uint32, ctypes.POINTER(vp)]),
+            "vxr_batch_layout": (ctypes.c_int, [vp, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i64)]),
+            "vxr_scene_count": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int)]),
+            "vxr_advance": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, i64]),
+            "vxr_autotune": (ctypes.c_int, [vp, ctypes.c_int]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(l, name)
@@ -98,6 +104,38 @@ class Sim:
         h = ctypes.c_void_p()
         check(lib().vxr_create(kind, d, int(particles), int(flags), ctypes.byref(h)))
         self.h = h
+
+    @classmethod
+    def batch(cls, nx, ny, particles, flags: int = 0) -> "Sim":
+        """A batch of 2D scenes (vxr_create_batch): scene k is nx[k] x ny[k] with particles[k]."""
+        n = len(nx)
+        if not (len(ny) == n and len(particles) == n):
+            raise ValueError("nx, ny and particles need one entry per scene")
+        h = ctypes.c_void_p()
+        check(lib().vxr_create_batch(n, (ctypes.c_int32 * n)(*map(int, nx)), (ctypes.c_int32 * n)(*map(int, ny)),
+                                     (ctypes.c_int64 * n)(*map(int, particles)), int(flags), ctypes.byref(h)))
+        self = cls.__new__(cls)
+        self.h = h
+        return self
+
+    def scene_count(self) -> int:
+        n = ctypes.c_int()
+        check(lib().vxr_scene_count(self.h, ctypes.byref(n)))
+        return n.value
+
+    def batch_layout(self):
+        """(particle, node, start) offsets of each scene, scenes + 1 entries each."""
+        n = self.scene_count() + 1
+        po, go, so = (ctypes.c_int64 * n)(), (ctypes.c_int64 * n)(), (ctypes.c_int64 * n)()
+        check(lib().vxr_batch_layout(self.h, po, go, so))
+        return np.array(po[:], np.int64), np.array(go[:], np.int64), np.array(so[:], np.int64)
+
+    def advance(self, count: int, sort_every: int, first_step: int) -> None:
+        check(lib().vxr_advance(self.h, int(count), int(sort_every), int(first_step)))
+
+    def autotune(self, reps: int = 3) -> dict:
+        check(lib().vxr_autotune(self.h, int(reps)))
+        return {k: self.option(k) for k in ("p2g", "run", "radix_bits", "rows", "blocks")}
 
     def close(self) -> None:
         if self.h:

@@ -68,16 +68,8 @@ class _Runner:
         self.sim.close()
 
     def _advance(self, count: int, sort_every: int | None) -> None:
-        k = 0
-        while k < count:
-            if sort_every and self._steps % sort_every == 0:
-                self.sim.run(_native.GROUP_SORT)
-            run = count - k
-            if sort_every:
-                run = min(run, sort_every - self._steps % sort_every)
-            self.sim.run(_native.GROUP_STEP, run)
-            self._steps += run
-            k += run
+        self.sim.advance(count, sort_every or 0, self._steps)
+        self._steps += count
 
 
 class ParticleSimulation(_Runner):
@@ -203,3 +195,81 @@ class ParticleSimulation3(_Runner):
 
     def active(self) -> int:
         return int(np.count_nonzero(self._read("m") > 0))
+
+
+class ParticleBatch:
+    """Many 2D scenes stepped together: one vxr batch (vxr_create_batch), one launch per pass for
+    every scene. Scene k is what a ParticleSimulation of the same size would hold, and steps to
+    the same bits as one with the fixed-point P2G (option p2g = 2). No reference counterpart: the
+    reference steps one gui/ParticleSimulation per scene."""
+
+    def __init__(self, sizes, particles):
+        """``sizes``: (nx, ny) per scene; ``particles``: the particle count per scene."""
+        nx = [int(s[0]) for s in sizes]
+        ny = [int(s[1]) for s in sizes]
+        self.sim = _native.Sim.batch(nx, ny, particles)
+        self._sizes = list(zip(nx, ny))
+        self._counts = [int(c) for c in particles]
+        self.poff, self.goff, self.soff = self.sim.batch_layout()
+        self._steps = 0
+
+    def scenes(self) -> int:
+        return len(self._counts)
+
+    def on_gpu(self) -> bool:
+        return True
+
+    def _gather(self, per_scene, offsets, length, fill=0.0):
+        out = np.full(length, fill, np.float32)
+        for k, values in enumerate(per_scene):
+            v = np.asarray(values, np.float32)
+            out[offsets[k]:offsets[k] + len(v)] = v
+        return out
+
+    def load(self, scenes, params) -> None:
+        """``scenes[k]``: dict with x, y and m (u, v optional), node units; ``params[k]``: the 28
+        Flip.params words of scene k. J = 1, C = 0, as ParticleSimulation.load sets them."""
+        if len(scenes) != self.scenes() or len(params) != self.scenes():
+            raise ValueError(f"{self.scenes()} scenes and their params, got {len(scenes)} and {len(params)}")
+        cap = int(self.poff[-1])
+        for k, sc in enumerate(scenes):
+            if len(sc["x"]) != self._counts[k]:
+                raise ValueError(f"scene {k} holds {self._counts[k]} particles, got {len(sc['x'])}")
+        for name in ("x", "y", "u", "v", "m"):
+            self.sim.write(name, self._gather([sc.get(name, np.zeros(len(sc["x"]))) for sc in scenes],
+                                              self.poff, cap))
+        self.sim.write("j", self._gather([np.ones(c) for c in self._counts], self.poff, cap))
+        zero = np.zeros(cap, np.float32)
+        for c in AFFINE2:
+            self.sim.write(c, zero)
+        self.sim.write("params", np.concatenate([np.asarray(w, np.int32) for w in params]))
+        self._steps = 0
+
+    def advance(self, count: int, sort_every: int) -> None:
+        """ParticleSimulation.advance for every scene at once."""
+        self.sim.advance(count, sort_every, self._steps)
+        self._steps += count
+
+    def steps(self) -> int:
+        return self._steps
+
+    def scene(self, k: int, names=("x", "y", "u", "v", "m", "j", "c00", "c01", "c10", "c11")) -> dict:
+        """Scene k's particle fields."""
+        lo, n = int(self.poff[k]), self._counts[k]
+        return {name: self.sim.read(name)[lo:lo + n].copy() for name in names}
+
+    def scene_grid(self, k: int, names=("gm", "gmu", "gmv", "gu", "gv")) -> dict:
+        nx, ny = self._sizes[k]
+        lo = int(self.goff[k])
+        return {name: self.sim.read(name)[lo:lo + nx * ny].copy() for name in names}
+
+    def scene_starts(self, k: int) -> np.ndarray:
+        nx, ny = self._sizes[k]
+        lo = int(self.soff[k])
+        return self.sim.read_words("starts")[lo:lo + (nx - 1) * (ny - 1) + 1].copy()
+
+    def finish(self) -> None:
+        self.sim.finish()
+
+    def close(self) -> None:
+        self.sim.close()

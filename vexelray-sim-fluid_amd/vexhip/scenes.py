@@ -163,6 +163,41 @@ def mpm2_dam_block(n=128, width=32, height=64, ppc=4, seed=71):
     return {"x": x, "y": y, "m": m}
 
 
+def mpm2_batch_layouts(scenes=256, seed=3, lo=4096, hi=262144, ppc=4, sizes=(64, 128, 256, 512)):
+    """The batched 2D workload: ``scenes`` scenes with particle counts log-uniform in [lo, hi]
+    and a grid drawn from ``sizes`` among those whose lower half holds the scene's block of
+    water at ``ppc`` a cell. Returns a list of dicts: n, count, origin, block (w, h)."""
+    r = JavaRandom(seed)
+    out = []
+    for _ in range(scenes):
+        count = int(round(math.exp(math.log(lo) + r.next_double() * (math.log(hi) - math.log(lo)))))
+        cells = -(-count // ppc)
+        w = max(4, math.ceil(math.sqrt(cells / 2.0)))
+        h = -(-cells // w)
+        fits = [n for n in sizes if 1 + w <= n - 2 and 1 + h <= (n - 2)]
+        n = fits[r.next_int(len(fits))]
+        out.append({"n": n, "count": count, "origin": (1, 1), "block": (w, h)})
+    return out
+
+
+def mpm2_batch_scene(layout, seed):
+    """One scene of the batch: FlipTest.column's jittered fill of the layout's block, stopped at
+    its count, mass rho0 / ppc with rho0 = 1 (4 a cell), and Flip.params words in node units
+    (h = 1/(n-1): g = -9.8/h, B = 10/h^2, dt scaled so the acoustic Courant number is about 0.04
+    on every grid, 1e-4 at 128^2)."""
+    n, count = layout["n"], layout["count"]
+    w, h = layout["block"]
+    x, y = jittered_block2(layout["origin"], (w, h), 4, JavaRandomFloats(seed))
+    x, y = x[:count].copy(), y[:count].copy()
+    m = np.full(count, np.float32(0.25), np.float32)
+    spacing = 1.0 / (n - 1)
+    dt = 1e-4 * 127.0 / (n - 1)
+    words = np.zeros(28, np.int32)
+    words[:6] = np.array([dt, 0.0, -9.8 / spacing, 10.0 / spacing ** 2, 1.0, 0.0],
+                         np.float64).astype(np.float32).view(np.int32)
+    return {"x": x, "y": y, "m": m}, words
+
+
 # ------------------------------------------------------------------ shallow water
 
 def swe_dam_bumps(n=4096, bumps=32, seed=0, radius=0.2):
