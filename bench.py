@@ -31,16 +31,20 @@ BYTES = {
     "keys3": ("particle", 20), "radix_hist": ("element", 4), "radix_scatter": ("element", 16),
     "radix_scan": ("none", 0), "permute": ("particle", 140),
     "hll_step": ("cell", 24),
+    # batched 2D: 10 particle fields in for P2G; x y j in and 9 fields out for G2P; three 64-bit
+    # sums in and five floats out per node for the grid update; three 64-bit sums cleared per node
+    "batch2_p2g": ("particle", 40), "batch2_g2p": ("particle", 48), "batch2_grid": ("node", 44),
+    "batch2_clear": ("node", 24), "batch2_keys": ("particle", 16), "batch2_starts": ("none", 0),
 }
 
 
-def kernel_table(sim, names, units):
+def kernel_table(sim, names, units, bytes_override=None):
     out = {}
     for k in names:
         ms, n = sim.kernel_time(k)
         if n == 0:
             continue
-        unit, per = BYTES.get(k, ("none", 0))
+        unit, per = (bytes_override or {}).get(k, BYTES.get(k, ("none", 0)))
         count = units.get(unit, 0)
         avg_ms = ms / n
         gbs = per * count / (avg_ms * 1e-3) / 1e9 if per and avg_ms > 0 else None
@@ -96,6 +100,72 @@ def bench_hll(steps, n=4096):
     table = kernel_table(sim.sim, ["hll_step"], {"cell": n * n})
     sim.close()
     return n * n * steps / (t1 - t0), table
+
+
+def bench_batch2(steps, scenes=256, sort_every=SORT_EVERY, graph=True, tune=True):
+    """The batched 2D workload: 256 seeded scenes (counts log-uniform in [4096, 262144], grids of
+    64^2 .. 512^2), every scene stepped by one launch per pass; particle-substeps/s over all."""
+    from vexhip import particles, scenes as sc_mod
+
+    layouts = sc_mod.mpm2_batch_layouts(scenes)
+    loaded = [sc_mod.mpm2_batch_scene(l, 1000 + k) for k, l in enumerate(layouts)]
+    sizes = [(l["n"], l["n"]) for l in layouts]
+    counts = [l["count"] for l in layouts]
+    batch = particles.ParticleBatch(sizes, counts)
+    batch.load([x[0] for x in loaded], [x[1] for x in loaded])
+    chosen = batch.sim.autotune(2) if tune else None
+    batch.sim.set_option("graph", 1 if graph else 0)
+    batch.advance(sort_every, sort_every)  # warm: builds the graphs
+    batch.finish()
+    t0 = time.perf_counter()
+    batch.advance(steps, sort_every)
+    batch.finish()
+    t1 = time.perf_counter()
+    total = sum(counts)
+    # per-kernel times from an eager stretch (graphs are replayed only with timing off)
+    batch.sim.timing(True)
+    batch.advance(2 * sort_every, sort_every)
+    names = ["batch2_clear", "batch2_p2g", "batch2_grid", "batch2_g2p", "batch2_keys", "radix_hist", "radix_scan",
+             "radix_scatter", "permute", "batch2_starts"]
+    nodes = sum(n * n for n, _ in sizes)
+    # the 2D permutation moves 10 fields: index + 10 gathers + 10 stores
+    table = kernel_table(batch.sim, names, {"particle": total, "node": nodes, "element": int(batch.poff[-1])},
+                         {"permute": ("element", 84)})
+    batch.sim.timing(False)
+    grid = batch.sim.read("gm")
+    mass = float(np.sum(grid, dtype=np.float64)) / (0.25 * total) - 1.0
+    batch.close()
+    return {"value": round(total * steps / (t1 - t0), 1), "unit": "particle-substeps/s", "steps": steps,
+            "ms_per_step": round((t1 - t0) / steps * 1e3, 4), "scenes": scenes, "particles": total,
+            "sort_every": sort_every, "graph": graph, "autotuned": chosen, "grid_mass_drift": mass,
+            "kernels": table,
+            "workload": "batched 2D MLS-MPM: 256 seeded scenes, counts log-uniform in [4096, 262144], grids "
+                        "64^2..512^2, water columns at 4 per cell, fixed-point P2G, sort every 10"}
+
+
+def bench_small2(steps, graph):
+    """The 2D reference-sized scene alone: 8,192 particles (a 32x64-cell column at 4 a cell) on
+    128^2, the launch-bound case that graphs address."""
+    from vexhip import particles, scenes as sc_mod
+
+    s = sc_mod.mpm2_dam_block()
+    count = len(s["x"])
+    h = 1.0 / 127
+    words = particles.params2(1e-4, 0, -9.8 / h, 10.0 / h ** 2, 1.0)
+    sim = particles.ParticleSimulation(128, 128, count)
+    z = np.zeros(count, np.float32)
+    sim.load(s["x"], s["y"], z, z, s["m"], words)
+    sim.sim.set_option("p2g", 2)
+    sim.sim.set_option("graph", 1 if graph else 0)
+    sim.advance(SORT_EVERY, SORT_EVERY)
+    sim.finish()
+    t0 = time.perf_counter()
+    sim.advance(steps, SORT_EVERY)
+    sim.finish()
+    t1 = time.perf_counter()
+    sim.close()
+    return {"value": round(count * steps / (t1 - t0), 1), "us_per_step": round((t1 - t0) / steps * 1e6, 2),
+            "graph": graph}
 
 
 def cpu_baseline_mpm3():
@@ -155,6 +225,8 @@ def main():
     ap.add_argument("--hll-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hll", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (counter calibration) and exit")
     args = ap.parse_args()
@@ -204,6 +276,12 @@ def main():
                                     "frac": round(hk["achieved_gbs"] / PEAK_HBM_GBS, 4),
                                     "traffic": measured_traffic("hll_step"),
                                     "avg_us": hk["avg_us"]}}
+    if not args.no_batch:
+        print(f"[bench] batched 2D: 256 scenes, {args.batch_steps} steps", file=sys.stderr, flush=True)
+        line["batch2d"] = bench_batch2(args.batch_steps)
+        print("[bench] 2D 8,192 particles on 128^2, eager and graph", file=sys.stderr, flush=True)
+        line["small2d"] = {"eager": bench_small2(500, False), "graph": bench_small2(500, True),
+                           "workload": "2D MLS-MPM, 8,192 particles, 128^2, fixed-point P2G, sort every 10"}
     if not args.no_cpu:
         print("[bench] CPU oracle baseline (one substep, one thread)", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline_mpm3()

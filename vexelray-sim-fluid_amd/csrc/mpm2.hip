@@ -274,43 +274,105 @@ __device__ __forceinline__ void add_fixed(unsigned long long* p, long long v) {
 }
 
 // P2G in fixed point. A block takes a chunk of kBatchAlign particles of one scene; a thread
-// folds `run` consecutive particles sharing a stencil base into 27 integer sums and adds each
-// to the grid with one 64-bit atomic.
+// folds `run` consecutive particles sharing a stencil base into 27 integer sums. When the
+// chunk's stencils fit kTile2 nodes (sorted particles: a few rows of cells) the sums go to an
+// LDS tile by 64-bit LDS atomics, and the tile's touched nodes to the grid by one 64-bit global
+// atomic each; otherwise straight to the grid. Integer sums: the same bits either way.
+constexpr int kTile2 = 2048;  // 3 x 8 B a node: 48 KB
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
 __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a, int run) {
+    __shared__ unsigned long long tile[3][kTile2];
+    __shared__ int box[4];
     const float mscale = 0x1p40f, pscale = 0x1p32f;
+    const int passes = kBatchAlign / (kBlock * run);
     for (int64_t c = blockIdx.x; c < a.pchunks; c += gridDim.x) {
-        const Scene2 sc = a.scenes[a.pscene[c]];
-        const int32_t* params = a.params + 28 * (int64_t)a.pscene[c];
+        const int s = a.pscene[c];
+        const Scene2 sc = a.scenes[s];
+        const int32_t* params = a.params + 28 * (int64_t)s;
         const float dt = bits_f(params[DT]), bulk = bits_f(params[BULK]), rho0 = bits_f(params[RHO0]);
+        const int64_t live = sc.poff + sc.count;  // the chunk's live particles are a prefix of it
+
+        // the chunk's box of stencil bases
+        if (threadIdx.x == 0) {
+            box[0] = box[1] = 0x7fffffff;
+            box[2] = box[3] = -1;
+        }
+        __syncthreads();
+        int x0 = 0x7fffffff, y0 = 0x7fffffff, x1 = -1, y1 = -1;
+        for (int i = threadIdx.x; i < kBatchAlign; i += kBlock) {
+            const int64_t q = c * kBatchAlign + i;
+            if (q >= live) break;
+            const int bx = axis(a.p[0][q], sc.nx).cell, by = axis(a.p[1][q], sc.ny).cell;
+            x0 = min(x0, bx);
+            y0 = min(y0, by);
+            x1 = max(x1, bx);
+            y1 = max(y1, by);
+        }
+        x0 = wave_min(x0);
+        y0 = wave_min(y0);
+        x1 = wave_max(x1);
+        y1 = wave_max(y1);
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+            atomicMin(&box[0], x0);
+            atomicMin(&box[1], y0);
+            atomicMax(&box[2], x1);
+            atomicMax(&box[3], y1);
+        }
+        __syncthreads();
+        const int bx0 = box[0], by0 = box[1];
+        const int w = box[2] - bx0 + 3, h = box[3] - by0 + 3;
+        const bool tiled = box[2] >= 0 && w * h <= kTile2;
+        if (tiled) {
+            for (int i = threadIdx.x; i < w * h; i += kBlock) tile[0][i] = tile[1][i] = tile[2][i] = 0ull;
+        }
+        __syncthreads();
+
         unsigned long long* q0 = a.q[0] + sc.goff;
         unsigned long long* q1 = a.q[1] + sc.goff;
         unsigned long long* q2 = a.q[2] + sc.goff;
-        const int passes = kBatchAlign / (kBlock * run);
         for (int pass = 0; pass < passes; ++pass) {
             const int64_t first = c * kBatchAlign + ((int64_t)pass * kBlock + threadIdx.x) * run;
             long long acc[27];
 #pragma unroll
             for (int i = 0; i < 27; ++i) acc[i] = 0;
-            int64_t run_base = -1;
-            auto flush = [&](int64_t base) {
+            int rx = -1, ry = -1;
+            auto flush = [&]() {
 #pragma unroll
                 for (int k = 0; k < 9; ++k) {
-                    const int64_t node = base + (k / 3) * sc.nx + k % 3;
-                    add_fixed(q0 + node, acc[3 * k]);
-                    add_fixed(q1 + node, acc[3 * k + 1]);
-                    add_fixed(q2 + node, acc[3 * k + 2]);
+                    if (tiled) {
+                        const int at = (ry - by0 + k / 3) * w + (rx - bx0 + k % 3);
+                        atomicAdd(&tile[0][at], (unsigned long long)acc[3 * k]);
+                        atomicAdd(&tile[1][at], (unsigned long long)acc[3 * k + 1]);
+                        atomicAdd(&tile[2][at], (unsigned long long)acc[3 * k + 2]);
+                    } else {
+                        const int64_t node = (int64_t)(ry + k / 3) * sc.nx + rx + k % 3;
+                        add_fixed(q0 + node, acc[3 * k]);
+                        add_fixed(q1 + node, acc[3 * k + 1]);
+                        add_fixed(q2 + node, acc[3 * k + 2]);
+                    }
                 }
 #pragma unroll
                 for (int i = 0; i < 27; ++i) acc[i] = 0;
             };
             for (int r = 0; r < run; ++r) {
                 const int64_t q = first + r;
-                if (q - sc.poff >= sc.count) break;
+                if (q >= live) break;
                 const Axis ax = axis(a.p[0][q], sc.nx), ay = axis(a.p[1][q], sc.ny);
-                const int64_t base = (int64_t)ay.cell * sc.nx + ax.cell;
-                if (base != run_base) {
-                    if (run_base >= 0) flush(run_base);
-                    run_base = base;
+                if (ax.cell != rx || ay.cell != ry) {
+                    if (rx >= 0) flush();
+                    rx = ax.cell;
+                    ry = ay.cell;
                 }
                 amounts(a.p, q, ax, ay, dt, bulk, rho0, [&](int k, float m, float mu, float mv) {
                     acc[3 * k] += quantize(m, mscale);
@@ -318,8 +380,20 @@ __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a, int ru
                     acc[3 * k + 2] += quantize(mv, pscale);
                 });
             }
-            if (run_base >= 0) flush(run_base);
+            if (rx >= 0) flush();
         }
+        __syncthreads();
+        if (tiled) {
+            for (int i = threadIdx.x; i < w * h; i += kBlock) {
+                const unsigned long long v0 = tile[0][i], v1 = tile[1][i], v2 = tile[2][i];
+                if ((v0 | v1 | v2) == 0ull) continue;
+                const int64_t node = (int64_t)(by0 + i / w) * sc.nx + bx0 + i % w;
+                atomicAdd(q0 + node, v0);
+                atomicAdd(q1 + node, v1);
+                atomicAdd(q2 + node, v2);
+            }
+        }
+        __syncthreads();  // the tile and box serve the next chunk
     }
 }
 
