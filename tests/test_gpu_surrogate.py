@@ -1,0 +1,146 @@
+"""The surrogate's kernels against a plain PyTorch FP32 reference of the same operation (on the
+CPU), the FP64 HLL step against the CPU restatement bit for bit, and a short training run."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from vexhip import _native, scenes
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+F = torch.nn.functional
+
+
+def _bf(shape, gen, scale=1.0):
+    return (torch.from_numpy(gen.standard_normal(shape).astype(np.float32)) * scale).to(torch.bfloat16)
+
+
+def _conv_ref(x_nhwc, w_ctc, bias, relu):
+    """FP32 reference: x [N,H,W,Ci], w [Co, 9, Ci] -> [N,H,W,Co] with zero padding."""
+    co, _, ci = w_ctc.shape
+    w = w_ctc.float().reshape(co, 3, 3, ci).permute(0, 3, 1, 2).contiguous()
+    y = F.conv2d(x_nhwc.float().permute(0, 3, 1, 2).contiguous(), w, bias, padding=1)
+    if relu:
+        y = torch.relu(y)
+    return y.permute(0, 2, 3, 1).contiguous()
+
+
+@pytest.mark.parametrize("cin,cout,relu,f32", [(16, 64, True, False), (64, 64, True, False), (64, 16, False, True),
+                                               (16, 64, False, True)])
+def test_conv_matches_torch(gpu, cin, cout, relu, f32):
+    gen = np.random.default_rng(cin + cout)
+    n, h, w = 2, 37, 70  # a row not a multiple of the 64-pixel tile
+    x = _bf((n, h, w, cin), gen)
+    k = _bf((cout, 9, cin), gen, 0.1)
+    b = torch.from_numpy(gen.standard_normal(cout).astype(np.float32))
+    print(f"[surrogate] conv {cin}->{cout}", flush=True)
+    dx, dk, db = x.cuda(), k.cuda(), b.cuda()
+    outs = []
+    for staged in (0, 16):  # the LDS-staged kernel (default) and the unstaged one
+        out = torch.empty((n, h, w, cout), dtype=torch.float32 if f32 else torch.bfloat16, device="cuda")
+        flags = (1 if relu else 0) | (4 if f32 else 0) | staged
+        _native.vxs_check(_native.lib().vxs_conv3x3(torch.cuda.current_stream().cuda_stream, dx.data_ptr(), n, h, w,
+                                                    cin, dk.data_ptr(), db.data_ptr(), cout, out.data_ptr(), cout,
+                                                    None, flags))
+        outs.append(out.float().cpu())
+    ref = _conv_ref(x, k, b, relu)
+    tol = 1e-4 if f32 else 1e-2  # FP32 sums in another order; the bf16 store rounds to 2^-8
+    for got in outs:
+        err = (got - ref).abs() / (ref.abs() + 1.0)
+        assert float(err.max()) <= tol, float(err.max())
+    # both walk K in the same order through the same matrix-core sequence
+    assert torch.equal(outs[0], outs[1])
+
+
+def test_conv_masked_input_gradient_matches_autograd(gpu):
+    """The input gradient as the forward kernel over flipped weights, masked by the ReLU that made
+    the input: equal to autograd's through conv2d and the ReLU."""
+    gen = np.random.default_rng(5)
+    n, h, w, cin, cout = 1, 20, 33, 64, 16
+    x = _bf((n, h, w, cin), gen)
+    k = _bf((cout, 9, cin), gen, 0.1)
+    dy = _bf((n, h, w, cout), gen)
+    xr = x.float().requires_grad_(True)
+    y = _conv_ref(xr, k, None, False)
+    (y * dy.float()).sum().backward()
+    ref = xr.grad * (x.float() > 0)
+    s = torch.cuda.current_stream().cuda_stream
+    lib = _native.lib()
+    dk, ddy = k.cuda(), dy.cuda()
+    wt = torch.empty_like(dk)
+    _native.vxs_check(lib.vxs_flip(s, dk.data_ptr(), cout, cin, wt.data_ptr()))
+    out = torch.empty((n, h, w, cin), dtype=torch.float32, device="cuda")
+    mask = x.cuda()
+    _native.vxs_check(lib.vxs_conv3x3(s, ddy.data_ptr(), n, h, w, cout, wt.data_ptr(), None, cin, out.data_ptr(),
+                                      cin, mask.data_ptr(), 2 | 4))
+    got = out.cpu()
+    assert float(((got - ref).abs() / (ref.abs() + 1.0)).max()) <= 1e-4
+
+
+@pytest.mark.parametrize("cin,cout", [(16, 64), (64, 64), (64, 16)])
+def test_weight_gradient_matches_autograd(gpu, cin, cout):
+    gen = np.random.default_rng(cin * 3 + cout)
+    n, h, w = 2, 23, 41
+    x = _bf((n, h, w, cin), gen)
+    dy = _bf((n, h, w, cout), gen)
+    k = torch.zeros((cout, 9, cin), requires_grad=True)
+    b = torch.zeros(cout, requires_grad=True)
+    y = _conv_ref(x, k, b, False)
+    (y * dy.float()).sum().backward()
+    print(f"[surrogate] wgrad {cin}->{cout}", flush=True)
+    s = torch.cuda.current_stream().cuda_stream
+    lib = _native.lib()
+    px = n * h * w
+    chunk = 256  # several chunks, the last partial
+    scratch = torch.empty(lib.vxs_wgrad_scratch(px, cin, cout, chunk), dtype=torch.float32, device="cuda")
+    dw = torch.empty(cout * 9 * cin, dtype=torch.float32, device="cuda")
+    db = torch.empty(cout, dtype=torch.float32, device="cuda")
+    dx, ddy = x.cuda(), dy.cuda()
+    _native.vxs_check(lib.vxs_conv3x3_wgrad(s, dx.data_ptr(), ddy.data_ptr(), n, h, w, cin, cout, chunk,
+                                            scratch.data_ptr(), dw.data_ptr(), db.data_ptr()))
+    got = dw.cpu().reshape(cout, 9, cin)
+    ref = k.grad
+    assert float(((got - ref).abs() / (ref.abs().max() + 1e-6)).max()) <= 1e-4
+    assert float(((db.cpu() - b.grad).abs() / (b.grad.abs().max() + 1e-6)).max()) <= 1e-4
+
+
+def test_hll64_bit_exact_with_oracle(gpu):
+    from vexhip import surrogate
+
+    nx = ny = 40
+    h, hu, hv = scenes.swe_dam_bumps(nx, seed=3)
+    h, hu, hv = (a.astype(np.float64) for a in (h, hu, hv))
+    hu = hu + 0.1 * np.sin(np.arange(nx * ny))  # currents both ways
+    hll = surrogate.Hll64(nx)
+    hll.load(*(torch.from_numpy(a).cuda() for a in (h, hu, hv)))
+    dt = hll.dt
+    hll.step(12)
+    ref = (h, hu, hv)
+    for _ in range(12):
+        ref = oracle.swe_step_f64(nx, ny, (0, 0, 0, 0), *ref, 9.81, 1.0 / nx, 1.0 / ny, 1e-6, dt)
+    for got, exp in zip(hll.a, ref):
+        np.testing.assert_array_equal(got.cpu().numpy().view(np.int64), np.asarray(exp).view(np.int64))
+
+
+def test_device_scene_matches_host(gpu):
+    from vexhip import surrogate
+
+    n = 96
+    hll = surrogate.Hll64(n)
+    hll.scene(7)
+    h, _, _ = scenes.swe_dam_bumps(n, seed=7)
+    np.testing.assert_allclose(hll.a[0].float().cpu().numpy(), h, rtol=0, atol=1e-6)
+
+
+def test_training_reduces_the_loss(gpu):
+    from vexhip import surrogate
+
+    print("[surrogate] train 60 iterations at 64^2", flush=True)
+    model, losses = surrogate.train(n=64, seeds=range(60), iters=60, lr=2e-3)
+    assert np.isfinite(losses).all()
+    assert np.mean(losses[-10:]) < 0.5 * np.mean(losses[:5])
+    result = surrogate.evaluate(model, n=64, seeds=range(10000, 10004))
+    assert result["scenes"] == 4 and np.isfinite(result["rel_l2_h_mean"])

@@ -1,0 +1,639 @@
+// The convolutional surrogate of the shallow-water step, and the FP64 HLL step that judges it.
+//
+// No reference counterpart: the reference has only the HLL solver (core/stencil/ShallowWater.java).
+// The surrogate maps a state (h, hu, hv) to the state a fixed time later with a residual stack of
+// 3x3 convolutions: 16 -> 64, four 64 -> 64, 64 -> 16 channels (the 3 state channels padded to
+// 16, the padding held at zero), ReLU between, the last layer's output added to the state.
+//
+// Activations are NHWC bf16; weights bf16 [cout][tap][cin] with an FP32 master copy; every
+// product accumulates in FP32 on the matrix cores (v_mfma_f32_16x16x32_bf16). A 3x3 convolution
+// is an implicit GEMM: M = pixels, N = output channels, K = 9 taps x input channels, each lane
+// loading 8 consecutive channels of one tap (16 bytes). Zero padding at the image edges, so the
+// input gradient is the same kernel run with the taps flipped and the channel roles swapped. The
+// weight gradient is a GEMM over pixels, staged through LDS transposed.
+#include "common.h"
+
+namespace vxr {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kBlock = 256;
+constexpr int kTileX = 64;  // output pixels of a block: 4 waves of 16
+
+enum { kRelu = 1, kMask = 2, kOutF32 = 4, kAccumulate = 8 };
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x8 zero8() {
+    bf16x8 z;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) z[i] = (__bf16)0.0f;
+    return z;
+}
+
+// out[n][y][x][co] (co < out_ch) = act(bias[co] + sum_{tap, ci} w[co][tap][ci] in[n][y+dy][x+dx][ci]),
+// a block a run of 64 pixels of one row, a wave 16 of them against all 16 * kCoTiles channels.
+// kMask multiplies by (mask[n][y][x][co] > 0), kRelu clamps at zero, kOutF32 stores FP32 and
+// kAccumulate adds the result into the FP32 output.
+template <int kCin, int kCoTiles>
+__global__ __launch_bounds__(kBlock) void conv3x3_kernel(const __bf16* __restrict__ in, int H, int W,
+                                                         const __bf16* __restrict__ w,
+                                                         const float* __restrict__ bias, void* out, int out_ch,
+                                                         const __bf16* __restrict__ mask, int flags) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tiles_x = (W + kTileX - 1) / kTileX;
+    const int y = blockIdx.x / tiles_x;
+    const int x0 = (blockIdx.x % tiles_x) * kTileX + wave * 16;
+    const int64_t img = (int64_t)blockIdx.y * H * W;
+    const int i = lane & 15, kg = lane >> 4;
+    const int px = x0 + i;
+    constexpr int kK = 9 * kCin;
+    constexpr int kSteps = (kK + 31) / 32;
+    f32x4 acc[kCoTiles];
+#pragma unroll
+    for (int t = 0; t < kCoTiles; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+    for (int s = 0; s < kSteps; ++s) {
+        const int k = 32 * s + 8 * kg;
+        const int tap = k / kCin, ci = k % kCin;
+        bf16x8 a = zero8();
+        if (tap < 9) {
+            const int yy = y + tap / 3 - 1, xx = px + tap % 3 - 1;
+            if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                a = *(const bf16x8*)(in + (img + (int64_t)yy * W + xx) * kCin + ci);
+        }
+#pragma unroll
+        for (int t = 0; t < kCoTiles; ++t) {
+            const bf16x8 b = tap < 9 ? *(const bf16x8*)(w + ((int64_t)(16 * t + i) * 9 + tap) * kCin + ci) : zero8();
+            acc[t] = mfma(a, b, acc[t]);
+        }
+    }
+    // lane holds C[4 kg + r][i]: pixel x0 + 4 kg + r, channel 16 t + i
+#pragma unroll
+    for (int t = 0; t < kCoTiles; ++t) {
+        const int co = 16 * t + i;
+        if (co >= out_ch) continue;
+        const float b = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int p = x0 + 4 * kg + r;
+            if (p >= W) continue;
+            const int64_t at = (img + (int64_t)y * W + p) * out_ch + co;
+            float v = acc[t][r] + b;
+            if ((flags & kMask) && !((float)mask[at] > 0.0f)) v = 0.0f;
+            if (flags & kRelu) v = v > 0.0f ? v : 0.0f;
+            if (flags & kOutF32) {
+                float* o = (float*)out;
+                o[at] = (flags & kAccumulate) ? o[at] + v : v;
+            } else {
+                ((__bf16*)out)[at] = (__bf16)v;
+            }
+        }
+    }
+}
+
+// The same convolution with its operands staged: a block stays resident over tiles of 2 rows x 64
+// pixels; each tile's input with its one-pixel halo (4 x 66 pixels, every channel) goes to LDS
+// once and serves all 9 taps and all waves, while each wave keeps the weights of its output
+// channels in registers for the whole launch. Waves split a tile 2 x 2 (pixels x channels) for
+// 64 output channels, 4 x 1 for 16. Pixel stride in LDS is cin + 8 channels, so the 16 lanes
+// reading 16 consecutive pixels hit distinct banks.
+constexpr int kRows = 2;
+constexpr int kCols = 64;
+
+template <int kCin, int kCoTiles>
+__global__ __launch_bounds__(kBlock, 2) void conv3x3_lds_kernel(const __bf16* __restrict__ in, int images, int H,
+                                                                int W, const __bf16* __restrict__ w,
+                                                                const float* __restrict__ bias, void* out,
+                                                                int out_ch, const __bf16* __restrict__ mask,
+                                                                int flags) {
+    constexpr int kStride = kCin + 8;
+    constexpr int kTileW = kCols + 2, kTileH = kRows + 2;
+    constexpr int kNT = kCoTiles >= 2 ? 2 : 1;         // channel tiles a wave holds
+    constexpr int kWavesN = kCoTiles / kNT;
+    constexpr int kWavesM = 4 / kWavesN;
+    constexpr int kMT = (kRows * kCols / 16) / kWavesM;  // 16-pixel tiles a wave computes
+    constexpr int kK = 9 * kCin;
+    constexpr int kSteps = (kK + 31) / 32;
+    __shared__ __align__(16) __bf16 tile[kTileH * kTileW * kStride];
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wn = wave % kWavesN, wm = wave / kWavesN;
+    const int i = lane & 15, kg = lane >> 4;
+
+    // this wave's weights: B[k][n] for its kNT channel tiles, every K step
+    bf16x8 b[kNT][kSteps];
+#pragma unroll
+    for (int t = 0; t < kNT; ++t) {
+        const int co = 16 * (wn * kNT + t) + i;
+#pragma unroll
+        for (int s = 0; s < kSteps; ++s) {
+            const int k = 32 * s + 8 * kg;
+            const int tap = k / kCin, ci = k % kCin;
+            b[t][s] = tap < 9 ? *(const bf16x8*)(w + ((int64_t)co * 9 + tap) * kCin + ci) : zero8();
+        }
+    }
+    const int tiles_x = (W + kCols - 1) / kCols, tiles_y = (H + kRows - 1) / kRows;
+    const int64_t tiles = (int64_t)images * tiles_y * tiles_x;
+    for (int64_t tt = blockIdx.x; tt < tiles; tt += gridDim.x) {
+        const int n = (int)(tt / ((int64_t)tiles_y * tiles_x));
+        const int rem = (int)(tt % ((int64_t)tiles_y * tiles_x));
+        const int y0 = (rem / tiles_x) * kRows, x0 = (rem % tiles_x) * kCols;
+        const int64_t img = (int64_t)n * H * W;
+        // stage the halo tile: 16 bytes a thread at a time
+        constexpr int kChunks = kTileH * kTileW * (kCin / 8);
+        for (int e = threadIdx.x; e < kChunks; e += kBlock) {
+            const int pix = e / (kCin / 8), cg = e % (kCin / 8);
+            const int ty = pix / kTileW, tx = pix % kTileW;
+            const int yy = y0 + ty - 1, xx = x0 + tx - 1;
+            bf16x8 v = zero8();
+            if (yy >= 0 && yy < H && xx >= 0 && xx < W) v = *(const bf16x8*)(in + (img + (int64_t)yy * W + xx) * kCin + 8 * cg);
+            *(bf16x8*)&tile[pix * kStride + 8 * cg] = v;
+        }
+        __syncthreads();
+        f32x4 acc[kMT][kNT];
+#pragma unroll
+        for (int m = 0; m < kMT; ++m)
+#pragma unroll
+            for (int t = 0; t < kNT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < kSteps; ++s) {
+            const int k = 32 * s + 8 * kg;
+            const int tap = k / kCin, ci = k % kCin;
+            const int dy = tap / 3, dx = tap % 3;  // tile offsets of the tap (halo included)
+#pragma unroll
+            for (int m = 0; m < kMT; ++m) {
+                const int mt = wm * kMT + m;
+                const int r = mt / (kCols / 16), c = 16 * (mt % (kCols / 16)) + i;
+                const bf16x8 a = tap < 9 ? *(const bf16x8*)&tile[((r + dy) * kTileW + c + dx) * kStride + ci] : zero8();
+#pragma unroll
+                for (int t = 0; t < kNT; ++t) acc[m][t] = mfma(a, b[t][s], acc[m][t]);
+            }
+        }
+        // lane holds C[4 kg + q][i] of each (m, t): pixel (row of mt, col 16 (mt % 4) + 4 kg + q), channel
+#pragma unroll
+        for (int t = 0; t < kNT; ++t) {
+            const int co = 16 * (wn * kNT + t) + i;
+            if (co >= out_ch) continue;
+            const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+            for (int m = 0; m < kMT; ++m) {
+                const int mt = wm * kMT + m;
+                const int yy = y0 + mt / (kCols / 16);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int xx = x0 + 16 * (mt % (kCols / 16)) + 4 * kg + q;
+                    if (yy >= H || xx >= W) continue;
+                    const int64_t at = (img + (int64_t)yy * W + xx) * out_ch + co;
+                    float v = acc[m][t][q] + bv;
+                    if ((flags & kMask) && !((float)mask[at] > 0.0f)) v = 0.0f;
+                    if (flags & kRelu) v = v > 0.0f ? v : 0.0f;
+                    if (flags & kOutF32) {
+                        float* o = (float*)out;
+                        o[at] = (flags & kAccumulate) ? o[at] + v : v;
+                    } else {
+                        ((__bf16*)out)[at] = (__bf16)v;
+                    }
+                }
+            }
+        }
+        __syncthreads();  // the tile is restaged for the next one
+    }
+}
+
+// Weight gradient of one tap over a chunk of pixels:
+//   part[chunk][tap][co][ci] = sum_{p in chunk} dy[p][co] * x[p + off(tap)][ci]
+// (and, for the centre tap, part_b[chunk][co] = sum_p dy[p][co]). 32 pixels at a time are staged
+// into LDS transposed -- dyT[co][p], xT[ci][p] -- so each lane's 8 consecutive pixels are one
+// 16-byte LDS read; a wave takes every fourth of the (co, ci) 16x16 tiles.
+template <int kCin, int kCout>
+__global__ __launch_bounds__(kBlock) void conv3x3_wgrad_kernel(const __bf16* __restrict__ x,
+                                                               const __bf16* __restrict__ dy, int H, int W,
+                                                               int64_t pixels, int64_t chunk,
+                                                               float* __restrict__ part,
+                                                               float* __restrict__ part_b) {
+    constexpr int kTiles = (kCout / 16) * (kCin / 16);
+    constexpr int kPerWave = kTiles / 4;
+    __shared__ __align__(16) __bf16 dyT[kCout][32 + 8];
+    __shared__ __align__(16) __bf16 xT[kCin][32 + 8];
+    const int tap = blockIdx.y;
+    const int dyo = tap / 3 - 1, dxo = tap % 3 - 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = lane & 15, kg = lane >> 4;
+    const int64_t p_begin = (int64_t)blockIdx.x * chunk;
+    const int64_t p_end = p_begin + chunk < pixels ? p_begin + chunk : pixels;
+    f32x4 acc[kPerWave];
+#pragma unroll
+    for (int t = 0; t < kPerWave; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float bsum = 0.0f;
+    const int64_t hw = (int64_t)H * W;
+    for (int64_t p0 = p_begin; p0 < p_end; p0 += 32) {
+        // stage: thread t copies channel group (t % (C/8)) of pixel p0 + t / (C/8), transposed
+        for (int e = threadIdx.x; e < 32 * (kCout / 8); e += kBlock) {
+            const int pl = e / (kCout / 8), cg = e % (kCout / 8);
+            const int64_t p = p0 + pl;
+            bf16x8 v = zero8();
+            if (p < p_end) v = *(const bf16x8*)(dy + p * kCout + 8 * cg);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) dyT[8 * cg + j][pl] = v[j];
+        }
+        for (int e = threadIdx.x; e < 32 * (kCin / 8); e += kBlock) {
+            const int pl = e / (kCin / 8), cg = e % (kCin / 8);
+            const int64_t p = p0 + pl;
+            bf16x8 v = zero8();
+            if (p < p_end) {
+                const int64_t n = p / hw, rem = p % hw;
+                const int yy = (int)(rem / W) + dyo, xx = (int)(rem % W) + dxo;
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                    v = *(const bf16x8*)(x + (n * hw + (int64_t)yy * W + xx) * kCin + 8 * cg);
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) xT[8 * cg + j][pl] = v[j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < kPerWave; ++t) {
+            const int tile = wave + 4 * t;
+            const int ct = tile / (kCin / 16), it = tile % (kCin / 16);
+            const bf16x8 a = *(const bf16x8*)&dyT[16 * ct + i][8 * kg];
+            const bf16x8 b = *(const bf16x8*)&xT[16 * it + i][8 * kg];
+            acc[t] = mfma(a, b, acc[t]);
+        }
+        if (tap == 4 && threadIdx.x < kCout) {
+            for (int j = 0; j < 32; ++j) bsum += (float)dyT[threadIdx.x][j];
+        }
+        __syncthreads();
+    }
+    // lane holds C[4 kg + r][i]: co = 16 ct + 4 kg + r, ci = 16 it + i
+    float* out = part + ((int64_t)blockIdx.x * 9 + tap) * kCout * kCin;
+#pragma unroll
+    for (int t = 0; t < kPerWave; ++t) {
+        const int tile = wave + 4 * t;
+        const int ct = tile / (kCin / 16), it = tile % (kCin / 16);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) out[(16 * ct + 4 * kg + r) * kCin + 16 * it + i] = acc[t][r];
+    }
+    if (tap == 4 && threadIdx.x < kCout) part_b[(int64_t)blockIdx.x * kCout + threadIdx.x] = bsum;
+}
+
+// dw[co][tap][ci] = sum over chunks of part[chunk][tap][co][ci]; db[co] likewise, in chunk order.
+__global__ __launch_bounds__(kBlock) void wgrad_reduce_kernel(const float* __restrict__ part,
+                                                              const float* __restrict__ part_b, int chunks,
+                                                              int cout, int cin, float* __restrict__ dw,
+                                                              float* __restrict__ db) {
+    const int64_t n = (int64_t)9 * cout * cin;
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e < n) {
+        const int tap = (int)(e / ((int64_t)cout * cin));
+        const int rem = (int)(e % ((int64_t)cout * cin));
+        const int co = rem / cin, ci = rem % cin;
+        float s = 0.0f;
+        for (int c = 0; c < chunks; ++c) s += part[(int64_t)c * n + e];
+        dw[((int64_t)co * 9 + tap) * cin + ci] = s;
+    }
+    if (e < cout) {
+        float s = 0.0f;
+        for (int c = 0; c < chunks; ++c) s += part_b[(int64_t)c * cout + e];
+        db[e] = s;
+    }
+}
+
+// wt[ci][8 - tap][co] = w[co][tap][ci]: the kernel that carries output gradients back to inputs.
+__global__ void flip_kernel(const __bf16* __restrict__ w, int cout, int cin, __bf16* __restrict__ wt) {
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= (int64_t)9 * cout * cin) return;
+    const int co = (int)(e / (9 * cin)), tap = (int)(e / cin % 9), ci = (int)(e % cin);
+    wt[((int64_t)ci * 9 + (8 - tap)) * cout + co] = w[e];
+}
+
+// Adam on the FP32 master weights, writing the bf16 copy the convolutions read.
+__global__ void adam_kernel(float* __restrict__ p, __bf16* __restrict__ p16, const float* __restrict__ g,
+                            float* __restrict__ m, float* __restrict__ v, int64_t n, float lr, float b1, float b2,
+                            float eps, float c1, float c2) {
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= n) return;
+    const float ge = g[e];
+    const float me = b1 * m[e] + (1.0f - b1) * ge;
+    const float ve = b2 * v[e] + (1.0f - b2) * ge * ge;
+    m[e] = me;
+    v[e] = ve;
+    const float pe = p[e] - lr * (me / c1) / (sqrtf(ve / c2) + eps);
+    p[e] = pe;
+    if (p16) p16[e] = (__bf16)pe;
+}
+
+// The state (planar FP32 h, hu, hv, one image after another) as the network's input: NHWC bf16,
+// 16 channels, the first three (q - offset[c]) * scale[c], the rest zero.
+__global__ void pack_kernel(const float* __restrict__ h, const float* __restrict__ hu, const float* __restrict__ hv,
+                            int64_t pixels, const float* __restrict__ norm, __bf16* __restrict__ out) {
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= pixels) return;
+    bf16x8 a = zero8(), b = zero8();
+    a[0] = (__bf16)((h[p] - norm[0]) * norm[3]);
+    a[1] = (__bf16)((hu[p] - norm[1]) * norm[4]);
+    a[2] = (__bf16)((hv[p] - norm[2]) * norm[5]);
+    *(bf16x8*)(out + p * 16) = a;
+    *(bf16x8*)(out + p * 16 + 8) = b;
+}
+
+// next[c] = state[c] + delta[p][c] / scale_out[c] for the three state channels (delta FP32, 16
+// channels a pixel).
+__global__ void unpack_kernel(const float* __restrict__ h, const float* __restrict__ hu,
+                              const float* __restrict__ hv, const float* __restrict__ delta, int64_t pixels,
+                              const float* __restrict__ norm, float* __restrict__ oh, float* __restrict__ ohu,
+                              float* __restrict__ ohv) {
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= pixels) return;
+    oh[p] = h[p] + delta[p * 16] / norm[6];
+    ohu[p] = hu[p] + delta[p * 16 + 1] / norm[7];
+    ohv[p] = hv[p] + delta[p * 16 + 2] / norm[8];
+}
+
+// The loss 1/(3P) sum_c sum_p (delta[p][c] - target_c)^2 with target_c = (next_c - state_c) *
+// scale_out[c], in the output's scaled units; writes its gradient (bf16 NHWC, 16 channels, zero
+// padding channels) and adds each block's partial loss to loss[0].
+__global__ void loss_kernel(const float* __restrict__ delta, const float* __restrict__ h,
+                            const float* __restrict__ hu, const float* __restrict__ hv,
+                            const float* __restrict__ th, const float* __restrict__ thu,
+                            const float* __restrict__ thv, int64_t pixels, const float* __restrict__ norm,
+                            __bf16* __restrict__ grad, float* __restrict__ loss) {
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    float l = 0.0f;
+    if (p < pixels) {
+        const float r0 = delta[p * 16] - (th[p] - h[p]) * norm[6];
+        const float r1 = delta[p * 16 + 1] - (thu[p] - hu[p]) * norm[7];
+        const float r2 = delta[p * 16 + 2] - (thv[p] - hv[p]) * norm[8];
+        l = r0 * r0 + r1 * r1 + r2 * r2;
+        const float k = 2.0f / (3.0f * (float)pixels);
+        bf16x8 a = zero8(), b = zero8();
+        a[0] = (__bf16)(k * r0);
+        a[1] = (__bf16)(k * r1);
+        a[2] = (__bf16)(k * r2);
+        *(bf16x8*)(grad + p * 16) = a;
+        *(bf16x8*)(grad + p * 16 + 8) = b;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss, l / (3.0f * (float)pixels));
+}
+
+// ------------------------------------------------------------------ FP64 HLL
+
+__device__ __forceinline__ double dmn(double a, double b) { return b < a ? b : a; }
+__device__ __forceinline__ double dmx(double a, double b) { return a < b ? b : a; }
+
+struct S64 {
+    double h, hu, hv;
+};
+
+__device__ __forceinline__ void flux64(const S64& l, const S64& r, bool along_x, double g, double dry,
+                                       double f[3]) {
+    const double qnL = along_x ? l.hu : l.hv, qtL = along_x ? l.hv : l.hu;
+    const double qnR = along_x ? r.hu : r.hv, qtR = along_x ? r.hv : r.hu;
+    const double unL = l.h > dry ? qnL / l.h : 0.0, utL = l.h > dry ? qtL / l.h : 0.0;
+    const double unR = r.h > dry ? qnR / r.h : 0.0, utR = r.h > dry ? qtR / r.h : 0.0;
+    const double cL = sqrt(g * l.h), cR = sqrt(g * r.h);
+    double sL = dmn(unL - cL, unR - cR), sR = dmx(unL + cL, unR + cR);
+    if (!(r.h > dry)) {
+        sL = unL - cL;
+        sR = unL + 2 * cL;
+    }
+    if (!(l.h > dry)) {
+        sL = unR - 2 * cR;
+        sR = unR + cR;
+    }
+    const double lo = dmn(sL, 0.0), hi = dmx(sR, 0.0), span = dmx(hi - lo, 1e-300), lohi = lo * hi;
+    const double pL = 0.5 * g * l.h * l.h, pR = 0.5 * g * r.h * r.h;
+    f[0] = ((hi * qnL - lo * qnR) + lohi * (r.h - l.h)) / span;
+    f[1] = ((hi * (qnL * unL + pL) - lo * (qnR * unR + pR)) + lohi * (qnR - qnL)) / span;
+    f[2] = ((hi * (qnL * utL) - lo * (qnR * utR)) + lohi * (qtR - qtL)) / span;
+}
+
+// One HLL step in FP64 with a given dt, the operations of the CPU restatement's FP64 twin in its
+// order (walls: the ghost's normal momentum negated; open: the cell itself).
+__global__ __launch_bounds__(kBlock) void hll64_kernel(int nx, int ny, int4 edges, const double* __restrict__ ih,
+                                                       const double* __restrict__ ihu,
+                                                       const double* __restrict__ ihv, double* __restrict__ oh,
+                                                       double* __restrict__ ohu, double* __restrict__ ohv, double g,
+                                                       double rx, double ry, double dry) {
+    const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (cell >= (int64_t)nx * ny) return;
+    const int x = (int)(cell % nx), y = (int)(cell / nx);
+    auto nb = [&](bool edge, int64_t off, int kind, bool along_x) {
+        const int64_t at = edge ? cell : cell + off;
+        S64 s{ih[at], ihu[at], ihv[at]};
+        if (kind == 0 && edge) {
+            if (along_x) s.hu = -s.hu;
+            else s.hv = -s.hv;
+        }
+        return s;
+    };
+    const S64 c{ih[cell], ihu[cell], ihv[cell]};
+    const S64 w = nb(x == 0, -1, edges.x, true), e = nb(x == nx - 1, 1, edges.y, true);
+    const S64 s = nb(y == 0, -(int64_t)nx, edges.z, false), n = nb(y == ny - 1, nx, edges.w, false);
+    double fw[3], fe[3], fs[3], fn[3];
+    flux64(w, c, true, g, dry, fw);
+    flux64(c, e, true, g, dry, fe);
+    flux64(s, c, false, g, dry, fs);
+    flux64(c, n, false, g, dry, fn);
+    const double h = (c.h - rx * (fe[0] - fw[0])) - ry * (fn[0] - fs[0]);
+    oh[cell] = h < 0 ? 0 : h;
+    ohu[cell] = (c.hu - rx * (fe[1] - fw[1])) - ry * (fn[2] - fs[2]);
+    ohv[cell] = (c.hv - rx * (fe[2] - fw[2])) - ry * (fn[1] - fs[1]);
+}
+
+__global__ void f64_to_f32_kernel(const double* __restrict__ a, float* __restrict__ b, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e < n) b[e] = (float)a[e];
+}
+
+// The circular dam and Gaussian bumps of the shallow-water scenes on the device: depth 2 inside
+// radius r of the centre, 1 outside, plus amp * exp(-d^2 / width^2) per bump; hu = hv = 0.
+__global__ void dam_bumps_kernel(int n, int bumps, const double* __restrict__ bp, double radius,
+                                 double* __restrict__ h, double* __restrict__ hu, double* __restrict__ hv) {
+    const int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (cell >= (int64_t)n * n) return;
+    const double x = ((double)(cell % n) + 0.5) / n, y = ((double)(cell / n) + 0.5) / n;
+    double v = (x - 0.5) * (x - 0.5) + (y - 0.5) * (y - 0.5) < radius * radius ? 2.0 : 1.0;
+    for (int b = 0; b < bumps; ++b) {
+        const double amp = bp[4 * b], width = bp[4 * b + 1], cx = bp[4 * b + 2], cy = bp[4 * b + 3];
+        v += amp * exp(-((x - cx) * (x - cx) + (y - cy) * (y - cy)) / (width * width));
+    }
+    h[cell] = v;
+    hu[cell] = 0.0;
+    hv[cell] = 0.0;
+}
+
+inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+}  // namespace vxr
+
+using namespace vxr;
+
+namespace {
+thread_local char g_serr[256] = "no error";
+int serr(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) return 0;
+    snprintf(g_serr, sizeof g_serr, "%s: %s", what, hipGetErrorString(e));
+    return 3;
+}
+int sfail(const char* msg) {
+    snprintf(g_serr, sizeof g_serr, "%s", msg);
+    return 1;
+}
+}  // namespace
+
+// The surrogate's kernels as plain entry points over device pointers on a caller's stream. They
+// return 0, 1 (bad argument) or 3 (HIP error); vxs_last_error says which.
+extern "C" {
+
+const char* vxs_last_error(void) { return g_serr; }
+
+// flags: 1 ReLU, 2 mask by (mask > 0), 4 FP32 output, 8 add into the FP32 output, 16 the
+// unstaged kernel (operands straight from global memory; kept as a cross-check).
+int vxs_conv3x3(void* stream, const void* in, int images, int H, int W, int cin, const void* w, const float* bias,
+                int cout, void* out, int out_ch, const void* mask, int flags) {
+    if (!in || !w || !out || images < 1 || H < 1 || W < 1) return sfail("conv3x3: bad argument");
+    if (out_ch < 1 || out_ch > cout) return sfail("conv3x3: out_ch must be 1..cout");
+    if ((flags & kMask) && !mask) return sfail("conv3x3: mask flag without a mask");
+    hipStream_t s = (hipStream_t)stream;
+    const __bf16* x = (const __bf16*)in;
+    const __bf16* k = (const __bf16*)w;
+    const __bf16* mk = (const __bf16*)mask;
+    if (!(flags & 16)) {  // the staged kernel, resident blocks over 2 x 64 tiles
+        const int64_t tiles = (int64_t)images * ((H + kRows - 1) / kRows) * ((W + kCols - 1) / kCols);
+        const unsigned g = (unsigned)(tiles < 2048 ? tiles : 2048);
+        const int f = flags & 15;
+        if (cin == 16 && cout == 64)
+            hipLaunchKernelGGL((conv3x3_lds_kernel<16, 4>), dim3(g), dim3(kBlock), 0, s, x, images, H, W, k, bias, out, out_ch, mk, f);
+        else if (cin == 64 && cout == 64)
+            hipLaunchKernelGGL((conv3x3_lds_kernel<64, 4>), dim3(g), dim3(kBlock), 0, s, x, images, H, W, k, bias, out, out_ch, mk, f);
+        else if (cin == 64 && cout == 16)
+            hipLaunchKernelGGL((conv3x3_lds_kernel<64, 1>), dim3(g), dim3(kBlock), 0, s, x, images, H, W, k, bias, out, out_ch, mk, f);
+        else if (cin == 16 && cout == 16)
+            hipLaunchKernelGGL((conv3x3_lds_kernel<16, 1>), dim3(g), dim3(kBlock), 0, s, x, images, H, W, k, bias, out, out_ch, mk, f);
+        else
+            return sfail("conv3x3: channels (cin, cout) must be 16 or 64 each");
+        return serr("conv3x3");
+    }
+    const dim3 grid((unsigned)(H * ((W + kTileX - 1) / kTileX)), (unsigned)images);
+    flags &= 15;
+    if (cin == 16 && cout == 64)
+        hipLaunchKernelGGL((conv3x3_kernel<16, 4>), grid, dim3(kBlock), 0, s, x, H, W, k, bias, out, out_ch, mk, flags);
+    else if (cin == 64 && cout == 64)
+        hipLaunchKernelGGL((conv3x3_kernel<64, 4>), grid, dim3(kBlock), 0, s, x, H, W, k, bias, out, out_ch, mk, flags);
+    else if (cin == 64 && cout == 16)
+        hipLaunchKernelGGL((conv3x3_kernel<64, 1>), grid, dim3(kBlock), 0, s, x, H, W, k, bias, out, out_ch, mk, flags);
+    else if (cin == 16 && cout == 16)
+        hipLaunchKernelGGL((conv3x3_kernel<16, 1>), grid, dim3(kBlock), 0, s, x, H, W, k, bias, out, out_ch, mk, flags);
+    else
+        return sfail("conv3x3: channels (cin, cout) must be 16 or 64 each");
+    return serr("conv3x3");
+}
+
+// Scratch words the weight gradient needs for `pixels` pixels: partial sums per chunk.
+int64_t vxs_wgrad_scratch(int64_t pixels, int cin, int cout, int64_t chunk) {
+    const int64_t chunks = (pixels + chunk - 1) / chunk;
+    return chunks * 9 * cin * cout + chunks * cout;
+}
+
+// dw[co][tap][ci] and db[co] of a 3x3 convolution from its input x and output gradient dy (both
+// NHWC bf16 over `images` images of H x W), reduced in a fixed order.
+int vxs_conv3x3_wgrad(void* stream, const void* x, const void* dy, int images, int H, int W, int cin, int cout,
+                      int64_t chunk, float* scratch, float* dw, float* db) {
+    if (!x || !dy || !scratch || !dw || !db || chunk < 32 || chunk % 32) return sfail("wgrad: bad argument");
+    const int64_t pixels = (int64_t)images * H * W;
+    const int64_t chunks = (pixels + chunk - 1) / chunk;
+    float* part = scratch;
+    float* part_b = scratch + chunks * 9 * cin * cout;
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 grid((unsigned)chunks, 9);
+    const __bf16* a = (const __bf16*)x;
+    const __bf16* g = (const __bf16*)dy;
+    if (cin == 16 && cout == 64)
+        hipLaunchKernelGGL((conv3x3_wgrad_kernel<16, 64>), grid, dim3(kBlock), 0, s, a, g, H, W, pixels, chunk, part, part_b);
+    else if (cin == 64 && cout == 64)
+        hipLaunchKernelGGL((conv3x3_wgrad_kernel<64, 64>), grid, dim3(kBlock), 0, s, a, g, H, W, pixels, chunk, part, part_b);
+    else if (cin == 64 && cout == 16)
+        hipLaunchKernelGGL((conv3x3_wgrad_kernel<64, 16>), grid, dim3(kBlock), 0, s, a, g, H, W, pixels, chunk, part, part_b);
+    else
+        return sfail("wgrad: channels (cin, cout) must be (16, 64), (64, 64) or (64, 16)");
+    if (int rc = serr("wgrad")) return rc;
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(blocks_for((int64_t)9 * cin * cout)), dim3(kBlock), 0, s, part,
+                       part_b, (int)chunks, cout, cin, dw, db);
+    return serr("wgrad reduce");
+}
+
+int vxs_flip(void* stream, const void* w, int cout, int cin, void* wt) {
+    if (!w || !wt) return sfail("flip: bad argument");
+    hipLaunchKernelGGL(flip_kernel, dim3(blocks_for((int64_t)9 * cout * cin)), dim3(kBlock), 0,
+                       (hipStream_t)stream, (const __bf16*)w, cout, cin, (__bf16*)wt);
+    return serr("flip");
+}
+
+int vxs_adam(void* stream, float* p, void* p16, const float* g, float* m, float* v, int64_t n, float lr, float b1,
+             float b2, float eps, int t) {
+    if (!p || !g || !m || !v || t < 1) return sfail("adam: bad argument");
+    const float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, (hipStream_t)stream, p, (__bf16*)p16, g, m,
+                       v, n, lr, b1, b2, eps, c1, c2);
+    return serr("adam");
+}
+
+// norm: offsets (3), input scales (3), output scales (3) on the device.
+int vxs_pack(void* stream, const float* h, const float* hu, const float* hv, int64_t pixels, const float* norm,
+             void* out) {
+    if (!h || !hu || !hv || !norm || !out) return sfail("pack: bad argument");
+    hipLaunchKernelGGL(pack_kernel, dim3(blocks_for(pixels)), dim3(kBlock), 0, (hipStream_t)stream, h, hu, hv, pixels,
+                       norm, (__bf16*)out);
+    return serr("pack");
+}
+
+int vxs_unpack(void* stream, const float* h, const float* hu, const float* hv, const float* delta, int64_t pixels,
+               const float* norm, float* oh, float* ohu, float* ohv) {
+    if (!h || !hu || !hv || !delta || !norm || !oh || !ohu || !ohv) return sfail("unpack: bad argument");
+    hipLaunchKernelGGL(unpack_kernel, dim3(blocks_for(pixels)), dim3(kBlock), 0, (hipStream_t)stream, h, hu, hv,
+                       delta, pixels, norm, oh, ohu, ohv);
+    return serr("unpack");
+}
+
+int vxs_loss(void* stream, const float* delta, const float* h, const float* hu, const float* hv, const float* th,
+             const float* thu, const float* thv, int64_t pixels, const float* norm, void* grad, float* loss) {
+    if (!delta || !h || !th || !norm || !grad || !loss) return sfail("loss: bad argument");
+    hipLaunchKernelGGL(loss_kernel, dim3(blocks_for(pixels)), dim3(kBlock), 0, (hipStream_t)stream, delta, h, hu, hv,
+                       th, thu, thv, pixels, norm, (__bf16*)grad, loss);
+    return serr("loss");
+}
+
+// One FP64 HLL step with a fixed dt; edges west, east, south, north: 0 wall, 1 open.
+int vxs_hll64(void* stream, int nx, int ny, const int* edges, const double* h, const double* hu, const double* hv,
+              double* oh, double* ohu, double* ohv, double g, double dx, double dy, double dry, double dt) {
+    if (nx < 1 || ny < 1 || !edges || !h || !hu || !hv || !oh || !ohu || !ohv) return sfail("hll64: bad argument");
+    const int4 e = make_int4(edges[0], edges[1], edges[2], edges[3]);
+    hipLaunchKernelGGL(hll64_kernel, dim3(blocks_for((int64_t)nx * ny)), dim3(kBlock), 0, (hipStream_t)stream, nx, ny,
+                       e, h, hu, hv, oh, ohu, ohv, g, dt / dx, dt / dy, dry);
+    return serr("hll64");
+}
+
+// bump_params: amp, width, cx, cy per bump (device memory).
+int vxs_dam_bumps(void* stream, int n, int bumps, const double* bump_params, double radius, double* h, double* hu,
+                  double* hv) {
+    if (n < 1 || bumps < 0 || (bumps && !bump_params) || !h || !hu || !hv) return sfail("dam_bumps: bad argument");
+    hipLaunchKernelGGL(dam_bumps_kernel, dim3(blocks_for((int64_t)n * n)), dim3(kBlock), 0, (hipStream_t)stream, n,
+                       bumps, bump_params, radius, h, hu, hv);
+    return serr("dam_bumps");
+}
+
+int vxs_f64_to_f32(void* stream, const double* a, float* b, int64_t n) {
+    if (!a || !b) return sfail("f64_to_f32: bad argument");
+    hipLaunchKernelGGL(f64_to_f32_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, (hipStream_t)stream, a, b, n);
+    return serr("f64_to_f32");
+}
+
+}  // extern "C"

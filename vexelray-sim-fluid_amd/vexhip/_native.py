@@ -73,6 +73,24 @@ def lib() -> ctypes.CDLL:
             "vxr_scene_count": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int)]),
             "vxr_advance": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, i64]),
             "vxr_autotune": (ctypes.c_int, [vp, ctypes.c_int]),
+            # the shallow-water surrogate's kernels (csrc/surrogate.hip)
+            "vxs_last_error": (cs, []),
+            "vxs_conv3x3": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp,
+                                           ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int]),
+            "vxs_wgrad_scratch": (i64, [i64, ctypes.c_int, ctypes.c_int, i64]),
+            "vxs_conv3x3_wgrad": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_int, i64, vp, vp, vp]),
+            "vxs_flip": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, vp]),
+            "vxs_adam": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, i64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                        ctypes.c_float, ctypes.c_int]),
+            "vxs_pack": (ctypes.c_int, [vp, vp, vp, vp, i64, vp, vp]),
+            "vxs_unpack": (ctypes.c_int, [vp, vp, vp, vp, vp, i64, vp, vp, vp, vp]),
+            "vxs_loss": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, i64, vp, vp, vp]),
+            "vxs_hll64": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), vp, vp, vp, vp,
+                                         vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double]),
+            "vxs_dam_bumps": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_double, vp, vp, vp]),
+            "vxs_f64_to_f32": (ctypes.c_int, [vp, vp, vp, i64]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(l, name)
@@ -88,6 +106,14 @@ def check(rc: int) -> None:
     msg = lib().vxr_last_error().decode()
     cls = {EINVAL: InvalidArgument, ESTATE: InvalidState, EUNSUPPORTED: Unsupported}.get(rc, VxrError)
     raise cls(rc, msg)
+
+
+def vxs_check(rc: int) -> None:
+    """Status of a surrogate kernel call: 0, 1 (bad argument) or 3 (HIP error)."""
+    if rc == 0:
+        return
+    msg = lib().vxs_last_error().decode()
+    raise (InvalidArgument if rc == 1 else VxrError)(rc, msg)
 
 
 def device_count() -> int:
