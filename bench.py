@@ -168,6 +168,76 @@ def bench_small2(steps, graph):
             "graph": graph}
 
 
+def bench_surrogate(iters, n=1024):
+    """The convolutional surrogate of 8 HLL steps at 1024^2: trained here (deterministically) on one
+    pair from each of 2,000 seeded dam-and-bumps trajectories, judged against the FP64 HLL on 200
+    held-out seeds; its time per application against the FP32 HLL's for 8 steps."""
+    import torch
+
+    from vexhip import patch, scenes as sc_mod, surrogate
+
+    t0 = time.perf_counter()
+    model, losses = surrogate.train(n=n, seeds=range(2000), iters=iters)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    quality = surrogate.evaluate(model, n=n, seeds=range(10000, 10200))
+    hll = surrogate.Hll64(n)
+    hll.scene(0)
+    hll.step(64)
+    state = hll.state32()
+    out = model.forward(*state)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    reps = 20
+    ev[0].record()
+    for _ in range(reps):
+        model.forward(*state, out=out)
+    ev[1].record()
+    layer = (64, 64)
+    ev[2].record()
+    for _ in range(reps):
+        model._conv(1, model.acts[0], model.acts[1], True)
+    ev[3].record()
+    torch.cuda.synchronize()
+    fwd_ms = ev[0].elapsed_time(ev[1]) / reps
+    conv_ms = ev[2].elapsed_time(ev[3]) / reps
+    ev[0].record()
+    for _ in range(reps):
+        hll.step(surrogate.STEPS)
+    ev[1].record()
+    torch.cuda.synchronize()
+    hll64_ms = ev[0].elapsed_time(ev[1]) / reps
+    h, hu, hv = sc_mod.swe_dam_bumps(n, seed=0)
+    sim = patch.PatchSimulation(n, n)
+    sim.load(h, hu, hv, patch.params(9.81, 0.45, 1.0 / n, 1.0 / n))
+    sim.write_budget(2**31 - 1)
+    sim.dispatch(surrogate.STEPS)
+    sim.sim.finish()
+    sim.sim.timing(True)
+    sim.dispatch(surrogate.STEPS * reps)
+    ms, launches = sim.sim.kernel_time("hll_step")
+    sim.close()
+    hll32_ms = ms / launches * surrogate.STEPS
+    flop = 2.0 * n * n * 9 * sum(ci * co for ci, co in ((3, 64), (64, 64), (64, 64), (64, 64), (64, 64), (64, 3)))
+    conv_flop = 2.0 * n * n * 9 * layer[0] * layer[1]
+    cells = n * n * surrogate.STEPS
+    return {"workload": f"residual 3x3 conv net (16->64, 4x 64->64, 64->16), bf16 MFMA, FP32 accumulate, one "
+                        f"application = {surrogate.STEPS} HLL steps of fixed dt at {n}^2; trained on 2,000 "
+                        f"dam-and-bumps trajectories (seeds 0-1999), judged on seeds 10000-10199 vs FP64 HLL",
+            "train_iters": iters, "train_seconds": round(train_s, 1), "final_loss": losses[-1],
+            "quality": quality,
+            "surrogate_ms_per_application": round(fwd_ms, 4),
+            "surrogate_cell_updates_per_s": round(cells / (fwd_ms * 1e-3), 1),
+            "hll_fp32_ms_per_8_steps": round(hll32_ms, 4),
+            "hll_fp32_cell_updates_per_s": round(cells / (hll32_ms * 1e-3), 1),
+            "hll_fp64_ms_per_8_steps": round(hll64_ms, 4),
+            "surrogate_speed_vs_hll_fp32": round(hll32_ms / fwd_ms, 4),
+            "roofline": {"kernel": "conv3x3_lds 64->64", "bound": "mfma",
+                         "achieved": round(conv_flop / (conv_ms * 1e-3) / 1e12, 1), "peak": 2500.0,
+                         "unit": "TFLOP/s", "frac": round(conv_flop / (conv_ms * 1e-3) / 1e12 / 2500.0, 4),
+                         "avg_us": round(conv_ms * 1e3, 2)},
+            "forward_tflops": round(flop / (fwd_ms * 1e-3) / 1e12, 1)}
+
+
 def cpu_baseline_mpm3():
     """The oracle (scalar C++, one thread) on one whole substep of the same 8,388,608-particle
     workload: about 10-30 s of CPU."""
@@ -226,6 +296,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hll", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--surrogate-iters", type=int, default=2000)
+    ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (counter calibration) and exit")
@@ -238,6 +310,8 @@ def main():
               "the design notes say how it would shard)", file=sys.stderr)
         sys.exit(2)
 
+    if not args.no_surrogate:
+        import torch  # noqa: F401  (first: the surrogate's buffers are torch's, on torch's HIP runtime)
     from vexhip import _native
     if _native.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
@@ -282,6 +356,10 @@ def main():
         print("[bench] 2D 8,192 particles on 128^2, eager and graph", file=sys.stderr, flush=True)
         line["small2d"] = {"eager": bench_small2(500, False), "graph": bench_small2(500, True),
                            "workload": "2D MLS-MPM, 8,192 particles, 128^2, fixed-point P2G, sort every 10"}
+    if not args.no_surrogate:
+        print(f"[bench] surrogate: train {args.surrogate_iters} iterations at 1024^2, judge, time",
+              file=sys.stderr, flush=True)
+        line["surrogate"] = bench_surrogate(args.surrogate_iters)
     if not args.no_cpu:
         print("[bench] CPU oracle baseline (one substep, one thread)", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline_mpm3()

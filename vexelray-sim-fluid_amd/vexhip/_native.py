@@ -6,6 +6,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import sys
 
 import numpy as np
 
@@ -38,11 +39,13 @@ class Unsupported(VxrError):
 
 
 _lib = None
+_torch_first = False  # whether torch (and its HIP runtime) was loaded before this library
 
 
 def lib() -> ctypes.CDLL:
-    global _lib
+    global _lib, _torch_first
     if _lib is None:
+        _torch_first = "torch" in sys.modules
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is not built: run vexelray-sim-fluid_amd/build.py")
         l = ctypes.CDLL(LIB_PATH)

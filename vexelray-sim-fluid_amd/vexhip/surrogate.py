@@ -34,8 +34,13 @@ _torch = None
 
 
 def _t():
+    """torch, for device memory and the stream. It must be imported before libvexelhip.so is
+    loaded, so that both use torch's HIP runtime."""
     global _torch
     if _torch is None:
+        if _native._lib is not None and not _native._torch_first:
+            raise RuntimeError("import torch before libvexelhip.so is loaded: the surrogate's buffers are torch's, "
+                               "and both must use one HIP runtime")
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("the surrogate runs on a HIP device; none is visible")
