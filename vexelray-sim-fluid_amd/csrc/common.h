@@ -69,7 +69,16 @@ struct Mpm3Args {
     float* p[17];
     float* g[7];  // gm gmu gmv gmw gu gv gw
     const int32_t* params;
+    // Bricks of 8x8x8 nodes: P2G sets `touched` for every brick its stencils reach; the grid pass
+    // then moves it to `live` (bricks whose sums and velocities may be nonzero) and clears it.
+    // Clear and grid visit, in each row of nodes, only the span of bricks flagged in either.
+    uint8_t* touched;
+    uint8_t* live;
 };
+constexpr int kBrick = 8;
+inline int64_t brick_count3(int nx, int ny, int nz) {
+    return (int64_t)((nx + kBrick - 1) / kBrick) * ((ny + kBrick - 1) / kBrick) * ((nz + kBrick - 1) / kBrick);
+}
 int mpm3_clear(const Launch& l, const Mpm3Args& a);
 int mpm3_scatter_atomic(const Launch& l, const Mpm3Args& a);
 // Chunks of 256 * run particles, run in {1, 2, 4}.

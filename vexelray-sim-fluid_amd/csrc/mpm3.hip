@@ -92,19 +92,71 @@ __device__ __forceinline__ void load_particle(const Mpm3Args& a, int64_t q, floa
     p.s = (4.0f * (dt * (p.m / rho0))) * (bulk * (1.0f - a.p[7][q]));
 }
 
-// ------------------------------------------------------------------ clear
+// ------------------------------------------------------------------ touched bricks, clear
 
-
-__global__ void clear4_kernel(float* __restrict__ g0, float* __restrict__ g1, float* __restrict__ g2,
-                              float* __restrict__ g3, int64_t n4) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const float4 z = make_float4(0, 0, 0, 0);
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
-        reinterpret_cast<float4*>(g0)[i] = z;
-        reinterpret_cast<float4*>(g1)[i] = z;
-        reinterpret_cast<float4*>(g2)[i] = z;
-        reinterpret_cast<float4*>(g3)[i] = z;
+// Marks the bricks holding nodes [lo, hi] (inclusive, per axis) as touched; lanes of a wave
+// share the stores.
+__device__ __forceinline__ void mark_bricks(const Mpm3Args& a, const int (&lo)[3], const int (&hi)[3], int lane,
+                                            int lanes) {
+    const int bxn = (a.nx + kBrick - 1) / kBrick, byn = (a.ny + kBrick - 1) / kBrick;
+    const int b0[3] = {lo[0] / kBrick, lo[1] / kBrick, lo[2] / kBrick};
+    const int e[3] = {hi[0] / kBrick - b0[0] + 1, hi[1] / kBrick - b0[1] + 1, hi[2] / kBrick - b0[2] + 1};
+    const int total = e[0] * e[1] * e[2];
+    for (int t = lane; t < total; t += lanes) {
+        const int bx = b0[0] + t % e[0], by = b0[1] + (t / e[0]) % e[1], bz = b0[2] + t / (e[0] * e[1]);
+        a.touched[((int64_t)bz * byn + by) * bxn + bx] = 1;
     }
+}
+
+// A wave a row of nodes (y, z): the x-range from the first to the last brick of the row flagged
+// in `touched` or `live` (brick rows of 64 at a time). Returns false when none is.
+__device__ __forceinline__ bool row_range(const Mpm3Args& a, int y, int z, int lane, int* x0, int* x1) {
+    const int bxn = (a.nx + kBrick - 1) / kBrick, byn = (a.ny + kBrick - 1) / kBrick;
+    const int64_t base = ((int64_t)(z / kBrick) * byn + y / kBrick) * bxn;
+    int first = INT_MAX, last = -1;
+    for (int b0 = 0; b0 < bxn; b0 += kWave) {
+        const int b = b0 + lane;
+        const bool on = b < bxn && (a.touched[base + b] | a.live[base + b]) != 0;
+        const uint64_t m = __ballot(on);
+        if (m) {
+            first = min(first, b0 + (int)__builtin_ctzll(m));
+            last = b0 + 63 - (int)__builtin_clzll(m);
+        }
+    }
+    if (last < 0) return false;
+    *x0 = first * kBrick;
+    *x1 = min((last + 1) * kBrick, a.nx);
+    return true;
+}
+
+// Zeroes the sums over the flagged range of every row (rows of 4 nodes in float4 when aligned).
+__global__ __launch_bounds__(256) void clear_rows_kernel(Mpm3Args a) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= (int64_t)a.ny * a.nz) return;
+    const int y = (int)(row % a.ny), z = (int)(row / a.ny);
+    int x0, x1;
+    if (!row_range(a, y, z, lane, &x0, &x1)) return;
+    const int64_t first = row * a.nx;
+    if ((a.nx & 3) == 0) {
+        const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int x = x0 + 4 * lane; x < x1; x += 4 * kWave)
+#pragma unroll
+            for (int f = 0; f < 4; ++f) *reinterpret_cast<float4*>(a.g[f] + first + x) = zero;
+    } else {
+        for (int x = x0 + lane; x < x1; x += kWave)
+#pragma unroll
+            for (int f = 0; f < 4; ++f) a.g[f][first + x] = 0.0f;
+    }
+}
+
+// After a grid pass: the velocities are nonzero at most where P2G reached; the sums will be
+// zeroed there by the next clear. live := touched, touched := 0.
+__global__ void flags_kernel(uint8_t* __restrict__ touched, uint8_t* __restrict__ live, int64_t bricks) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= bricks) return;
+    live[b] = touched[b];
+    touched[b] = 0;
 }
 
 // ------------------------------------------------------------------ P2G, the reference schedule
@@ -138,6 +190,9 @@ __global__ __launch_bounds__(kBlock) void p2g3_atomic_kernel(Mpm3Args a) {
     const float dt = bits_f(a.params[DT]), bulk = bits_f(a.params[BULK]), rho0 = bits_f(a.params[RHO0]);
     Particle p;
     load_particle(a, q, dt, bulk, rho0, p);
+    const int lo[3] = {p.ax[0].cell, p.ax[1].cell, p.ax[2].cell};
+    const int hi[3] = {lo[0] + 2, lo[1] + 2, lo[2] + 2};
+    mark_bricks(a, lo, hi, 0, 1);
     atomic_particle(a, p.ax, p.m, p.vel, p.c, p.s);
 }
 
@@ -417,6 +472,11 @@ __global__ __launch_bounds__(kWave) void p2g3_wave_kernel(Mpm3Args a) {
     }
     const Box box = wave_box(lo, hi, kWaveTile);
     if (box.empty) return;
+    {
+        const int nlo[3] = {box.lo[0], box.lo[1], box.lo[2]};
+        const int nhi[3] = {box.hi[0] + 2, box.hi[1] + 2, box.hi[2] + 2};
+        mark_bricks(a, nlo, nhi, lane, kWave);
+    }
     // A box larger than the tile goes through it in slabs across its longest axis, each slab
     // taking the particles whose base cell lies in it; past kMaxPasses slabs the runs go to
     // global memory directly.
@@ -492,24 +552,69 @@ __global__ __launch_bounds__(kWave) void p2g3_wave_kernel(Mpm3Args a) {
 
 // ------------------------------------------------------------------ grid update, Flip3.java:267-311
 
-__global__ __launch_bounds__(kBlock) void grid3_kernel(Mpm3Args a) {
-    const int64_t nodes = (int64_t)a.nx * a.ny * a.nz;
-    const int64_t node = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (node >= nodes) return;
-    const int at[3] = {(int)(node % a.nx), (int)((node / a.nx) % a.ny), (int)(node / ((int64_t)a.nx * a.ny))};
+// Flip3.grid for node `node` at `at` with its sums.
+__device__ __forceinline__ void grid_node(const Mpm3Args& a, const int (&at)[3], float mass, const float (&mom)[3],
+                                          float dt, float (&vel)[3]) {
     const int size[3] = {a.nx, a.ny, a.nz};
-    const float mass = a.g[0][node];
-    float vel[3] = {0.0f, 0.0f, 0.0f};
-    if (mass > kEmpty) {
-        const float dt = bits_f(a.params[DT]);
 #pragma unroll
-        for (int i = 0; i < 3; ++i) vel[i] = a.g[1 + i][node] / mass + dt * bits_f(a.params[GX + i]);
+    for (int i = 0; i < 3; ++i) vel[i] = 0.0f;
+    if (mass > kEmpty) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) vel[i] = mom[i] / mass + dt * bits_f(a.params[GX + i]);
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         if (!(at[i] > 1)) vel[i] = mx(vel[i], 0.0f);
         if (at[i] > size[i] - 3) vel[i] = mn(vel[i], 0.0f);
-        a.g[4 + i][node] = vel[i];
+    }
+}
+
+// Flip3.grid over the flagged range of every row. Inside it, a node of a brick P2G did not reach
+// holds zero sums (the clear zeroed them, nothing added since), so the formula gives it zero
+// velocity -- what the full pass gives it -- and overwrites what it held; outside the range every
+// node already holds zero sums and zero velocity.
+__global__ __launch_bounds__(256) void grid3_kernel(Mpm3Args a) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= (int64_t)a.ny * a.nz) return;
+    const int y = (int)(row % a.ny), z = (int)(row / a.ny);
+    int x0, x1;
+    if (!row_range(a, y, z, lane, &x0, &x1)) return;
+    const int64_t first = row * a.nx;
+    const float dt = bits_f(a.params[DT]);
+    if ((a.nx & 3) == 0) {
+        for (int x = x0 + 4 * lane; x < x1; x += 4 * kWave) {
+            float4 v[4];
+#pragma unroll
+            for (int f = 0; f < 4; ++f) v[f] = *reinterpret_cast<const float4*>(a.g[f] + first + x);
+            const float m4[4] = {v[0].x, v[0].y, v[0].z, v[0].w};
+            const float u4[4] = {v[1].x, v[1].y, v[1].z, v[1].w};
+            const float w4[4] = {v[2].x, v[2].y, v[2].z, v[2].w};
+            const float q4[4] = {v[3].x, v[3].y, v[3].z, v[3].w};
+            float out[3][4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int at[3] = {x + k, y, z};
+                const float mom[3] = {u4[k], w4[k], q4[k]};
+                float vel[3];
+                grid_node(a, at, m4[k], mom, dt, vel);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) out[i][k] = vel[i];
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                *reinterpret_cast<float4*>(a.g[4 + i] + first + x) = make_float4(out[i][0], out[i][1], out[i][2], out[i][3]);
+        }
+    } else {
+        for (int x = x0 + lane; x < x1; x += kWave) {
+            const int64_t node = first + x;
+            const int at[3] = {x, y, z};
+            const float mom[3] = {a.g[1][node], a.g[2][node], a.g[3][node]};
+            float vel[3];
+            grid_node(a, at, a.g[0][node], mom, dt, vel);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) a.g[4 + i][node] = vel[i];
+        }
     }
 }
 
@@ -726,13 +831,8 @@ int mpm3_key_bits(int nx, int ny, int nz) {
 }
 
 int mpm3_clear(const Launch& l, const Mpm3Args& a) {
-    const int64_t nodes = (int64_t)a.nx * a.ny * a.nz;
-    const int64_t n4 = (nodes + 3) / 4;  // node arrays are allocated in whole float4s
-    int64_t blocks = blocks_for(n4, kBlock);
-    if (blocks > 16384) blocks = 16384;
     timed_begin(l, "mpm3_clear");
-    hipLaunchKernelGGL(clear4_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, l.stream, a.g[0], a.g[1], a.g[2],
-                       a.g[3], n4);
+    hipLaunchKernelGGL(clear_rows_kernel, dim3(blocks_for((int64_t)a.ny * a.nz, 4)), dim3(256), 0, l.stream, a);
     timed_end(l, "mpm3_clear");
     return check_launch("mpm3_clear");
 }
@@ -761,9 +861,10 @@ int mpm3_scatter_tiled(const Launch& l, const Mpm3Args& a, int run) {
 }
 
 int mpm3_grid(const Launch& l, const Mpm3Args& a) {
-    const int64_t nodes = (int64_t)a.nx * a.ny * a.nz;
     timed_begin(l, "grid3");
-    hipLaunchKernelGGL(grid3_kernel, dim3(blocks_for(nodes, kBlock)), dim3(kBlock), 0, l.stream, a);
+    hipLaunchKernelGGL(grid3_kernel, dim3(blocks_for((int64_t)a.ny * a.nz, 4)), dim3(256), 0, l.stream, a);
+    const int64_t bricks = brick_count3(a.nx, a.ny, a.nz);
+    hipLaunchKernelGGL(flags_kernel, dim3(blocks_for(bricks, 256)), dim3(256), 0, l.stream, a.touched, a.live, bricks);
     timed_end(l, "grid3");
     return check_launch("grid3");
 }

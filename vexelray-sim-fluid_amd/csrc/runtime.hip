@@ -147,6 +147,9 @@ struct vxr_sim {
     int32_t* scratch = nullptr;
     int32_t* starts = nullptr;  // 2D: cells + 1
     int64_t nodes = 0;
+    uint8_t* touched = nullptr;  // 3D: bricks P2G reached, and bricks holding velocities
+    uint8_t* live = nullptr;
+    int64_t bricks = 0;
 
     // SWE
     float* state[2][3] = {};
@@ -265,7 +268,18 @@ vxr::Mpm3Args mpm3_args(vxr_sim* s) {
     for (int f = 0; f < 17; ++f) a.p[f] = s->p[f];
     for (int f = 0; f < 7; ++f) a.g[f] = s->g[f];
     a.params = s->params;
+    a.touched = s->touched;
+    a.live = s->live;
     return a;
+}
+
+// Every brick flagged: the next clear and grid passes cover the whole grid (after the host wrote
+// grid fields, or cleared everything). Never inside a captured graph.
+int mark_all_bricks(vxr_sim* s) {
+    if (!s->touched) return 0;
+    HIP_TRY(hipMemsetAsync(s->touched, 1, s->bricks, s->stream));
+    HIP_TRY(hipMemsetAsync(s->live, 1, s->bricks, s->stream));
+    return 0;
 }
 
 vxr::Mpm2Args mpm2_args(vxr_sim* s) {
@@ -561,6 +575,7 @@ int clear_all(vxr_sim* s) {
         for (float* f : s->p_alt) HIP_TRY(hipMemsetAsync(f, 0, s->capacity * 4, s->stream));
         for (float* f : s->g) HIP_TRY(hipMemsetAsync(f, 0, round_up(s->nodes, 4) * 4, s->stream));
         HIP_TRY(hipMemsetAsync(s->params, 0, s->param_words * 4, s->stream));
+        if (int rc = mark_all_bricks(s)) return rc;
         HIP_TRY(hipMemsetAsync(s->keys, 0, s->capacity * 4, s->stream));
         HIP_TRY(hipMemsetAsync(s->index, 0, s->capacity * 4, s->stream));
         if (s->starts) HIP_TRY(hipMemsetAsync(s->starts, 0, s->starts_len * 4, s->stream));
@@ -756,6 +771,11 @@ int vxr_create(int kind, const int32_t dims[3], int64_t particles, uint32_t flag
         if (!rc) rc = alloc(s, s->capacity, &ptr), s->keys = (int32_t*)ptr;
         if (!rc) rc = alloc(s, s->capacity, &ptr), s->index = (int32_t*)ptr;
         if (!rc) rc = alloc(s, vxr::radix_scratch_words(s->capacity, 8), &ptr), s->scratch = (int32_t*)ptr;
+        if (!rc && three) {
+            s->bricks = vxr::brick_count3(s->nx, s->ny, s->nz);
+            rc = alloc(s, (s->bricks + 3) / 4, &ptr), s->touched = (uint8_t*)ptr;
+            if (!rc) rc = alloc(s, (s->bricks + 3) / 4, &ptr), s->live = (uint8_t*)ptr;
+        }
         if (!rc && !three) {
             s->starts_len = (int64_t)(s->nx - 1) * (s->ny - 1) + 1;
             rc = alloc(s, s->starts_len, &ptr), s->starts = (int32_t*)ptr;
@@ -886,6 +906,8 @@ int vxr_write(vxr_sim* s, const char* name, const void* words, int64_t n) {
     if (n != b.len && !(prefix_ok && n >= 0 && n <= b.len))
         return fail(VXR_EINVAL, "buffer '%s' holds %lld words, got %lld", name, (long long)b.len, (long long)n);
     HIP_TRY(hipMemcpyAsync(b.ptr, words, n * 4, hipMemcpyHostToDevice, s->stream));
+    if (s->kind == VXR_KIND_MPM3 && name[0] == 'g' && name[1] != 0)  // a grid field: sweep it all next time
+        if (int rc = mark_all_bricks(s)) return rc;
     HIP_TRY(hipStreamSynchronize(s->stream));
     return VXR_OK;
 }
