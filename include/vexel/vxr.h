@@ -122,6 +122,27 @@ int vxr_run_pass(vxr_sim* sim, const char* pass, int repeat);
  * (gui/PatchSimulation.java:197-205, core/stencil/ShallowWater.java:92-94). */
 int vxr_swe_dispatch(vxr_sim* sim, int count);
 
+/* Diagnostics.of (core/stencil/Diagnostics.java:48-81) of the current shallow-water state, on the
+ * device: out = volume, min depth, max depth, max speed, max Froude, max Courant, non-finite,
+ * negative and wet cell counts. Minima, maxima and counts are the sequential loop's exactly; the
+ * volume is summed in a fixed tree order. Synchronous. */
+int vxr_swe_diagnostics(vxr_sim* sim, double g, double dry, double dt, double dx, double dy, double out[9]);
+
+/* ParticleDiagnostics.of (core/particle/ParticleDiagnostics.java:45-65) of the particles' J, on
+ * the device: out = min density, max density, low density (99% above), high density (99% below),
+ * mach, courant, non-finite count. Reuses the sort's key buffers ("keys" then holds J's bits).
+ * For a batch, over every scene's particles together. Synchronous. */
+int vxr_particle_diagnostics(vxr_sim* sim, double speed, double sound, double dt, double out[7]);
+
+/* The fixed-point bilinear scatter (core/particle/FixedScatter.java:151-183: FixedScatter.kernel,
+ * schedule 0, and FixedScatter.segmented, schedule 1) of `count` particles (device arrays x y u v m,
+ * node units) into the i32 grids gridMq, gridMuq, gridMvq of an n x n grid (device arrays, added to),
+ * at the scale FixedScatter.FixedPoint(mass_bits, velocity_bits) (:91-148; each 0..30). Queued on
+ * `stream` (a hipStream_t; 0 the null stream). Every schedule and order gives the same bits. */
+int vxr_fixed_scatter(void* stream, int n, int64_t count, const float* x, const float* y, const float* u,
+                      const float* v, const float* m, int mass_bits, int velocity_bits, int schedule, int32_t* grid_m,
+                      int32_t* grid_mu, int32_t* grid_mv);
+
 /* Zeroes every buffer (PassRunner.clear). */
 int vxr_clear(vxr_sim* sim);
 

@@ -72,6 +72,12 @@ def _load() -> ctypes.CDLL:
         "orc_mpm2_cell_starts": (_c_i64, [_c_i, _c_i, _c_i64, _f32p, _f32p, _i32p]),
         "orc_mpm2_sort": (None, [_c_i, _c_i, _c_i64, _vp]),
         "orc_stable_order": (None, [_c_i64, _i32p, _i32p]),
+        "orc_particle_diagnostics": (None, [_c_i64, _f32p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                            _f64p]),
+        "orc_fixed_split": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int32]),
+        "orc_fixed_scatter": (None, [_c_i, _c_i64, _f32p, _f32p, _f32p, _f32p, _f32p, _c_i, _c_i, _i32p, _i32p,
+                                     _i32p]),
+        "orc_scatter_f64": (None, [_c_i, _c_i64, _f32p, _f32p, _f32p, _f32p, _f32p, _f64p, _f64p, _f64p]),
         "orc_bucket_starts": (None, [_c_i64, _i32p, ctypes.c_int32, _i32p]),
     }
     for name, (res, args) in sig.items():
@@ -399,3 +405,74 @@ def bucket_starts(keys, cells) -> np.ndarray:
     out = np.empty(cells + 1, np.int32)
     lib.orc_bucket_starts(len(keys), keys, cells, out)
     return out
+
+
+# ------------------------------------------------------------------ the fixed-point bilinear scatter
+
+FIXED_OFFSET_BITS = 10
+FIXED_MAX_MASS = 1 << 21
+
+
+def fixed_split(q: int, mass: int) -> int:
+    """Quanta's share floor(q * M / 2^20) in i32 (core/particle/FixedScatter.java:186-193)."""
+    return lib.orc_fixed_split(int(q), int(mass))
+
+
+def fixed_scatter(n, p: dict, mass_bits, velocity_bits, grid=None):
+    """FixedScatter's deposit into three i32 grids (added to `grid` if given), the host restatement
+    of test/particle/FixedScatterTest.java:402-432."""
+    grid = [np.zeros(n * n, np.int32) for _ in range(3)] if grid is None else grid
+    lib.orc_fixed_scatter(n, len(p["x"]), *(_c(p[k]) for k in ("x", "y", "u", "v", "m")), int(mass_bits),
+                          int(velocity_bits), *grid)
+    return grid
+
+
+def scatter_f64(n, p: dict):
+    """The FP64 bilinear deposit of test/particle/ScatterTest.java:429-450."""
+    grid = [np.zeros(n * n, np.float64) for _ in range(3)]
+    lib.orc_scatter_f64(n, len(p["x"]), *(_c(p[k]) for k in ("x", "y", "u", "v", "m")), *grid)
+    return grid
+
+
+def jittered_particles(cells, ppc, rng: "JavaRandom") -> dict:
+    """ScatterTest.Particles.jittered (test/particle/ScatterTest.java:460-478): per cell in row,
+    column order, per particle nextFloat x, nextFloat y, nextGaussian u, nextGaussian v, nextFloat m;
+    x kept below col + 1 in binary32."""
+    count = cells * cells * ppc
+    out = {k: np.empty(count, np.float32) for k in ("x", "y", "u", "v", "m")}
+    k = 0
+    for row in range(cells):
+        for col in range(cells):
+            for _ in range(ppc):
+                below_x = np.nextafter(np.float32(col + 1), np.float32(0))
+                below_y = np.nextafter(np.float32(row + 1), np.float32(0))
+                out["x"][k] = min(np.float32(np.float32(col) + np.float32(rng.next_float())), below_x)
+                out["y"][k] = min(np.float32(np.float32(row) + np.float32(rng.next_float())), below_y)
+                out["u"][k] = np.float32(rng.next_gaussian())
+                out["v"][k] = np.float32(rng.next_gaussian())
+                out["m"][k] = np.float32(np.float32(0.5) + np.float32(rng.next_float()))
+                k += 1
+    return out
+
+
+def shuffled(p: dict, rng: "JavaRandom") -> dict:
+    """ScatterTest.Particles.shuffled (:481-493): Fisher-Yates with nextInt(k + 1), k from the end."""
+    count = len(p["x"])
+    order = list(range(count))
+    for k in range(count - 1, 0, -1):
+        j = rng.next_int(k + 1)
+        order[k], order[j] = order[j], order[k]
+    return {key: v[np.array(order)] for key, v in p.items()}
+
+
+PARTICLE_DIAGNOSTICS = ("min_density", "max_density", "low_density", "high_density", "mach", "courant", "non_finite")
+
+
+def particle_diagnostics(j, speed, sound, dt) -> dict:
+    """ParticleDiagnostics.of (core/particle/ParticleDiagnostics.java:45-65)."""
+    out = np.zeros(7, np.float64)
+    j = _c(np.asarray(j, np.float32))
+    lib.orc_particle_diagnostics(len(j), j, float(speed), float(sound), float(dt), out)
+    d = dict(zip(PARTICLE_DIAGNOSTICS, out.tolist()))
+    d["non_finite"] = int(d["non_finite"])
+    return d

@@ -115,6 +115,23 @@ int64_t orc_mpm2_cell_starts(int nx, int ny, int64_t count, const float* x, cons
 /* The sort by cell with the copy back (core/particle/FlipStep.java:183-192), made stable. */
 void orc_mpm2_sort(int nx, int ny, int64_t count, float* const p[10]);
 
+/* ParticleDiagnostics.of (core/particle/ParticleDiagnostics.java:45-65): out = min density,
+ * max density, low (99% above), high (99% below), mach, courant, non-finite count. */
+void orc_particle_diagnostics(int64_t count, const float* j, double speed, double sound, double dt, double out[7]);
+
+/* ---- the fixed-point bilinear scatter (core/particle/FixedScatter.java) ----
+ * orc_fixed_scatter adds into i32 grids (wrapping) what FixedScatter.kernel/segmented deposit:
+ * test/particle/FixedScatterTest.java:402-432. orc_fixed_split is Quanta's share,
+ * floor(q * M / 2^20) in i32 (FixedScatter.java:186-193). orc_scatter_f64 is the FP64 bilinear
+ * deposit of test/particle/ScatterTest.java:429-450. */
+#define ORC_FIXED_OFFSET_BITS 10
+#define ORC_FIXED_MAX_MASS (1 << 21)
+int32_t orc_fixed_split(int32_t q, int32_t mass);
+void orc_fixed_scatter(int n, int64_t count, const float* x, const float* y, const float* u, const float* v,
+                       const float* m, int mass_bits, int velocity_bits, int32_t* gm, int32_t* gmu, int32_t* gmv);
+void orc_scatter_f64(int n, int64_t count, const float* x, const float* y, const float* u, const float* v,
+                     const float* m, double* gm, double* gmu, double* gmv);
+
 /* ---- stable sort by key: order[i] = index of the i-th particle in key order ---- */
 void orc_stable_order(int64_t count, const int32_t* keys, int32_t* order);
 /* Exclusive starts of `cells` buckets, cells + 1 words (the last is `count`). */

@@ -5,6 +5,9 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <vector>
+
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -256,6 +259,33 @@ void orc_swe_step_f64(int nx, int ny, const int edges[4], double* out_h, double*
         out_hu[cell] = (c.hu - rx * (fe[1] - fw[1])) - ry * (fn[2] - fs[2]);
         out_hv[cell] = (c.hv - rx * (fe[2] - fw[2])) - ry * (fn[1] - fs[1]);
     }
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// ParticleDiagnostics.of (core/particle/ParticleDiagnostics.java:45-65): the valid J (finite and
+// above zero) sorted; density 1/J at the extremes and at the 1st and 99th percentile.
+void orc_particle_diagnostics(int64_t count, const float* j, double speed, double sound, double dt, double out[7]) {
+    std::vector<float> valid;
+    valid.reserve((size_t)count);
+    for (int64_t k = 0; k < count; k++)
+        if (std::isfinite(j[k]) && j[k] > 0) valid.push_back(j[k]);
+    const int64_t n = (int64_t)valid.size();
+    out[4] = speed / sound;
+    out[5] = (speed + sound) * dt;
+    out[6] = (double)(count - n);
+    if (n == 0) {
+        out[0] = out[1] = out[2] = out[3] = 0;
+        return;
+    }
+    std::sort(valid.begin(), valid.end());
+    auto percentile = [&](double p) { return std::min<int64_t>(n - 1, (int64_t)(p * (double)n)); };
+    out[0] = 1 / (double)valid[n - 1];
+    out[1] = 1 / (double)valid[0];
+    out[2] = 1 / (double)valid[percentile(0.99)];
+    out[3] = 1 / (double)valid[percentile(0.01)];
 }
 
 }  // extern "C"

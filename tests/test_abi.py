@@ -65,6 +65,10 @@ def test_arguments_refused_before_the_device():
     assert lib.vxr_create(_native.KIND_MPM2_BATCH, dims, 10, 0, ctypes.byref(h)) == _native.EINVAL
     assert lib.vxr_advance(None, 1, 10, 0) == _native.EINVAL
     assert lib.vxr_autotune(None, 1) == _native.EINVAL
+    # the fixed-point scatter checks its scale and schedule before queueing anything
+    assert lib.vxr_fixed_scatter(None, 8, 0, None, None, None, None, None, 31, 10, 0, 1, 1, 1) == _native.EINVAL
+    assert lib.vxr_fixed_scatter(None, 8, 0, None, None, None, None, None, 10, 10, 2, 1, 1, 1) == _native.EINVAL
+    assert lib.vxr_fixed_scatter(None, 1, 0, None, None, None, None, None, 10, 10, 0, 1, 1, 1) == _native.EINVAL
 
 
 def test_params_words():

@@ -134,6 +134,27 @@ int batch2_advect(const Launch& l, const Batch2Args& a, int blocks);
 int batch2_keys(const Launch& l, const Batch2Args& a, int32_t* keys, int32_t* index, int blocks);
 int batch2_starts(const Launch& l, const Batch2Args& a, const int32_t* keys, int32_t* starts, int blocks);
 
+// ------------------------------------------------------------------ fixed-point bilinear scatter
+
+// core/particle/FixedScatter.java: run <= 0 one particle a thread, else runs of `run` consecutive
+// particles folded before their atomics. Adds into the i32 grids (n x n).
+int fixed_scatter(const Launch& l, int n, int64_t count, const float* x, const float* y, const float* u,
+                  const float* v, const float* m, int mass_bits, int velocity_bits, int run, int* gm, int* gmu,
+                  int* gmv);
+
+// ------------------------------------------------------------------ diagnostics
+
+// Diagnostics.of on the device: out = volume, min depth, max depth, max speed, max Froude, max
+// Courant, non-finite, negative, wet. Scratch of diag_scratch_words words. Synchronous.
+int64_t diag_scratch_words();
+int swe_diagnostics(const Launch& l, const float* h, const float* hu, const float* hv, int64_t cells, double g,
+                    double dry, double dt, double dx, double dy, int32_t* scratch, double out[9]);
+// ParticleDiagnostics.of on the device, sorting the J keys in keys/index (capacity words each)
+// with the radix sort's scratch; `small`: 16 words. Synchronous.
+int particle_diagnostics(const Launch& l, const float* j, int64_t count, int64_t capacity, int32_t* keys,
+                         int32_t* index, int32_t* sort_scratch, int digit_bits, int32_t* small, double speed,
+                         double sound, double dt, double out[7]);
+
 // ------------------------------------------------------------------ sort
 
 // Stable LSD radix sort of (key, index) pairs, keys in [0, 2^bits). Scratch sized by

@@ -374,17 +374,50 @@ __device__ __forceinline__ void clear_tile(float4* tile, const Slab& sl) {
 // Called by the whole wave; `mine` says whether this lane has a run at tile node `run` to flush.
 // Lanes flushing the same node take turns: each writes its lane id into claim[run] and the lane
 // it reads back wins the round, so within a round no two lanes touch one node at one offset.
+// When every winner's base lies in one z layer (the usual case for cell-sorted particles), the
+// three z layers of their stencils cannot meet, so the three read-add-writes of one (dx, dy)
+// go together and the round is a chain of 9 instead of 27.
 __device__ __forceinline__ void flush_run(float4* tile, int* claim, const Slab& sl, int run, bool mine,
                                           float (&acc)[108]) {
     const int lane = threadIdx.x;
     const int b = mine ? run : 0;  // lanes with nothing to flush never index with their run
     const int ex = sl.dims[0], ey = sl.dims[1];
+    const int layer = b / (ex * ey);
     bool pending = mine;
     while (__ballot(pending)) {
         if (pending) claim[b] = lane;
         lds_order();
         const bool won = pending && claim[b] == lane;
-        if (won) {
+        int zlo = won ? layer : INT_MAX, zhi = won ? layer : INT_MIN;
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+            zlo = min(zlo, __shfl_xor(zlo, off));
+            zhi = max(zhi, __shfl_xor(zhi, off));
+        }
+        if (zlo == zhi) {
+            if (won) {
+#pragma unroll
+                for (int j = 0; j < 9; ++j) {
+                    float4 t[3];
+                    int node[3];
+#pragma unroll
+                    for (int dz = 0; dz < 3; ++dz) {
+                        node[dz] = b + (dz * ey + j / 3) * ex + j % 3;
+                        t[dz] = tile[node[dz]];
+                    }
+#pragma unroll
+                    for (int dz = 0; dz < 3; ++dz) {
+                        const int k = 9 * dz + j;
+                        t[dz].x += acc[4 * k];
+                        t[dz].y += acc[4 * k + 1];
+                        t[dz].z += acc[4 * k + 2];
+                        t[dz].w += acc[4 * k + 3];
+                        tile[node[dz]] = t[dz];
+                    }
+                    lds_order();
+                }
+            }
+        } else if (won) {
 #pragma unroll
             for (int k = 0; k < 27; ++k) {
                 const int node = b + ((k / 9) * ey + (k / 3) % 3) * ex + k % 3;

@@ -172,6 +172,8 @@ struct vxr_sim {
     bool flipped = false;  // whether p holds the second set of particle buffers (each sort swaps)
     std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;  // (sort first, steps, flipped)
 
+    int32_t* diag = nullptr;  // diagnostics scratch, made on first use
+
     std::map<std::string, int64_t> options;
 
     vxr::Launch launch() const { return vxr::Launch{stream, timer}; }
@@ -982,6 +984,53 @@ int vxr_autotune(vxr_sim* s, int reps) {
     if (s->timer) return fail(VXR_ESTATE, "turn timing off before tuning");
     drop_graphs(s);
     return autotune(s, reps);
+}
+
+int vxr_fixed_scatter(void* stream, int n, int64_t count, const float* x, const float* y, const float* u,
+                      const float* v, const float* m, int mass_bits, int velocity_bits, int schedule, int32_t* grid_m,
+                      int32_t* grid_mu, int32_t* grid_mv) {
+    if (n < 2) return fail(VXR_EINVAL, "a grid needs at least two nodes each way, got %d", n);
+    if (count < 0) return fail(VXR_EINVAL, "negative particle count");
+    if (count > 0 && (!x || !y || !u || !v || !m)) return fail(VXR_EINVAL, "null particle field");
+    if (!grid_m || !grid_mu || !grid_mv) return fail(VXR_EINVAL, "null grid");
+    if (mass_bits < 0 || velocity_bits < 0 || mass_bits > 30 || velocity_bits > 30)
+        return fail(VXR_EINVAL, "bits out of range: %d, %d", mass_bits, velocity_bits);
+    if (schedule != 0 && schedule != 1) return fail(VXR_EINVAL, "schedule is 0 (direct) or 1 (segmented)");
+    const vxr::Launch l{(hipStream_t)stream, nullptr};
+    return vxr::fixed_scatter(l, n, count, x, y, u, v, m, mass_bits, velocity_bits, schedule ? 4 : 0, grid_m, grid_mu,
+                              grid_mv);
+}
+
+int vxr_swe_diagnostics(vxr_sim* s, double g, double dry, double dt, double dx, double dy, double out[9]) {
+    if (!s || !out) return fail(VXR_EINVAL, "null argument");
+    if (s->kind != VXR_KIND_SWE) return fail(VXR_ESTATE, "vxr_swe_diagnostics is for a shallow-water patch");
+    void* ptr = nullptr;
+    if (!s->diag) {
+        if (int rc = alloc(s, vxr::diag_scratch_words(), &ptr)) return rc;
+        s->diag = (int32_t*)ptr;
+    }
+    const int slot = (int)(s->dispatched % 2);
+    return vxr::swe_diagnostics(s->launch(), s->state[slot][0], s->state[slot][1], s->state[slot][2],
+                                (int64_t)s->nx * s->ny, g, dry, dt, dx, dy, s->diag, out);
+}
+
+int vxr_particle_diagnostics(vxr_sim* s, double speed, double sound, double dt, double out[7]) {
+    if (!s || !out) return fail(VXR_EINVAL, "null argument");
+    if (s->kind == VXR_KIND_SWE) return fail(VXR_ESTATE, "vxr_particle_diagnostics is for particles");
+    void* ptr = nullptr;
+    if (!s->diag) {
+        if (int rc = alloc(s, vxr::diag_scratch_words(), &ptr)) return rc;
+        s->diag = (int32_t*)ptr;
+    }
+    const float* j = s->p[s->kind == VXR_KIND_MPM3 ? 7 : 5];
+    const int64_t count = s->kind == VXR_KIND_MPM2_BATCH ? s->capacity : s->count;
+    // a batch's pad particles hold J = 0 and count as not valid; they are left out below
+    int64_t pads = 0;
+    if (s->kind == VXR_KIND_MPM2_BATCH) pads = s->capacity - s->count;
+    int rc = vxr::particle_diagnostics(s->launch(), j, count, s->capacity, s->keys, s->index, s->scratch,
+                                       (int)opt(s, "radix_bits"), s->diag, speed, sound, dt, out);
+    if (!rc && pads) out[6] -= (double)pads;
+    return rc;
 }
 
 int vxr_swe_dispatch(vxr_sim* s, int count) {

@@ -76,6 +76,12 @@ def lib() -> ctypes.CDLL:
             "vxr_scene_count": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int)]),
             "vxr_advance": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, i64]),
             "vxr_autotune": (ctypes.c_int, [vp, ctypes.c_int]),
+            "vxr_swe_diagnostics": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                   ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]),
+            "vxr_particle_diagnostics": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                        ctypes.POINTER(ctypes.c_double)]),
+            "vxr_fixed_scatter": (ctypes.c_int, [vp, ctypes.c_int, i64, vp, vp, vp, vp, vp, ctypes.c_int,
+                                                 ctypes.c_int, ctypes.c_int, vp, vp, vp]),
             # the shallow-water surrogate's kernels (csrc/surrogate.hip)
             "vxs_last_error": (cs, []),
             "vxs_conv3x3": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp,
@@ -161,6 +167,21 @@ class Sim:
 
     def advance(self, count: int, sort_every: int, first_step: int) -> None:
         check(lib().vxr_advance(self.h, int(count), int(sort_every), int(first_step)))
+
+    def swe_diagnostics(self, g, dry, dt, dx, dy) -> dict:
+        out = (ctypes.c_double * 9)()
+        check(lib().vxr_swe_diagnostics(self.h, g, dry, dt, dx, dy, out))
+        keys = ("volume", "min_depth", "max_depth", "max_speed", "max_froude", "max_courant", "non_finite",
+                "negative", "wet")
+        return dict(zip(keys, list(out)))
+
+    def particle_diagnostics(self, speed, sound, dt) -> dict:
+        out = (ctypes.c_double * 7)()
+        check(lib().vxr_particle_diagnostics(self.h, speed, sound, dt, out))
+        keys = ("min_density", "max_density", "low_density", "high_density", "mach", "courant", "non_finite")
+        d = dict(zip(keys, list(out)))
+        d["non_finite"] = int(d["non_finite"])
+        return d
 
     def autotune(self, reps: int = 3) -> dict:
         check(lib().vxr_autotune(self.h, int(reps)))

@@ -67,6 +67,11 @@ class _Runner:
     def close(self) -> None:
         self.sim.close()
 
+    def diagnostics(self, speed, sound, dt) -> dict:
+        """ParticleDiagnostics.of (core/particle/ParticleDiagnostics.java:45-65) of the particles' J,
+        reduced on the device (demo/FlatRun.java:189 and demo/BoxRun.java:365 read it back instead)."""
+        return self.sim.particle_diagnostics(speed, sound, dt)
+
     def _advance(self, count: int, sort_every: int | None) -> None:
         self.sim.advance(count, sort_every or 0, self._steps)
         self._steps += count

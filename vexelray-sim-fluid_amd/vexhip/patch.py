@@ -140,6 +140,11 @@ class PatchSimulation:
         f = self.fastest_wave()
         return courant * min(dx, dy) / f if f > 0 else 0.0
 
+    def diagnostics(self, g, dry, dt, dx, dy) -> dict:
+        """Diagnostics.of (core/stencil/Diagnostics.java:48-81) of the current state, reduced on the
+        device (no readback of the state)."""
+        return self.sim.swe_diagnostics(g, dry, dt, dx, dy)
+
     def clamped(self) -> int:
         return int(self.sim.read_words("clamped")[0])
 

@@ -23,6 +23,13 @@ _SHORT = {
     "g2p3_tiled_kernel": "g2p3_tiled", "g2p3_kernel": "g2p3", "grid3_kernel": "grid3", "clear4_kernel": "mpm3_clear",
     "keys3_kernel": "keys3", "hist_kernel": "radix_hist", "scan_rows_kernel": "radix_scan",
     "scatter_kernel": "radix_scatter", "permute_kernel": "permute", "hll_step_kernel": "hll_step",
+    "clear_rows_kernel": "mpm3_clear", "flags_kernel": "grid3_flags", "batch2_p2g": "batch2_p2g",
+    "batch2_grid": "batch2_grid", "batch2_g2p": "batch2_g2p", "batch2_clear": "batch2_clear",
+    "batch2_keys": "batch2_keys", "batch2_starts": "batch2_starts", "conv3x3_lds_kernel": "conv3x3",
+    "conv3x3_wgrad_kernel": "conv3x3_wgrad", "wgrad_reduce_kernel": "wgrad_reduce", "hll64_kernel": "hll64",
+    "adam_kernel": "adam", "loss_kernel": "loss", "unpack_kernel": "unpack", "pack_kernel": "pack",
+    "flip_kernel": "flip", "dam_bumps_kernel": "dam_bumps", "keys2_kernel": "keys2", "g2p2_kernel": "g2p2",
+    "starts_kernel": "starts2",
 }
 
 
@@ -31,7 +38,7 @@ def short_name(kernel: str) -> str:
     for key, name in _SHORT.items():
         if key in kernel:
             return name
-    return kernel
+    return kernel.split("(")[0][-40:]
 
 
 def _counters(directory):
