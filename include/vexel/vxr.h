@@ -52,6 +52,7 @@ extern "C" {
  * (keys, radix passes, permutation of every particle field). */
 #define VXR_GROUP_STEP 0
 #define VXR_GROUP_SORT 1
+#define VXR_GROUP_LOD 2 /* MPM3: lod_target then lod (gui/ParticleSimulation3.java:134-135,346-349) */
 
 typedef struct vxr_sim vxr_sim;
 
@@ -91,7 +92,7 @@ int vxr_destroy(vxr_sim* sim);
 /* Writes `n` 32-bit words into the named buffer, synchronously, from host memory the call
  * only borrows (PassRunner.write / ResidentBuffer.write). `n` must equal the buffer's
  * length; "params" also takes a shorter prefix. Names: MPM3 x y z u v w m j c00..c22 gm gmu
- * gmv gmw gu gv gw params keys; MPM2 x y u v m j c00 c01 c10 c11 gm gmu gmv gu gv params keys
+ * gmv gmw gu gv gw params keys level target stay view; MPM2 x y u v m j c00 c01 c10 c11 gm gmu gmv gu gv params keys
  * starts; SWE h hu hv params speed budget clamped (the current slot of each rotating set,
  * as gui/PatchSimulation.java:147-213 resolves it). */
 int vxr_write(vxr_sim* sim, const char* name, const void* words, int64_t n);
@@ -114,7 +115,11 @@ int vxr_run(vxr_sim* sim, int group, int repeat);
 int vxr_advance(vxr_sim* sim, int count, int sort_every, int64_t first_step);
 
 /* Queues one named pass: "clear", "scatter", "grid", "advect", "sort" (a sub-list of the
- * step, as test/particle/Flip3Test.java:102 runs step().subList(0, 2)). */
+ * step, as test/particle/Flip3Test.java:102 runs step().subList(0, 2)); MPM3 also "lod_target" and
+ * "lod" (core/particle/Flip3.java:420-641) over count / 8 groups of 8 consecutive slots, with the
+ * buffers "level", "target" (a word a group), "stay" (a word a slot) and "view" (eye x y z, near,
+ * floor). The groups are the slots as loaded: a sort regroups particles, so a simulation that
+ * thins and fills does not sort (the reference's 3D step has no sort). */
 int vxr_run_pass(vxr_sim* sim, const char* pass, int repeat);
 
 /* SWE only: queues `count` steps with the reference's buffer rotation -- state k%2 -> (k+1)%2,

@@ -72,6 +72,8 @@ def _load() -> ctypes.CDLL:
         "orc_mpm2_cell_starts": (_c_i64, [_c_i, _c_i, _c_i64, _f32p, _f32p, _i32p]),
         "orc_mpm2_sort": (None, [_c_i, _c_i, _c_i64, _vp]),
         "orc_stable_order": (None, [_c_i64, _i32p, _i32p]),
+        "orc_mpm3_lod": (None, [_c_i, _c_i, _c_i, _c_i64, _vp, _f32p, _f32p, _f32p]),
+        "orc_mpm3_lod_target": (None, [_c_i64, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]),
         "orc_particle_diagnostics": (None, [_c_i64, _f32p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                             _f64p]),
         "orc_fixed_split": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int32]),
@@ -298,6 +300,15 @@ class Mpm3:
         out = np.empty(self.count, np.int32)
         lib.orc_mpm3_keys(*self.n, self.count, self.p["x"], self.p["y"], self.p["z"], out)
         return out
+
+    def lod(self, level, target, stay):
+        """Flip3.lod over count // 8 groups; level, target and stay are float32 arrays, updated."""
+        lib.orc_mpm3_lod(*self.n, self.count // 8, self._pp(), level, target, stay)
+
+    def lod_target(self, level, target, view):
+        """Flip3.lodTarget; writes target."""
+        lib.orc_mpm3_lod_target(self.count // 8, self.p["x"], self.p["y"], self.p["z"], level, target,
+                                _c(np.asarray(view, np.float32)))
 
     def sort(self):
         order = stable_order(self.keys())

@@ -83,6 +83,14 @@ int orc_mpm3_key_bits(int nx, int ny, int nz);
 void orc_mpm3_keys(int nx, int ny, int nz, int64_t count, const float* x, const float* y,
                    const float* z, int32_t* keys);
 
+/* The 3D level of detail: Flip3.lod (core/particle/Flip3.java:420-586) over `groups` groups of 8
+ * consecutive slots, with level (per group), target (per group), stay (per slot); and
+ * Flip3.lodTarget (:600-641) with view = eye x, y, z, near, floor. */
+void orc_mpm3_lod(int nx, int ny, int nz, int64_t groups, float* const p[17], float* level, const float* target,
+                  float* stay);
+void orc_mpm3_lod_target(int64_t groups, const float* x, const float* y, const float* z, const float* level,
+                         float* target, const float view[5]);
+
 /* ---- 2D MLS-MPM / APIC, core/particle/Flip.java ----
  * Particle fields in FlipStep order: x y u v m j c00 c01 c10 c11 (10). Grid: gm gmu gmv, gu gv. */
 void orc_mpm2_clear(int nx, int ny, float* gm, float* gmu, float* gmv);

@@ -87,6 +87,9 @@ int mpm3_grid(const Launch& l, const Mpm3Args& a);
 int mpm3_advect(const Launch& l, const Mpm3Args& a);
 int mpm3_advect_tiled(const Launch& l, const Mpm3Args& a, int run, bool fast);
 int mpm3_keys(const Launch& l, const Mpm3Args& a, int32_t* keys, int32_t* index);
+// Flip3.lod / Flip3.lodTarget over count / 8 groups of 8 consecutive slots.
+int mpm3_lod(const Launch& l, const Mpm3Args& a, float* level, const float* target, float* stay);
+int mpm3_lod_target(const Launch& l, const Mpm3Args& a, const float* level, float* target, const float* view);
 int mpm3_key_bits(int nx, int ny, int nz);
 
 struct Mpm2Args {

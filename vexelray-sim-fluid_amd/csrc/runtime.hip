@@ -147,6 +147,7 @@ struct vxr_sim {
     int32_t* scratch = nullptr;
     int32_t* starts = nullptr;  // 2D: cells + 1
     int64_t nodes = 0;
+    float *level = nullptr, *target = nullptr, *stay = nullptr, *view = nullptr;  // 3D level of detail
     uint8_t* touched = nullptr;  // 3D: bricks P2G reached, and bricks holding velocities
     uint8_t* live = nullptr;
     int64_t bricks = 0;
@@ -238,6 +239,19 @@ bool lookup(vxr_sim* s, const std::string& name, Buf* b) {
         }
         if (name == "keys") {
             *b = Buf{s->keys, s->count, s->capacity};
+            return true;
+        }
+        if (three && (name == "level" || name == "target")) {
+            const int64_t groups = s->count / 8;
+            *b = Buf{name == "level" ? s->level : s->target, groups, groups};
+            return true;
+        }
+        if (three && name == "stay") {
+            *b = Buf{s->stay, s->count, s->capacity};
+            return true;
+        }
+        if (three && name == "view") {
+            *b = Buf{s->view, 5, 8};
             return true;
         }
         if (!three && name == "starts") {
@@ -367,6 +381,8 @@ int run_pass(vxr_sim* s, const std::string& pass) {
         if (pass == "scatter")
             return opt(s, "p2g") == 0 ? vxr::mpm3_scatter_atomic(l, a) : vxr::mpm3_scatter_tiled(l, a, (int)opt(s, "run"));
         if (pass == "grid") return vxr::mpm3_grid(l, a);
+        if (pass == "lod") return vxr::mpm3_lod(l, a, s->level, s->target, s->stay);
+        if (pass == "lod_target") return vxr::mpm3_lod_target(l, a, s->level, s->target, s->view);
         if (pass == "advect") {
             const int64_t g2p = opt(s, "g2p");
             return g2p == 0 ? vxr::mpm3_advect(l, a) : vxr::mpm3_advect_tiled(l, a, (int)opt(s, "run"), g2p == 2);
@@ -578,6 +594,13 @@ int clear_all(vxr_sim* s) {
         for (float* f : s->g) HIP_TRY(hipMemsetAsync(f, 0, round_up(s->nodes, 4) * 4, s->stream));
         HIP_TRY(hipMemsetAsync(s->params, 0, s->param_words * 4, s->stream));
         if (int rc = mark_all_bricks(s)) return rc;
+        if (s->stay) {
+            const int64_t groups = std::max<int64_t>(s->count / 8, 1);
+            HIP_TRY(hipMemsetAsync(s->level, 0, groups * 4, s->stream));
+            HIP_TRY(hipMemsetAsync(s->target, 0, groups * 4, s->stream));
+            HIP_TRY(hipMemsetAsync(s->stay, 0, s->capacity * 4, s->stream));
+            HIP_TRY(hipMemsetAsync(s->view, 0, 8 * 4, s->stream));
+        }
         HIP_TRY(hipMemsetAsync(s->keys, 0, s->capacity * 4, s->stream));
         HIP_TRY(hipMemsetAsync(s->index, 0, s->capacity * 4, s->stream));
         if (s->starts) HIP_TRY(hipMemsetAsync(s->starts, 0, s->starts_len * 4, s->stream));
@@ -774,6 +797,13 @@ int vxr_create(int kind, const int32_t dims[3], int64_t particles, uint32_t flag
         if (!rc) rc = alloc(s, s->capacity, &ptr), s->index = (int32_t*)ptr;
         if (!rc) rc = alloc(s, vxr::radix_scratch_words(s->capacity, 8), &ptr), s->scratch = (int32_t*)ptr;
         if (!rc && three) {
+            const int64_t groups = std::max<int64_t>(s->count / 8, 1);
+            rc = alloc(s, groups, &ptr), s->level = (float*)ptr;
+            if (!rc) rc = alloc(s, groups, &ptr), s->target = (float*)ptr;
+            if (!rc) rc = alloc(s, s->capacity, &ptr), s->stay = (float*)ptr;
+            if (!rc) rc = alloc(s, 8, &ptr), s->view = (float*)ptr;
+        }
+        if (!rc && three) {
             s->bricks = vxr::brick_count3(s->nx, s->ny, s->nz);
             rc = alloc(s, (s->bricks + 3) / 4, &ptr), s->touched = (uint8_t*)ptr;
             if (!rc) rc = alloc(s, (s->bricks + 3) / 4, &ptr), s->live = (uint8_t*)ptr;
@@ -944,6 +974,10 @@ int vxr_run(vxr_sim* s, int group, int repeat) {
                 if (int rc = run_pass(s, p)) return rc;
         } else if (group == VXR_GROUP_SORT) {
             if (int rc = sort_pass(s)) return rc;
+        } else if (group == VXR_GROUP_LOD) {
+            if (s->kind != VXR_KIND_MPM3) return fail(VXR_ESTATE, "the level of detail is the 3D step's");
+            if (int rc = run_pass(s, "lod_target")) return rc;
+            if (int rc = run_pass(s, "lod")) return rc;
         } else {
             return fail(VXR_EINVAL, "unknown group %d", group);
         }

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libvexelhip.so")
 
 OK, EINVAL, ESTATE, EDEVICE, ENOMEM, EUNSUPPORTED = range(6)
 KIND_MPM2, KIND_MPM3, KIND_SWE, KIND_MPM2_BATCH = 1, 2, 3, 4
-GROUP_STEP, GROUP_SORT = 0, 1
+GROUP_STEP, GROUP_SORT, GROUP_LOD = 0, 1, 2
 FLAG_TENSION, FLAG_HEAT, FLAG_CONVECTION, FLAG_RELAX, FLAG_PUMP, FLAG_PLASTIC = (1 << k for k in range(6))
 FLAG_OPEN_WEST, FLAG_OPEN_EAST, FLAG_OPEN_SOUTH, FLAG_OPEN_NORTH = (1 << k for k in range(16, 20))
 

@@ -167,6 +167,13 @@ class ParticleSimulation3(_Runner):
             self._write(name, v)
         m = np.asarray(mass, np.float32)
         self._write("m", m)
+        # the level of detail starts full: every slot staying, every group at 8 (:266-273)
+        groups = n // 8
+        if groups:
+            self._write("level", np.full(groups, 8, np.float32))
+            self._write("target", np.full(groups, 8, np.float32))
+        self._write("stay", np.ones(n, np.float32))
+        self._write("view", np.array([0, 0, 0, 1e9, 1], np.float32))
         live = m[m > 0]
         self._nominal = live.min() if live.size else np.float32(np.finfo(np.float32).max)
         self._write("j", np.ones(n, np.float32))
@@ -182,6 +189,16 @@ class ParticleSimulation3(_Runner):
 
     def sort(self) -> None:
         self.sim.run(_native.GROUP_SORT)
+
+    def refine(self, eye, near, floor) -> None:
+        """Moves every group of 8 slots one step toward the level its distance from `eye` asks for
+        (:346-349): lodTarget, then lod. Groups are the slots as loaded, so do not sort as well."""
+        self._write("view", np.array([eye[0], eye[1], eye[2], near, floor], np.float32))
+        self.sim.run(_native.GROUP_LOD)
+
+    def active(self) -> int:
+        """Slots with mass (:352-360), a readback."""
+        return int(np.count_nonzero(self._read("m") > 0))
 
     def grid(self):
         return [self._read(g) for g in ("gm", "gmu", "gmv", "gmw")]
