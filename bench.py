@@ -31,6 +31,10 @@ BYTES = {
     "keys3": ("particle", 20), "radix_hist": ("element", 4), "radix_scatter": ("element", 16),
     "radix_scan": ("none", 0), "permute": ("particle", 140),
     "hll_step": ("cell", 24),
+    # 3D tension, dense over the grid: a blur reads one colour (or gm) and writes one; the stress
+    # reads the colour and writes six components; the grid with tension reads four sums, the
+    # colour and six components and writes three velocities
+    "tension3_blur": ("node", 8), "tension3_stress": ("node", 28), "grid3_tension": ("node", 56),
     # batched 2D: 10 particle fields in for P2G; x y j in and 9 fields out for G2P; three 64-bit
     # sums in and five floats out per node for the grid update; three 64-bit sums cleared per node
     "batch2_p2g": ("particle", 40), "batch2_g2p": ("particle", 48), "batch2_grid": ("node", 44),
@@ -53,14 +57,17 @@ def kernel_table(sim, names, units, bytes_override=None):
     return out
 
 
-def bench_mpm3(steps, warmup, p2g, g2p, run):
+def bench_mpm3(steps, warmup, p2g, g2p, run, tension=False):
+    """The 3D substep on the two-cubes scene; with ``tension`` Flip3Step's tension step (six
+    blurs, the stress and the grid with the capillary force), sigma = 0.01 B 8, the scale
+    Tension3Test gives a drop of radius 8 nodes."""
     from vexhip import particles, scenes
 
     s = scenes.mpm3_two_cubes()
     count = len(s["x"])
     g, bulk, rho0 = scenes.mpm3_units(256)
-    words = particles.params3(5e-5, 0, g, 0, bulk, rho0)
-    sim = particles.ParticleSimulation3(256, 256, 256, count)
+    words = particles.params3(5e-5, 0, g, 0, bulk, rho0, 0.01 * bulk * 8 if tension else 0.0)
+    sim = particles.ParticleSimulation3(256, 256, 256, count, tension=tension)
     sim.sim.set_option("p2g", p2g)
     sim.sim.set_option("g2p", g2p)
     sim.sim.set_option("run", run)
@@ -74,7 +81,8 @@ def bench_mpm3(steps, warmup, p2g, g2p, run):
     sim.finish()
     t1 = time.perf_counter()
     names = ["mpm3_clear", "p2g3_seg", "p2g3_atomic", "grid3", "g2p3_fast", "g2p3_tiled", "g2p3", "keys3",
-             "radix_hist", "radix_scan", "radix_scatter", "permute"]
+             "radix_hist", "radix_scan", "radix_scatter", "permute", "tension3_blur", "tension3_stress",
+             "grid3_tension"]
     units = {"particle": count, "node": 256 ** 3, "element": count}
     table = kernel_table(sim.sim, names, units)
     sim.sim.timing(False)
@@ -299,6 +307,8 @@ def main():
     ap.add_argument("--surrogate-iters", type=int, default=2000)
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--tension-steps", type=int, default=50)
+    ap.add_argument("--no-tension", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (counter calibration) and exit")
     args = ap.parse_args()
@@ -339,6 +349,14 @@ def main():
                    "run": args.run},
         "roofline": roofline, "kernels": table, "grid_mass_drift": drift,
     }
+    if not args.no_tension:
+        print(f"[bench] 3D MPM with tension: same scene, {args.tension_steps} steps", file=sys.stderr, flush=True)
+        tcount, tsecs, ttable, tdrift = bench_mpm3(args.tension_steps, 5, args.p2g, args.g2p, args.run, tension=True)
+        line["tension3"] = {"value": round(tcount * args.tension_steps / tsecs, 1), "unit": "particle-substeps/s",
+                            "steps": args.tension_steps, "ms_per_step": round(tsecs / args.tension_steps * 1e3, 4),
+                            "workload": "the same 3D scene stepped as Flip3Step's tension step (clear, P2G, six "
+                                        "blurs, stress, grid with tension, G2P), sigma = 0.01*B*8, sort every 10",
+                            "kernels": ttable, "grid_mass_drift": tdrift}
     if not args.no_hll:
         print(f"[bench] HLL 4096^2, {args.hll_steps} steps", file=sys.stderr, flush=True)
         rate, htable = bench_hll(args.hll_steps)

@@ -34,8 +34,11 @@ extern "C" {
 #define VXR_KIND_SWE 3  /* gui/PatchSimulation.java over core/stencil/ShallowWater.java */
 #define VXR_KIND_MPM2_BATCH 4 /* many ParticleSimulation scenes in one set of buffers (vxr_create_batch) */
 
-/* Creation flags. The reference's optional physics (FlipStep's tension, heat, convection,
- * relax, pump; Flip3Step's tension, plastic) are outside this path: VXR_EUNSUPPORTED. */
+/* Creation flags. Of the reference's optional physics, 3D surface tension is built: MPM3 with
+ * VXR_FLAG_TENSION steps as Flip3Step(..., tension) does (core/particle/Flip3Step.java:88-109:
+ * clear, scatter, blur0..blur5, stress, grid with the capillary force, advect; sigma is params[6];
+ * buffers c0 c1 txx tyy tzz txy txz tyz). FlipStep's tension, heat, convection, relax, pump and
+ * Flip3Step's plastic are VXR_EUNSUPPORTED. */
 #define VXR_FLAG_TENSION (1u << 0)
 #define VXR_FLAG_HEAT (1u << 1)
 #define VXR_FLAG_CONVECTION (1u << 2)

@@ -72,6 +72,11 @@ def _load() -> ctypes.CDLL:
         "orc_mpm2_cell_starts": (_c_i64, [_c_i, _c_i, _c_i64, _f32p, _f32p, _i32p]),
         "orc_mpm2_sort": (None, [_c_i, _c_i, _c_i64, _vp]),
         "orc_stable_order": (None, [_c_i64, _i32p, _i32p]),
+        "orc_tension3_blur": (None, [_c_i, _c_i, _c_i, _f32p, _f32p, _i32p, _c_i]),
+        "orc_tension3_stress": (None, [_c_i, _c_i, _c_i, _f32p, _vp]),
+        "orc_tension3_force": (None, [_c_i, _c_i, _c_i, _f32p, _vp, _c_i64, _f32p]),
+        "orc_mpm3_grid_tension": (None, [_c_i, _c_i, _c_i, _i32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                         _vp, _f32p]),
         "orc_mpm3_lod": (None, [_c_i, _c_i, _c_i, _c_i64, _vp, _f32p, _f32p, _f32p]),
         "orc_mpm3_lod_target": (None, [_c_i64, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]),
         "orc_particle_diagnostics": (None, [_c_i64, _f32p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -260,10 +265,17 @@ def _ptrs(arrays):
     return (ctypes.c_void_p * len(arrays))(*[a.ctypes.data for a in arrays])
 
 
-class Mpm3:
-    """A 3D particle state on the CPU, stepped pass by pass as Flip3Step lists them."""
+TENSION_BLURS = 6                      # Flip.TENSION_BLURS
+STRESS3 = ("txx", "tyy", "tzz", "txy", "txz", "tyz")
 
-    def __init__(self, nx, ny, nz, fields: dict, params):
+
+class Mpm3:
+    """A 3D particle state on the CPU, stepped pass by pass as Flip3Step lists them. With
+    ``tension`` the step is Flip3Step's tension one (core/particle/Flip3Step.java:88-109): clear,
+    scatter, six blurs (gm -> c0 -> c1 -> ...), stress, grid with tension, advect."""
+
+    def __init__(self, nx, ny, nz, fields: dict, params, tension=False):
+        self.tension = tension
         self.n = (nx, ny, nz)
         count = len(fields["x"])
         self.p = {f: _c(fields.get(f, np.zeros(count)), np.float32).copy() for f in FIELDS3}
@@ -272,10 +284,30 @@ class Mpm3:
         self.count = count
         nodes = nx * ny * nz
         self.g = {f: np.zeros(nodes, np.float32) for f in GRID3}
+        if tension:
+            for f in ("c0", "c1") + STRESS3:
+                self.g[f] = np.zeros(nodes, np.float32)
         self.params = np.asarray(params, np.int32)
 
     def _pp(self):
         return _ptrs([self.p[f] for f in FIELDS3])
+
+    def blur(self, k):
+        """Pass blur k: gm (scaled to a colour) into c0 for k = 0, then c0 and c1 in turn."""
+        src = "gm" if k == 0 else ("c0" if k % 2 == 1 else "c1")
+        dst = "c0" if k % 2 == 0 else "c1"
+        lib.orc_tension3_blur(*self.n, self.g[src], self.g[dst], self.params, 1 if k == 0 else 0)
+
+    def colour(self):
+        return "c0" if (TENSION_BLURS - 1) % 2 == 0 else "c1"
+
+    def stress(self):
+        lib.orc_tension3_stress(*self.n, self.g[self.colour()], _ptrs([self.g[f] for f in STRESS3]))
+
+    def force(self, node):
+        out = np.zeros(3, np.float32)
+        lib.orc_tension3_force(*self.n, self.g[self.colour()], _ptrs([self.g[f] for f in STRESS3]), int(node), out)
+        return out
 
     def clear(self):
         lib.orc_mpm3_clear(*self.n, self.g["gm"], self.g["gmu"], self.g["gmv"], self.g["gmw"])
@@ -285,6 +317,10 @@ class Mpm3:
                              self.g["gm"], self.g["gmu"], self.g["gmv"], self.g["gmw"])
 
     def grid(self):
+        if self.tension:
+            lib.orc_mpm3_grid_tension(*self.n, self.params, *[self.g[f] for f in GRID3],
+                                      _ptrs([self.g[f] for f in STRESS3]), self.g[self.colour()])
+            return
         lib.orc_mpm3_grid(*self.n, self.params, *[self.g[f] for f in GRID3])
 
     def advect(self):
@@ -292,6 +328,16 @@ class Mpm3:
                             self.g["gu"], self.g["gv"], self.g["gw"])
 
     def step(self, n=1):
+        if self.tension:
+            for _ in range(n):
+                self.clear()
+                self.scatter()
+                for k in range(TENSION_BLURS):
+                    self.blur(k)
+                self.stress()
+                self.grid()
+                self.advect()
+            return
         grid = _ptrs([self.g[f] for f in GRID3])
         for _ in range(n):
             lib.orc_mpm3_step(*self.n, self.count, self._pp(), self.params, grid)

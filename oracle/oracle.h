@@ -83,6 +83,17 @@ int orc_mpm3_key_bits(int nx, int ny, int nz);
 void orc_mpm3_keys(int nx, int ny, int nz, int64_t count, const float* x, const float* y,
                    const float* z, int32_t* keys);
 
+/* 3D surface tension: Tension3.blur (core/particle/Tension3.java:69-107; scaled: the mass to a
+ * colour), Tension3.stress (:122-147; t = txx tyy tzz txy txz tyz), Tension3.force (:156-196)
+ * at one node, and Flip3.gridWithTension (core/particle/Flip3.java:263-311). */
+void orc_tension3_blur(int nx, int ny, int nz, const float* src, float* dst, const int32_t params[7], int scaled);
+void orc_tension3_stress(int nx, int ny, int nz, const float* colour, float* const t[6]);
+void orc_tension3_force(int nx, int ny, int nz, const float* colour, const float* const t[6], int64_t node,
+                        float out[3]);
+void orc_mpm3_grid_tension(int nx, int ny, int nz, const int32_t params[7], const float* gm, const float* gmu,
+                           const float* gmv, const float* gmw, float* gu, float* gv, float* gw,
+                           const float* const t[6], const float* colour);
+
 /* The 3D level of detail: Flip3.lod (core/particle/Flip3.java:420-586) over `groups` groups of 8
  * consecutive slots, with level (per group), target (per group), stay (per slot); and
  * Flip3.lodTarget (:600-641) with view = eye x, y, z, near, floor. */

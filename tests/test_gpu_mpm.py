@@ -307,7 +307,7 @@ def test_runner_api_3d(gpu):
     pos = sim.positions()
     assert all(np.all((p >= 1) & (p <= 14)) for p in pos)
     with pytest.raises(_native.Unsupported):
-        particles.ParticleSimulation3(16, 16, 16, 10, tension=True)
+        particles.ParticleSimulation3(16, 16, 16, 10, plastic=True)
     with pytest.raises(_native.InvalidArgument):
         sim.sim.write("x", np.zeros(count + 1, np.float32))
     sim.close()

@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "vexhip", "libvexelhip.so")
 BUILD = os.path.join(HERE, "_build")
-SOURCES = ["runtime.hip", "hll.hip", "mpm3.hip", "mpm2.hip", "sort.hip", "surrogate.hip", "fixed_scatter.hip", "diagnostics.hip", "lod.hip"]
+SOURCES = ["runtime.hip", "hll.hip", "mpm3.hip", "mpm2.hip", "sort.hip", "surrogate.hip", "fixed_scatter.hip", "diagnostics.hip", "lod.hip", "tension.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

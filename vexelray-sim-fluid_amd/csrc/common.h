@@ -87,6 +87,13 @@ int mpm3_grid(const Launch& l, const Mpm3Args& a);
 int mpm3_advect(const Launch& l, const Mpm3Args& a);
 int mpm3_advect_tiled(const Launch& l, const Mpm3Args& a, int run, bool fast);
 int mpm3_keys(const Launch& l, const Mpm3Args& a, int32_t* keys, int32_t* index);
+// After a grid pass that did not go through mpm3_grid: touched bricks become live, touched clears.
+int mpm3_flags(const Launch& l, const Mpm3Args& a);
+// 3D surface tension (core/particle/Tension3.java): blur (scaled: gm to a colour), stress
+// (t = txx tyy tzz txy txz tyz), and the grid update with the capillary force.
+int tension3_blur(const Launch& l, const Mpm3Args& a, const float* src, float* dst, bool scaled);
+int tension3_stress(const Launch& l, const Mpm3Args& a, const float* colour, float* const t[6]);
+int mpm3_grid_tension(const Launch& l, const Mpm3Args& a, float* const t[6], const float* colour);
 // Flip3.lod / Flip3.lodTarget over count / 8 groups of 8 consecutive slots.
 int mpm3_lod(const Launch& l, const Mpm3Args& a, float* level, const float* target, float* stay);
 int mpm3_lod_target(const Launch& l, const Mpm3Args& a, const float* level, float* target, const float* view);

@@ -893,6 +893,12 @@ int mpm3_scatter_tiled(const Launch& l, const Mpm3Args& a, int run) {
     return check_launch("p2g3_seg");
 }
 
+int mpm3_flags(const Launch& l, const Mpm3Args& a) {
+    const int64_t bricks = brick_count3(a.nx, a.ny, a.nz);
+    hipLaunchKernelGGL(flags_kernel, dim3(blocks_for(bricks, 256)), dim3(256), 0, l.stream, a.touched, a.live, bricks);
+    return check_launch("grid3_flags");
+}
+
 int mpm3_grid(const Launch& l, const Mpm3Args& a) {
     timed_begin(l, "grid3");
     hipLaunchKernelGGL(grid3_kernel, dim3(blocks_for((int64_t)a.ny * a.nz, 4)), dim3(256), 0, l.stream, a);

@@ -148,6 +148,9 @@ struct vxr_sim {
     int32_t* starts = nullptr;  // 2D: cells + 1
     int64_t nodes = 0;
     float *level = nullptr, *target = nullptr, *stay = nullptr, *view = nullptr;  // 3D level of detail
+    bool tension = false;                       // 3D surface tension (VXR_FLAG_TENSION)
+    float* colour[2] = {};                      // c0, c1: the blurred colour, in turn
+    float* stress[6] = {};                      // txx tyy tzz txy txz tyz
     uint8_t* touched = nullptr;  // 3D: bricks P2G reached, and bricks holding velocities
     uint8_t* live = nullptr;
     int64_t bricks = 0;
@@ -249,6 +252,15 @@ bool lookup(vxr_sim* s, const std::string& name, Buf* b) {
         if (three && name == "stay") {
             *b = Buf{s->stay, s->count, s->capacity};
             return true;
+        }
+        if (three && s->tension) {
+            const char* tn[8] = {"c0", "c1", "txx", "tyy", "tzz", "txy", "txz", "tyz"};
+            for (int k = 0; k < 8; ++k) {
+                if (name == tn[k]) {
+                    *b = Buf{k < 2 ? s->colour[k] : s->stress[k - 2], s->nodes, s->nodes};
+                    return true;
+                }
+            }
         }
         if (three && name == "view") {
             *b = Buf{s->view, 5, 8};
@@ -380,7 +392,18 @@ int run_pass(vxr_sim* s, const std::string& pass) {
         if (pass == "clear") return vxr::mpm3_clear(l, a);
         if (pass == "scatter")
             return opt(s, "p2g") == 0 ? vxr::mpm3_scatter_atomic(l, a) : vxr::mpm3_scatter_tiled(l, a, (int)opt(s, "run"));
-        if (pass == "grid") return vxr::mpm3_grid(l, a);
+        if (pass == "grid") {
+            if (!s->tension) return vxr::mpm3_grid(l, a);
+            // the colour the last blur wrote: blur k writes c[k % 2], and there are six
+            if (int rc = vxr::mpm3_grid_tension(l, a, s->stress, s->colour[1])) return rc;
+            return vxr::mpm3_flags(l, a);
+        }
+        if (s->tension && pass.size() == 5 && pass.compare(0, 4, "blur") == 0 && pass[4] >= '0' && pass[4] <= '5') {
+            const int k = pass[4] - '0';
+            const float* src = k == 0 ? a.g[0] : s->colour[(k - 1) % 2];
+            return vxr::tension3_blur(l, a, src, s->colour[k % 2], k == 0);
+        }
+        if (s->tension && pass == "stress") return vxr::tension3_stress(l, a, s->colour[1], s->stress);
         if (pass == "lod") return vxr::mpm3_lod(l, a, s->level, s->target, s->stay);
         if (pass == "lod_target") return vxr::mpm3_lod_target(l, a, s->level, s->target, s->view);
         if (pass == "advect") {
@@ -409,6 +432,20 @@ int run_pass(vxr_sim* s, const std::string& pass) {
 // ------------------------------------------------------------------ advance, graphs
 
 const char* const kStep[4] = {"clear", "scatter", "grid", "advect"};
+// Flip3Step's step with tension (core/particle/Flip3Step.java:88-109)
+const char* const kTensionStep[10] = {"clear", "scatter", "blur0", "blur1", "blur2", "blur3", "blur4", "blur5",
+                                      "stress", "grid"};
+
+int run_step(vxr_sim* s) {
+    if (s->tension) {
+        for (const char* p : kTensionStep)
+            if (int rc = run_pass(s, p)) return rc;
+        return run_pass(s, "advect");
+    }
+    for (const char* p : kStep)
+        if (int rc = run_pass(s, p)) return rc;
+    return 0;
+}
 constexpr int kMaxSegment = 64;  // steps one captured graph holds at most
 
 // A segment: a sort when `sort`, then `n` steps.
@@ -416,8 +453,7 @@ int run_segment(vxr_sim* s, bool sort, int n) {
     if (sort)
         if (int rc = sort_pass(s)) return rc;
     for (int i = 0; i < n; ++i)
-        for (const char* p : kStep)
-            if (int rc = run_pass(s, p)) return rc;
+        if (int rc = run_step(s)) return rc;
     return 0;
 }
 
@@ -594,6 +630,10 @@ int clear_all(vxr_sim* s) {
         for (float* f : s->g) HIP_TRY(hipMemsetAsync(f, 0, round_up(s->nodes, 4) * 4, s->stream));
         HIP_TRY(hipMemsetAsync(s->params, 0, s->param_words * 4, s->stream));
         if (int rc = mark_all_bricks(s)) return rc;
+        for (float* f : s->colour)
+            if (f) HIP_TRY(hipMemsetAsync(f, 0, round_up(s->nodes, 4) * 4, s->stream));
+        for (float* f : s->stress)
+            if (f) HIP_TRY(hipMemsetAsync(f, 0, round_up(s->nodes, 4) * 4, s->stream));
         if (s->stay) {
             const int64_t groups = std::max<int64_t>(s->count / 8, 1);
             HIP_TRY(hipMemsetAsync(s->level, 0, groups * 4, s->stream));
@@ -706,8 +746,10 @@ int vxr_create(int kind, const int32_t dims[3], int64_t particles, uint32_t flag
     *out = nullptr;
     const uint32_t physics = VXR_FLAG_TENSION | VXR_FLAG_HEAT | VXR_FLAG_CONVECTION | VXR_FLAG_RELAX | VXR_FLAG_PUMP |
                              VXR_FLAG_PLASTIC;
-    if (flags & physics)
-        return fail(VXR_EUNSUPPORTED, "tension, heat, convection, relax, pump and plastic are not built");
+    const uint32_t built = kind == VXR_KIND_MPM3 ? VXR_FLAG_TENSION : 0u;
+    if (flags & physics & ~built)
+        return fail(VXR_EUNSUPPORTED, "of the optional physics only 3D tension is built (not heat, convection, relax, "
+                                      "pump, plastic, or 2D tension)");
     const uint32_t edges = VXR_FLAG_OPEN_WEST | VXR_FLAG_OPEN_EAST | VXR_FLAG_OPEN_SOUTH | VXR_FLAG_OPEN_NORTH;
     if (flags & ~(physics | edges)) return fail(VXR_EINVAL, "unknown flags 0x%x", flags & ~(physics | edges));
 
@@ -796,6 +838,11 @@ int vxr_create(int kind, const int32_t dims[3], int64_t particles, uint32_t flag
         if (!rc) rc = alloc(s, s->capacity, &ptr), s->keys = (int32_t*)ptr;
         if (!rc) rc = alloc(s, s->capacity, &ptr), s->index = (int32_t*)ptr;
         if (!rc) rc = alloc(s, vxr::radix_scratch_words(s->capacity, 8), &ptr), s->scratch = (int32_t*)ptr;
+        if (!rc && three && (flags & VXR_FLAG_TENSION)) {
+            s->tension = true;
+            for (int k = 0; k < 2 && !rc; ++k) rc = alloc(s, round_up(s->nodes, 4), &ptr), s->colour[k] = (float*)ptr;
+            for (int k = 0; k < 6 && !rc; ++k) rc = alloc(s, round_up(s->nodes, 4), &ptr), s->stress[k] = (float*)ptr;
+        }
         if (!rc && three) {
             const int64_t groups = std::max<int64_t>(s->count / 8, 1);
             rc = alloc(s, groups, &ptr), s->level = (float*)ptr;
@@ -967,11 +1014,9 @@ int vxr_run(vxr_sim* s, int group, int repeat) {
     if (!s) return fail(VXR_EINVAL, "null simulation");
     if (s->kind == VXR_KIND_SWE) return fail(VXR_ESTATE, "a shallow-water patch steps by vxr_swe_dispatch");
     if (repeat < 0) return fail(VXR_EINVAL, "negative repeat");
-    static const char* step[4] = {"clear", "scatter", "grid", "advect"};
     for (int r = 0; r < repeat; ++r) {
         if (group == VXR_GROUP_STEP) {
-            for (const char* p : step)
-                if (int rc = run_pass(s, p)) return rc;
+            if (int rc = run_step(s)) return rc;
         } else if (group == VXR_GROUP_SORT) {
             if (int rc = sort_pass(s)) return rc;
         } else if (group == VXR_GROUP_LOD) {

@@ -29,7 +29,8 @@ _SHORT = {
     "conv3x3_wgrad_kernel": "conv3x3_wgrad", "wgrad_reduce_kernel": "wgrad_reduce", "hll64_kernel": "hll64",
     "adam_kernel": "adam", "loss_kernel": "loss", "unpack_kernel": "unpack", "pack_kernel": "pack",
     "flip_kernel": "flip", "dam_bumps_kernel": "dam_bumps", "keys2_kernel": "keys2", "g2p2_kernel": "g2p2",
-    "starts_kernel": "starts2",
+    "starts_kernel": "starts2", "grid_tension_kernel": "grid3_tension", "blur_kernel": "tension3_blur",
+    "stress_kernel": "tension3_stress",
 }
 
 
