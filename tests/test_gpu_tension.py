@@ -68,6 +68,74 @@ def test_tension_passes_bit_exact(gpu, shape):
         np.testing.assert_array_equal(rig.read(name).view(np.int32), ref.p[name].view(np.int32), err_msg=name)
 
 
+def _tension_passes_match(rig, n, params, passes):
+    """Hands the device's grid sums to a fresh oracle, runs `passes` on both, and compares every
+    tension buffer and velocity over the whole grid bit for bit (so a node the device skipped
+    must hold what the oracle's full pass wrote there)."""
+    ref = oracle.Mpm3(*n, {"x": np.zeros(1, np.float32), "y": np.zeros(1, np.float32),
+                           "z": np.zeros(1, np.float32), "m": np.zeros(1, np.float32)}, params, tension=True)
+    for g in ("gm", "gmu", "gmv", "gmw"):
+        ref.g[g] = rig.read(g)
+    for p in passes:
+        rig.run([p])
+        if p.startswith("blur"):
+            ref.blur(int(p[4]))
+        else:
+            getattr(ref, p)()
+    names = ["c0", "c1"] + list(oracle.STRESS3) + (["gu", "gv", "gw"] if "grid" in passes else [])
+    for name in names:
+        got, exp = rig.read(name).view(np.int32), ref.g[name].view(np.int32)
+        bad = np.flatnonzero(got != exp)
+        assert bad.size == 0, f"{name}: {bad.size} nodes differ, first {bad[:5]}"
+    return ref
+
+
+def test_sparse_region_equals_the_full_passes(gpu):
+    """A drop thrown across the box, stepped with sorts: every few steps the six blurs, the stress
+    and the grid run on the device's own scatter, and every node of every tension buffer and
+    velocity equals the oracle's full-grid passes, bit for bit -- including the bricks the region
+    skipped and the ones the drop left behind."""
+    n = (48, 40, 44)
+    f = _blob(n, (6, 14, 8), (18, 26, 20), 8, 77)
+    f["u"] = np.full_like(f["u"], 120.0)
+    f["v"] = np.full_like(f["v"], 40.0)
+    f["w"] = np.full_like(f["w"], 90.0)
+    f["j"] = np.ones_like(f["j"])
+    params = oracle.mpm3_params(2e-3, 0, -60.0, 0, 400.0, 1.0, 30.0)
+    rig = maker3(tension=True)(*n, len(f["x"]))
+    _load(rig, f, params)
+    for k in range(60):
+        if k % 10 == 0:
+            rig.run(["sort"])
+        if k % 12 == 5:
+            rig.run(["clear", "scatter"])
+            _tension_passes_match(rig, n, params, PASSES)
+            rig.run(["advect"])
+        else:
+            rig.run("step")
+    x = rig.read("x")
+    assert x.max() - f["x"].max() > 8  # it crossed bricks
+
+
+def test_passes_out_of_order_fall_back_to_dense(gpu):
+    """Blurs run past the reach of the region (the colour spreading further than a brick), a
+    stress of that colour, then ordinary steps again: each pass equals the oracle's over the whole
+    grid, the ones past the reach run dense and the region is rebuilt after."""
+    n = (40, 36, 40)
+    f = _blob(n, (12, 10, 14), (22, 20, 24), 8, 5)
+    params = oracle.mpm3_params(1e-3, 0, -30.0, 0, 300.0, 1.0, 20.0)
+    rig = maker3(tension=True)(*n, len(f["x"]))
+    _load(rig, f, params)
+    rig.run("step")
+    rig.run(["clear", "scatter"])
+    _tension_passes_match(rig, n, params, ["blur0"] + ["blur1", "blur2"] * 6 + ["blur1", "stress", "grid"])
+    rig.run(["advect"])
+    for _ in range(3):
+        rig.run(["clear", "scatter"])
+        _tension_passes_match(rig, n, params, PASSES)
+        rig.run(["advect"])
+
+
 def test_tension_steps_track_the_oracle(gpu):
     """Ten whole tension steps of a falling drop: positions within 1e-5 of the oracle's (P2G sums
     in another order), grid mass equal to particle mass."""

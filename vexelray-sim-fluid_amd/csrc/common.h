@@ -74,6 +74,10 @@ struct Mpm3Args {
     // Clear and grid visit, in each row of nodes, only the span of bricks flagged in either.
     uint8_t* touched;
     uint8_t* live;
+    // With tension: per brick, bit 0 = within one brick of a touched or live brick at the last
+    // region computation, bit 1 = bit 0 then or at the computation before (the blur and stress
+    // passes visit the span of bit 1).
+    uint8_t* near;
 };
 constexpr int kBrick = 8;
 inline int64_t brick_count3(int nx, int ny, int nz) {
@@ -91,8 +95,9 @@ int mpm3_keys(const Launch& l, const Mpm3Args& a, int32_t* keys, int32_t* index)
 int mpm3_flags(const Launch& l, const Mpm3Args& a);
 // 3D surface tension (core/particle/Tension3.java): blur (scaled: gm to a colour), stress
 // (t = txx tyy tzz txy txz tyz), and the grid update with the capillary force.
-int tension3_blur(const Launch& l, const Mpm3Args& a, const float* src, float* dst, bool scaled);
-int tension3_stress(const Launch& l, const Mpm3Args& a, const float* colour, float* const t[6]);
+int tension3_region(const Launch& l, const Mpm3Args& a, bool all);
+int tension3_blur(const Launch& l, const Mpm3Args& a, const float* src, float* dst, bool scaled, bool dense);
+int tension3_stress(const Launch& l, const Mpm3Args& a, const float* colour, float* const t[6], bool dense);
 int mpm3_grid_tension(const Launch& l, const Mpm3Args& a, float* const t[6], const float* colour);
 // Flip3.lod / Flip3.lodTarget over count / 8 groups of 8 consecutive slots.
 int mpm3_lod(const Launch& l, const Mpm3Args& a, float* level, const float* target, float* stay);

@@ -151,6 +151,11 @@ struct vxr_sim {
     bool tension = false;                       // 3D surface tension (VXR_FLAG_TENSION)
     float* colour[2] = {};                      // c0, c1: the blurred colour, in turn
     float* stress[6] = {};                      // txx tyy tzz txy txz tyz
+    uint8_t* near = nullptr;                    // the blur/stress region flags, one byte a brick
+    // How many nodes c0 and c1 may have spread past the touched or live bricks of the last region
+    // computation (one a blur; -1 unknown), and whether a pass since then wrote past the region.
+    int spread[2] = {-1, -1};
+    bool stale = true;
     uint8_t* touched = nullptr;  // 3D: bricks P2G reached, and bricks holding velocities
     uint8_t* live = nullptr;
     int64_t bricks = 0;
@@ -286,6 +291,11 @@ bool lookup(vxr_sim* s, const std::string& name, Buf* b) {
     return true;
 }
 
+// A node outside the tension region is at least 9 nodes from every touched or live brick: the
+// colour may spread 8 nodes past them before a pass writing it must run dense, 7 before the
+// stress (which reads one node further) must.
+constexpr int kColourReach = 8;
+
 vxr::Mpm3Args mpm3_args(vxr_sim* s) {
     vxr::Mpm3Args a;
     a.nx = s->nx;
@@ -298,6 +308,7 @@ vxr::Mpm3Args mpm3_args(vxr_sim* s) {
     a.params = s->params;
     a.touched = s->touched;
     a.live = s->live;
+    a.near = s->near;
     return a;
 }
 
@@ -307,6 +318,7 @@ int mark_all_bricks(vxr_sim* s) {
     if (!s->touched) return 0;
     HIP_TRY(hipMemsetAsync(s->touched, 1, s->bricks, s->stream));
     HIP_TRY(hipMemsetAsync(s->live, 1, s->bricks, s->stream));
+    if (s->near) HIP_TRY(hipMemsetAsync(s->near, 3, s->bricks, s->stream));
     return 0;
 }
 
@@ -399,11 +411,26 @@ int run_pass(vxr_sim* s, const std::string& pass) {
             return vxr::mpm3_flags(l, a);
         }
         if (s->tension && pass.size() == 5 && pass.compare(0, 4, "blur") == 0 && pass[4] >= '0' && pass[4] <= '5') {
+            // blur k reads gm (k = 0) or the colour blur k - 1 wrote, and writes c[k % 2]
             const int k = pass[4] - '0';
+            int from = 0;
+            if (k == 0) {
+                if (int rc = vxr::tension3_region(l, a, s->stale)) return rc;
+                s->stale = false;
+            } else {
+                from = s->spread[(k - 1) % 2];
+            }
+            const bool dense = from < 0 || from + 1 > kColourReach;
+            s->spread[k % 2] = dense ? -1 : from + 1;
+            if (dense) s->stale = true;
             const float* src = k == 0 ? a.g[0] : s->colour[(k - 1) % 2];
-            return vxr::tension3_blur(l, a, src, s->colour[k % 2], k == 0);
+            return vxr::tension3_blur(l, a, src, s->colour[k % 2], k == 0, dense);
         }
-        if (s->tension && pass == "stress") return vxr::tension3_stress(l, a, s->colour[1], s->stress);
+        if (s->tension && pass == "stress") {
+            const bool dense = s->spread[1] < 0 || s->spread[1] + 1 > kColourReach;
+            if (dense) s->stale = true;
+            return vxr::tension3_stress(l, a, s->colour[1], s->stress, dense);
+        }
         if (pass == "lod") return vxr::mpm3_lod(l, a, s->level, s->target, s->stay);
         if (pass == "lod_target") return vxr::mpm3_lod_target(l, a, s->level, s->target, s->view);
         if (pass == "advect") {
@@ -432,6 +459,7 @@ int run_pass(vxr_sim* s, const std::string& pass) {
 // ------------------------------------------------------------------ advance, graphs
 
 const char* const kStep[4] = {"clear", "scatter", "grid", "advect"};
+
 // Flip3Step's step with tension (core/particle/Flip3Step.java:88-109)
 const char* const kTensionStep[10] = {"clear", "scatter", "blur0", "blur1", "blur2", "blur3", "blur4", "blur5",
                                       "stress", "grid"};
@@ -630,10 +658,16 @@ int clear_all(vxr_sim* s) {
         for (float* f : s->g) HIP_TRY(hipMemsetAsync(f, 0, round_up(s->nodes, 4) * 4, s->stream));
         HIP_TRY(hipMemsetAsync(s->params, 0, s->param_words * 4, s->stream));
         if (int rc = mark_all_bricks(s)) return rc;
-        for (float* f : s->colour)
-            if (f) HIP_TRY(hipMemsetAsync(f, 0, round_up(s->nodes, 4) * 4, s->stream));
-        for (float* f : s->stress)
-            if (f) HIP_TRY(hipMemsetAsync(f, 0, round_up(s->nodes, 4) * 4, s->stream));
+        if (s->tension) {
+            // what the full passes give a grid without mass: +0 colour and diagonal stress, -0
+            // off-diagonal stress
+            for (float* f : s->colour) HIP_TRY(hipMemsetAsync(f, 0, round_up(s->nodes, 4) * 4, s->stream));
+            for (int k = 0; k < 6; ++k)
+                HIP_TRY(hipMemsetD32Async((hipDeviceptr_t)s->stress[k], k < 3 ? 0u : 0x80000000u,
+                                          round_up(s->nodes, 4), s->stream));
+            s->spread[0] = s->spread[1] = 0;
+            s->stale = true;
+        }
         if (s->stay) {
             const int64_t groups = std::max<int64_t>(s->count / 8, 1);
             HIP_TRY(hipMemsetAsync(s->level, 0, groups * 4, s->stream));
@@ -842,6 +876,7 @@ int vxr_create(int kind, const int32_t dims[3], int64_t particles, uint32_t flag
             s->tension = true;
             for (int k = 0; k < 2 && !rc; ++k) rc = alloc(s, round_up(s->nodes, 4), &ptr), s->colour[k] = (float*)ptr;
             for (int k = 0; k < 6 && !rc; ++k) rc = alloc(s, round_up(s->nodes, 4), &ptr), s->stress[k] = (float*)ptr;
+            if (!rc) rc = alloc(s, (vxr::brick_count3(s->nx, s->ny, s->nz) + 3) / 4, &ptr), s->near = (uint8_t*)ptr;
         }
         if (!rc && three) {
             const int64_t groups = std::max<int64_t>(s->count / 8, 1);
@@ -987,6 +1022,13 @@ int vxr_write(vxr_sim* s, const char* name, const void* words, int64_t n) {
     HIP_TRY(hipMemcpyAsync(b.ptr, words, n * 4, hipMemcpyHostToDevice, s->stream));
     if (s->kind == VXR_KIND_MPM3 && name[0] == 'g' && name[1] != 0)  // a grid field: sweep it all next time
         if (int rc = mark_all_bricks(s)) return rc;
+    if (s->tension && (b.ptr == s->colour[0] || b.ptr == s->colour[1] || name[0] == 't')) {
+        // tension buffers: unknown content, so the next passes run dense
+        if (b.ptr == s->colour[0]) s->spread[0] = -1;
+        if (b.ptr == s->colour[1]) s->spread[1] = -1;
+        s->stale = true;
+        if (int rc = mark_all_bricks(s)) return rc;
+    }
     HIP_TRY(hipStreamSynchronize(s->stream));
     return VXR_OK;
 }

@@ -1,8 +1,18 @@
 // 3D surface tension for gfx950: Tension3.blur, Tension3.stress, Tension3.force
 // (core/particle/Tension3.java:69-196) and Flip3.gridWithTension (core/particle/Flip3.java:263-311),
 // one node a thread, neighbour reads held to the wall ring, in the reference's binary32 operations
-// and order (the CPU restatement's results bit for bit). These passes run over the whole grid: the
-// blurred colour reaches past the nodes P2G touched.
+// and order (the CPU restatement's results bit for bit).
+//
+// Where they run. The reference sweeps the whole grid each pass. Here the blur and stress passes
+// visit, in each row of nodes, the span of bricks in the region: the 3x3x3-brick neighbourhood
+// of every brick P2G touched or the grid pass left live, now or at the previous computation. The
+// colour spreads one node a blur from the nodes with mass, which lie in touched or live bricks,
+// and a node outside the region is at least 9 nodes from every such brick. The host counts the
+// spread: while the colour has spread at most 8 nodes (at most 7 for the stress, which reads one
+// node further), every node outside the region holds what the full pass would give it: +0 colour
+// and diagonal stress, -0 off-diagonal stress (the buffers start at those values). Past that, or
+// after the host wrote these buffers, a pass runs dense. The grid pass with tension visits the
+// span of touched or live bricks, as the plain grid pass does.
 #include "common.h"
 
 namespace vxr {
@@ -20,149 +30,246 @@ __device__ __forceinline__ int held(int c, int off, int size) {
     return (int)clampf((float)c + (float)off, kWall, (float)(size - 1) - kWall);
 }
 
-__device__ __forceinline__ int64_t held_at(const Mpm3Args& a, const int (&c)[3], int di, int dj, int dk) {
-    const int x = held(c[0], di, a.nx), y = held(c[1], dj, a.ny), z = held(c[2], dk, a.nz);
-    return ((int64_t)z * a.ny + y) * a.nx + x;
+// The x-range of row (y, z) from the first to the last brick whose flags, under `mask`, are
+// nonzero (brick rows 64 at a time, by ballot). Returns false when there is none.
+__device__ __forceinline__ bool span(const Mpm3Args& a, const uint8_t* __restrict__ f0,
+                                     const uint8_t* __restrict__ f1, uint8_t mask, int y, int z, int lane, int* x0,
+                                     int* x1) {
+    const int bxn = (a.nx + kBrick - 1) / kBrick, byn = (a.ny + kBrick - 1) / kBrick;
+    const int64_t base = ((int64_t)(z / kBrick) * byn + y / kBrick) * bxn;
+    int first = INT_MAX, last = -1;
+    for (int b0 = 0; b0 < bxn; b0 += kWave) {
+        const int b = b0 + lane;
+        const bool on = b < bxn && ((f0[base + b] | (f1 ? f1[base + b] : 0)) & mask) != 0;
+        const uint64_t m = __ballot(on);
+        if (m) {
+            first = min(first, b0 + (int)__builtin_ctzll(m));
+            last = b0 + 63 - (int)__builtin_clzll(m);
+        }
+    }
+    if (last < 0) return false;
+    *x0 = first * kBrick;
+    *x1 = min((last + 1) * kBrick, a.nx);
+    return true;
 }
 
-__device__ __forceinline__ void coords(const Mpm3Args& a, int64_t node, int (&c)[3]) {
-    c[0] = (int)(node % a.nx);
-    c[1] = (int)(node / a.nx % a.ny);
-    c[2] = (int)(node / ((int64_t)a.nx * a.ny));
-}
-
-__global__ __launch_bounds__(kBlock) void blur_kernel(Mpm3Args a, const float* __restrict__ src,
-                                                      float* __restrict__ dst, int scaled) {
-    const int64_t node = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (node >= (int64_t)a.nx * a.ny * a.nz) return;
-    int c[3];
-    coords(a, node, c);
-    const int size[3] = {a.nx, a.ny, a.nz};
-    const float rho0 = bits_f(a.params[RHO0]);
-    float sum = 0.0f;
-#pragma unroll
-    for (int dk = -1; dk <= 1; dk++)
-#pragma unroll
-        for (int dj = -1; dj <= 1; dj++)
-#pragma unroll
-            for (int di = -1; di <= 1; di++) {
-                const float weight = (float)((2 - abs(di)) * (2 - abs(dj)) * (2 - abs(dk)) / 64.0);
-                float value = src[held_at(a, c, di, dj, dk)];
-                if (scaled) {
-                    float full = rho0;
-                    const int d[3] = {di, dj, dk};
-#pragma unroll
-                    for (int ax = 0; ax < 3; ax++) {
-                        const int h = held(c[ax], d[ax], size[ax]);
-                        if (h == kLow) full = 0.5f * full;
-                        if (h == size[ax] - 1 - kLow) full = 0.5f * full;
-                    }
-                    value = mn(value / full, 1.0f);
-                }
-                sum = sum + weight * value;
+// The region of the blur and stress passes, per brick: bit 0 = some brick in its 3x3x3
+// neighbourhood is touched or live (the colour of this step reaches at most 7 nodes past those
+// bricks), bit 1 = bit 0 now or at the previous computation (what was written then and must
+// now be rewritten). With `all`, every brick.
+__global__ void region_kernel(Mpm3Args a, int all) {
+    const int bxn = (a.nx + kBrick - 1) / kBrick, byn = (a.ny + kBrick - 1) / kBrick,
+              bzn = (a.nz + kBrick - 1) / kBrick;
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= (int64_t)bxn * byn * bzn) return;
+    const int bx = (int)(b % bxn), by = (int)(b / bxn % byn), bz = (int)(b / ((int64_t)bxn * byn));
+    bool near = all != 0;
+    for (int dz = -1; dz <= 1 && !near; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int x = bx + dx, y = by + dy, z = bz + dz;
+                if (x < 0 || y < 0 || z < 0 || x >= bxn || y >= byn || z >= bzn) continue;
+                const int64_t nb = ((int64_t)z * byn + y) * bxn + x;
+                near = near || (a.touched[nb] | a.live[nb]) != 0;
             }
-    dst[node] = sum;
+    const uint8_t old = a.near[b];
+    a.near[b] = (uint8_t)((near ? 1 : 0) | ((near || (old & 1)) ? 2 : 0));
 }
 
-__global__ __launch_bounds__(kBlock) void stress_kernel(Mpm3Args a, const float* __restrict__ colour,
-                                                        float* __restrict__ t0, float* __restrict__ t1,
-                                                        float* __restrict__ t2, float* __restrict__ t3,
-                                                        float* __restrict__ t4, float* __restrict__ t5) {
-    const int64_t node = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (node >= (int64_t)a.nx * a.ny * a.nz) return;
-    int c[3];
-    coords(a, node, c);
-    const float gx = 0.5f * (colour[held_at(a, c, 1, 0, 0)] - colour[held_at(a, c, -1, 0, 0)]);
-    const float gy = 0.5f * (colour[held_at(a, c, 0, 1, 0)] - colour[held_at(a, c, 0, -1, 0)]);
-    const float gz = 0.5f * (colour[held_at(a, c, 0, 0, 1)] - colour[held_at(a, c, 0, 0, -1)]);
-    const float xx = gx * gx, yy = gy * gy, zz = gz * gz;
-    const float length = sqrtf((xx + (yy + zz)) + (float)(1e-4 * 1e-4));
-    t0[node] = (yy + zz) / length;
-    t1[node] = (xx + zz) / length;
-    t2[node] = (xx + yy) / length;
-    t3[node] = -((gx * gy) / length);
-    t4[node] = -((gx * gz) / length);
-    t5[node] = -((gy * gz) / length);
+// Row bases of the nine held (y, z) neighbour rows of row (y, z): base[k][j] for dz = k - 1,
+// dy = j - 1.
+__device__ __forceinline__ void row_bases(const Mpm3Args& a, int y, int z, int64_t (&base)[3][3],
+                                          int (&ys)[3], int (&zs)[3]) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) ys[j] = held(y, j - 1, a.ny);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) zs[k] = held(z, k - 1, a.nz);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) base[k][j] = ((int64_t)zs[k] * a.ny + ys[j]) * a.nx;
+}
+
+// Tension3.blur over one row a wave (the region's span of it, or all of it when dense): the
+// 27 taps in the reference's order, sum += weight * value, no contraction.
+template <bool kScaled>
+__global__ __launch_bounds__(kBlock) void blur_rows_kernel(Mpm3Args a, const float* __restrict__ src,
+                                                           float* __restrict__ dst, int dense) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t row = (int64_t)blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
+    if (row >= (int64_t)a.ny * a.nz) return;
+    const int y = (int)(row % a.ny), z = (int)(row / a.ny);
+    int x0 = 0, x1 = a.nx;
+    if (!dense && !span(a, a.near, nullptr, 2, y, z, lane, &x0, &x1)) return;
+    int64_t base[3][3];
+    int ys[3], zs[3];
+    row_bases(a, y, z, base, ys, zs);
+    const float rho0 = bits_f(a.params[RHO0]);
+    for (int x = x0 + lane; x < x1; x += kWave) {
+        const int xs[3] = {held(x, -1, a.nx), held(x, 0, a.nx), held(x, 1, a.nx)};
+        float sum = 0.0f;
+#pragma unroll
+        for (int dk = -1; dk <= 1; dk++)
+#pragma unroll
+            for (int dj = -1; dj <= 1; dj++)
+#pragma unroll
+                for (int di = -1; di <= 1; di++) {
+                    const float weight = (float)((2 - abs(di)) * (2 - abs(dj)) * (2 - abs(dk)) / 64.0);
+                    float value = src[base[dk + 1][dj + 1] + xs[di + 1]];
+                    if (kScaled) {
+                        // full at a rest mass, halved for each axis on which the read lies on the wall ring
+                        const int h[3] = {xs[di + 1], ys[dj + 1], zs[dk + 1]};
+                        const int size[3] = {a.nx, a.ny, a.nz};
+                        float full = rho0;
+#pragma unroll
+                        for (int ax = 0; ax < 3; ax++) {
+                            if (h[ax] == kLow) full = 0.5f * full;
+                            if (h[ax] == size[ax] - 1 - kLow) full = 0.5f * full;
+                        }
+                        value = mn(value / full, 1.0f);
+                    }
+                    sum = sum + weight * value;
+                }
+        dst[row * a.nx + x] = sum;
+    }
+}
+
+// Tension3.stress over one row a wave: central differences of the colour (held reads), then
+// the six components over |g|.
+__global__ __launch_bounds__(kBlock) void stress_rows_kernel(Mpm3Args a, const float* __restrict__ colour,
+                                                             float* __restrict__ t0, float* __restrict__ t1,
+                                                             float* __restrict__ t2, float* __restrict__ t3,
+                                                             float* __restrict__ t4, float* __restrict__ t5,
+                                                             int dense) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t row = (int64_t)blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
+    if (row >= (int64_t)a.ny * a.nz) return;
+    const int y = (int)(row % a.ny), z = (int)(row / a.ny);
+    int x0 = 0, x1 = a.nx;
+    if (!dense && !span(a, a.near, nullptr, 2, y, z, lane, &x0, &x1)) return;
+    int64_t base[3][3];
+    int ys[3], zs[3];
+    row_bases(a, y, z, base, ys, zs);
+    for (int x = x0 + lane; x < x1; x += kWave) {
+        const int xm = held(x, -1, a.nx), xh = held(x, 0, a.nx), xp = held(x, 1, a.nx);
+        const float gx = 0.5f * (colour[base[1][1] + xp] - colour[base[1][1] + xm]);
+        const float gy = 0.5f * (colour[base[1][2] + xh] - colour[base[1][0] + xh]);
+        const float gz = 0.5f * (colour[base[2][1] + xh] - colour[base[0][1] + xh]);
+        const float xx = gx * gx, yy = gy * gy, zz = gz * gz;
+        const float length = sqrtf((xx + (yy + zz)) + (float)(1e-4 * 1e-4));
+        const int64_t node = row * a.nx + x;
+        t0[node] = (yy + zz) / length;
+        t1[node] = (xx + zz) / length;
+        t2[node] = (xx + yy) / length;
+        t3[node] = -((gx * gy) / length);
+        t4[node] = -((gx * gz) / length);
+        t5[node] = -((gy * gz) / length);
+    }
 }
 
 struct Stress {
     const float* t[6];  // txx tyy tzz txy txz tyz
 };
 
-__device__ __forceinline__ const float* component(const Stress& s, int a, int k) {
-    // [[xx, xy, xz], [xy, yy, yz], [xz, yz, zz]]
-    const int idx = a == k ? a : (a + k == 1 ? 3 : (a + k == 2 ? 4 : 5));
-    return s.t[idx];
-}
+// [[xx, xy, xz], [xy, yy, yz], [xz, yz, zz]]
+__device__ __forceinline__ int component(int a, int k) { return a == k ? a : (a + k == 1 ? 3 : (a + k == 2 ? 4 : 5)); }
 
-__global__ __launch_bounds__(kBlock) void grid_tension_kernel(Mpm3Args a, Stress st,
-                                                              const float* __restrict__ colour) {
-    const int64_t node = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (node >= (int64_t)a.nx * a.ny * a.nz) return;
-    int c[3];
-    coords(a, node, c);
+// Flip3.gridWithTension over the span of touched or live bricks of each row, as the plain grid
+// pass: a node there without mass gets zero velocity, and outside the span every node already
+// holds zero sums and zero velocity. Tension3.force: the flux of the stress across the node's
+// six faces, open where the colour on both sides exceeds MIN_COLOUR; across a face in the wall
+// only the normal stress crosses.
+__global__ __launch_bounds__(kBlock) void grid_tension_rows_kernel(Mpm3Args a, Stress st,
+                                                                   const float* __restrict__ colour) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t row = (int64_t)blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
+    if (row >= (int64_t)a.ny * a.nz) return;
+    const int y = (int)(row % a.ny), z = (int)(row / a.ny);
+    int x0, x1;
+    if (!span(a, a.touched, a.live, 0xff, y, z, lane, &x0, &x1)) return;
+    int64_t base[3][3];
+    int ys[3], zs[3];
+    row_bases(a, y, z, base, ys, zs);
     const int size[3] = {a.nx, a.ny, a.nz};
-    const float mass = a.g[0][node];
-    float vel[3] = {0.0f, 0.0f, 0.0f};
-    if (mass > kEmpty) {
-        const float dt = bits_f(a.params[DT]);
+    const float dt = bits_f(a.params[DT]);
+    for (int x = x0 + lane; x < x1; x += kWave) {
+        const int64_t node = row * a.nx + x;
+        const int c[3] = {x, y, z};
+        const float mass = a.g[0][node];
+        float vel[3] = {0.0f, 0.0f, 0.0f};
+        if (mass > kEmpty) {
 #pragma unroll
-        for (int ax = 0; ax < 3; ax++) vel[ax] = a.g[1 + ax][node] / mass + dt * bits_f(a.params[GX + ax]);
-        // Tension3.force: the flux of the stress across the node's six faces
-        float force[3] = {0.0f, 0.0f, 0.0f};
-        bool ring = colour[node] > kMinColour;
+            for (int ax = 0; ax < 3; ax++) vel[ax] = a.g[1 + ax][node] / mass + dt * bits_f(a.params[GX + ax]);
+            const int xm = held(x, -1, a.nx), xh = held(x, 0, a.nx), xp = held(x, 1, a.nx);
+            // the held neighbour across each face: [axis][sign < 0 ? 0 : 1]
+            const int64_t there[3][2] = {{base[1][1] + xm, base[1][1] + xp},
+                                         {base[1][0] + xh, base[1][2] + xh},
+                                         {base[0][1] + xh, base[2][1] + xh}};
+            float force[3] = {0.0f, 0.0f, 0.0f};
+            bool ring = colour[node] > kMinColour;
 #pragma unroll
-        for (int ax = 0; ax < 3; ax++) ring = ring && !(c[ax] < kLow) && !(c[ax] > size[ax] - 1 - kLow);
+            for (int ax = 0; ax < 3; ax++) ring = ring && !(c[ax] < kLow) && !(c[ax] > size[ax] - 1 - kLow);
 #pragma unroll
-        for (int ax = 0; ax < 3; ax++) {
+            for (int ax = 0; ax < 3; ax++) {
 #pragma unroll
-            for (int sign = -1; sign <= 1; sign += 2) {
-                int d[3] = {0, 0, 0};
-                d[ax] = sign;
-                const int64_t there = held_at(a, c, d[0], d[1], d[2]);
-                const bool open = ring && colour[there] > kMinColour;
-                const bool wall = sign < 0 ? !(c[ax] > kLow) : !(c[ax] < size[ax] - 1 - kLow);
-                float shear = 1.0f;
-                if (wall) shear = 0.0f;
-                if (open) {
+                for (int side = 0; side < 2; side++) {
+                    const int sign = side == 0 ? -1 : 1;
+                    const int64_t other = there[ax][side];
+                    const bool open = ring && colour[other] > kMinColour;
+                    const bool wall = sign < 0 ? !(c[ax] > kLow) : !(c[ax] < size[ax] - 1 - kLow);
+                    float shear = 1.0f;
+                    if (wall) shear = 0.0f;
+                    if (open) {
 #pragma unroll
-                    for (int k = 0; k < 3; k++) {
-                        const float* comp = component(st, ax, k);
-                        const float flux = (float)(0.5 * sign) * (comp[node] + comp[there]);
-                        force[k] = force[k] + (k == ax ? flux : shear * flux);
+                        for (int k = 0; k < 3; k++) {
+                            const float* comp = st.t[component(ax, k)];
+                            const float flux = (float)(0.5 * sign) * (comp[node] + comp[other]);
+                            force[k] = force[k] + (k == ax ? flux : shear * flux);
+                        }
                     }
                 }
             }
+            const float push = (dt * bits_f(a.params[SIGMA])) / mx(mass, kMassFloor * bits_f(a.params[RHO0]));
+#pragma unroll
+            for (int ax = 0; ax < 3; ax++) vel[ax] = vel[ax] + push * force[ax];
         }
-        const float push = (dt * bits_f(a.params[SIGMA])) / mx(mass, kMassFloor * bits_f(a.params[RHO0]));
 #pragma unroll
-        for (int ax = 0; ax < 3; ax++) vel[ax] = vel[ax] + push * force[ax];
-    }
-#pragma unroll
-    for (int ax = 0; ax < 3; ax++) {
-        if (!(c[ax] > 1)) vel[ax] = mx(vel[ax], 0.0f);
-        if (c[ax] > size[ax] - 3) vel[ax] = mn(vel[ax], 0.0f);
-        a.g[4 + ax][node] = vel[ax];
+        for (int ax = 0; ax < 3; ax++) {
+            if (!(c[ax] > 1)) vel[ax] = mx(vel[ax], 0.0f);
+            if (c[ax] > size[ax] - 3) vel[ax] = mn(vel[ax], 0.0f);
+            a.g[4 + ax][node] = vel[ax];
+        }
     }
 }
 
-inline unsigned blocks(const Mpm3Args& a) {
-    return (unsigned)(((int64_t)a.nx * a.ny * a.nz + kBlock - 1) / kBlock);
+inline unsigned row_blocks(const Mpm3Args& a) {
+    return (unsigned)(((int64_t)a.ny * a.nz + kBlock / kWave - 1) / (kBlock / kWave));
 }
 
 }  // namespace
 
-int tension3_blur(const Launch& l, const Mpm3Args& a, const float* src, float* dst, bool scaled) {
+int tension3_region(const Launch& l, const Mpm3Args& a, bool all) {
+    const int64_t bricks = brick_count3(a.nx, a.ny, a.nz);
+    hipLaunchKernelGGL(region_kernel, dim3((unsigned)((bricks + 255) / 256)), dim3(256), 0, l.stream, a, all ? 1 : 0);
+    return check_launch("tension3_region");
+}
+
+int tension3_blur(const Launch& l, const Mpm3Args& a, const float* src, float* dst, bool scaled, bool dense) {
     timed_begin(l, "tension3_blur");
-    hipLaunchKernelGGL(blur_kernel, dim3(blocks(a)), dim3(kBlock), 0, l.stream, a, src, dst, scaled ? 1 : 0);
+    if (scaled)
+        hipLaunchKernelGGL(blur_rows_kernel<true>, dim3(row_blocks(a)), dim3(kBlock), 0, l.stream, a, src, dst,
+                           dense ? 1 : 0);
+    else
+        hipLaunchKernelGGL(blur_rows_kernel<false>, dim3(row_blocks(a)), dim3(kBlock), 0, l.stream, a, src, dst,
+                           dense ? 1 : 0);
     timed_end(l, "tension3_blur");
     return check_launch("tension3_blur");
 }
 
-int tension3_stress(const Launch& l, const Mpm3Args& a, const float* colour, float* const t[6]) {
+int tension3_stress(const Launch& l, const Mpm3Args& a, const float* colour, float* const t[6], bool dense) {
     timed_begin(l, "tension3_stress");
-    hipLaunchKernelGGL(stress_kernel, dim3(blocks(a)), dim3(kBlock), 0, l.stream, a, colour, t[0], t[1], t[2], t[3],
-                       t[4], t[5]);
+    hipLaunchKernelGGL(stress_rows_kernel, dim3(row_blocks(a)), dim3(kBlock), 0, l.stream, a, colour, t[0], t[1],
+                       t[2], t[3], t[4], t[5], dense ? 1 : 0);
     timed_end(l, "tension3_stress");
     return check_launch("tension3_stress");
 }
@@ -171,7 +278,7 @@ int mpm3_grid_tension(const Launch& l, const Mpm3Args& a, float* const t[6], con
     Stress st;
     for (int k = 0; k < 6; ++k) st.t[k] = t[k];
     timed_begin(l, "grid3_tension");
-    hipLaunchKernelGGL(grid_tension_kernel, dim3(blocks(a)), dim3(kBlock), 0, l.stream, a, st, colour);
+    hipLaunchKernelGGL(grid_tension_rows_kernel, dim3(row_blocks(a)), dim3(kBlock), 0, l.stream, a, st, colour);
     timed_end(l, "grid3_tension");
     return check_launch("grid3_tension");
 }
