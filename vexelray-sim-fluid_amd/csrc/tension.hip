@@ -90,47 +90,84 @@ __device__ __forceinline__ void row_bases(const Mpm3Args& a, int y, int z, int64
         for (int j = 0; j < 3; ++j) base[k][j] = ((int64_t)zs[k] * a.ny + ys[j]) * a.nx;
 }
 
-// Tension3.blur over one row a wave (the region's span of it, or all of it when dense): the
-// 27 taps in the reference's order, sum += weight * value, no contraction.
+// Tension3.blur. A wave takes one row y of one brick layer of z (8 values of z) and, for each
+// 64 values of x in the span of that brick row in the region (or the whole row when dense),
+// marches up z: the 3x3 taps of a plane of z are loaded once (scaled on the way when `kScaled`)
+// and serve as the dk = +1, 0 and -1 planes of three successive nodes. Each node sums its 27
+// taps in the reference's order, sum += weight * value, no contraction. Held reads make the
+// planes of node z exactly held(z - 1), held(z), held(z + 1), so the march shifts them.
 template <bool kScaled>
-__global__ __launch_bounds__(kBlock) void blur_rows_kernel(Mpm3Args a, const float* __restrict__ src,
-                                                           float* __restrict__ dst, int dense) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int64_t row = (int64_t)blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
-    if (row >= (int64_t)a.ny * a.nz) return;
-    const int y = (int)(row % a.ny), z = (int)(row / a.ny);
-    int x0 = 0, x1 = a.nx;
-    if (!dense && !span(a, a.near, nullptr, 2, y, z, lane, &x0, &x1)) return;
-    int64_t base[3][3];
-    int ys[3], zs[3];
-    row_bases(a, y, z, base, ys, zs);
-    const float rho0 = bits_f(a.params[RHO0]);
-    for (int x = x0 + lane; x < x1; x += kWave) {
-        const int xs[3] = {held(x, -1, a.nx), held(x, 0, a.nx), held(x, 1, a.nx)};
-        float sum = 0.0f;
+__device__ __forceinline__ void load_plane(const Mpm3Args& a, const float* __restrict__ src, int zp,
+                                           const int (&ys)[3], const int (&xs)[3], float rho0, float (&p)[9]) {
+    const int size[3] = {a.nx, a.ny, a.nz};
 #pragma unroll
-        for (int dk = -1; dk <= 1; dk++)
+    for (int j = 0; j < 3; ++j) {
+        const int64_t base = ((int64_t)zp * a.ny + ys[j]) * a.nx;
 #pragma unroll
-            for (int dj = -1; dj <= 1; dj++)
+        for (int i = 0; i < 3; ++i) {
+            float value = src[base + xs[i]];
+            if (kScaled) {
+                // full at a rest mass, halved for each axis on which the node read lies on the wall ring
+                const int h[3] = {xs[i], ys[j], zp};
+                float full = rho0;
 #pragma unroll
-                for (int di = -1; di <= 1; di++) {
-                    const float weight = (float)((2 - abs(di)) * (2 - abs(dj)) * (2 - abs(dk)) / 64.0);
-                    float value = src[base[dk + 1][dj + 1] + xs[di + 1]];
-                    if (kScaled) {
-                        // full at a rest mass, halved for each axis on which the read lies on the wall ring
-                        const int h[3] = {xs[di + 1], ys[dj + 1], zs[dk + 1]};
-                        const int size[3] = {a.nx, a.ny, a.nz};
-                        float full = rho0;
-#pragma unroll
-                        for (int ax = 0; ax < 3; ax++) {
-                            if (h[ax] == kLow) full = 0.5f * full;
-                            if (h[ax] == size[ax] - 1 - kLow) full = 0.5f * full;
-                        }
-                        value = mn(value / full, 1.0f);
-                    }
-                    sum = sum + weight * value;
+                for (int ax = 0; ax < 3; ax++) {
+                    if (h[ax] == kLow) full = 0.5f * full;
+                    if (h[ax] == size[ax] - 1 - kLow) full = 0.5f * full;
                 }
-        dst[row * a.nx + x] = sum;
+                value = mn(value / full, 1.0f);
+            }
+            p[3 * j + i] = value;
+        }
+    }
+}
+
+template <bool kScaled>
+__global__ __launch_bounds__(kBlock) void blur_march_kernel(Mpm3Args a, const float* __restrict__ src,
+                                                            float* __restrict__ dst, int dense) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int layers = (a.nz + kBrick - 1) / kBrick;
+    const int64_t job = (int64_t)blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
+    if (job >= (int64_t)a.ny * layers) return;
+    const int y = (int)(job % a.ny), z0 = (int)(job / a.ny) * kBrick;
+    const int z1 = min(z0 + kBrick, a.nz);
+    int x0 = 0, x1 = a.nx;
+    if (!dense && !span(a, a.near, nullptr, 2, y, z0, lane, &x0, &x1)) return;
+    const int ys[3] = {held(y, -1, a.ny), held(y, 0, a.ny), held(y, 1, a.ny)};
+    const float rho0 = bits_f(a.params[RHO0]);
+    for (int xb = x0; xb < x1; xb += kWave) {
+        // lanes past the span's end recompute its last node (held x) and do not store
+        const int x = min(xb + lane, x1 - 1);
+        const int xs[3] = {held(x, -1, a.nx), held(x, 0, a.nx), held(x, 1, a.nx)};
+        float lo[9], mid[9], hi[9];
+        load_plane<kScaled>(a, src, held(z0, -1, a.nz), ys, xs, rho0, lo);
+        load_plane<kScaled>(a, src, held(z0, 0, a.nz), ys, xs, rho0, mid);
+        load_plane<kScaled>(a, src, held(z0, 1, a.nz), ys, xs, rho0, hi);
+#pragma unroll
+        for (int dz = 0; dz < kBrick; ++dz) {
+            const int z = z0 + dz;
+            if (z >= z1) break;
+            float sum = 0.0f;
+#pragma unroll
+            for (int dk = -1; dk <= 1; dk++)
+#pragma unroll
+                for (int dj = -1; dj <= 1; dj++)
+#pragma unroll
+                    for (int di = -1; di <= 1; di++) {
+                        const float weight = (float)((2 - abs(di)) * (2 - abs(dj)) * (2 - abs(dk)) / 64.0);
+                        const int t = 3 * (dj + 1) + (di + 1);
+                        sum = sum + weight * (dk < 0 ? lo[t] : (dk == 0 ? mid[t] : hi[t]));
+                    }
+            if (xb + lane < x1) dst[((int64_t)z * a.ny + y) * a.nx + x] = sum;
+            if (z + 1 < z1) {
+#pragma unroll
+                for (int t = 0; t < 9; ++t) {
+                    lo[t] = mid[t];
+                    mid[t] = hi[t];
+                }
+                load_plane<kScaled>(a, src, held(z + 1, 1, a.nz), ys, xs, rho0, hi);
+            }
+        }
     }
 }
 
@@ -255,13 +292,13 @@ int tension3_region(const Launch& l, const Mpm3Args& a, bool all) {
 }
 
 int tension3_blur(const Launch& l, const Mpm3Args& a, const float* src, float* dst, bool scaled, bool dense) {
+    const int64_t jobs = (int64_t)a.ny * ((a.nz + kBrick - 1) / kBrick);
+    const dim3 grid((unsigned)((jobs + kBlock / kWave - 1) / (kBlock / kWave)));
     timed_begin(l, "tension3_blur");
     if (scaled)
-        hipLaunchKernelGGL(blur_rows_kernel<true>, dim3(row_blocks(a)), dim3(kBlock), 0, l.stream, a, src, dst,
-                           dense ? 1 : 0);
+        hipLaunchKernelGGL(blur_march_kernel<true>, grid, dim3(kBlock), 0, l.stream, a, src, dst, dense ? 1 : 0);
     else
-        hipLaunchKernelGGL(blur_rows_kernel<false>, dim3(row_blocks(a)), dim3(kBlock), 0, l.stream, a, src, dst,
-                           dense ? 1 : 0);
+        hipLaunchKernelGGL(blur_march_kernel<false>, grid, dim3(kBlock), 0, l.stream, a, src, dst, dense ? 1 : 0);
     timed_end(l, "tension3_blur");
     return check_launch("tension3_blur");
 }
