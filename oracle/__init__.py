@@ -59,7 +59,8 @@ def _load() -> ctypes.CDLL:
         "orc_mpm3_advect": (None, [_c_i, _c_i, _c_i, _c_i64, _vp, _i32p, _f32p, _f32p, _f32p]),
         "orc_mpm3_step": (None, [_c_i, _c_i, _c_i, _c_i64, _vp, _i32p, _vp]),
         "orc_mpm3_key_bits": (_c_i, [_c_i, _c_i, _c_i]),
-        "orc_mpm3_keys": (None, [_c_i, _c_i, _c_i, _c_i64, _f32p, _f32p, _f32p, _i32p]),
+        "orc_mpm3_keys": (None, [_c_i, _c_i, _c_i, _c_i64, _f32p, _f32p, _f32p, _f32p, _i32p]),
+        "orc_mpm3_inactive_key": (ctypes.c_int32, [_c_i, _c_i, _c_i]),
         "orc_mpm2_clear": (None, [_c_i, _c_i, _f32p, _f32p, _f32p]),
         "orc_mpm2_scatter": (None, [_c_i, _c_i, _c_i64, _vp, _i32p, _f32p, _f32p, _f32p]),
         "orc_mpm2_grid": (None, [_c_i, _c_i, _i32p, _f32p, _f32p, _f32p, _f32p, _f32p]),
@@ -344,7 +345,7 @@ class Mpm3:
 
     def keys(self):
         out = np.empty(self.count, np.int32)
-        lib.orc_mpm3_keys(*self.n, self.count, self.p["x"], self.p["y"], self.p["z"], out)
+        lib.orc_mpm3_keys(*self.n, self.count, self.p["x"], self.p["y"], self.p["z"], self.p["m"], out)
         return out
 
     def lod(self, level, target, stay):
@@ -365,6 +366,10 @@ class Mpm3:
 
 def mpm3_key_bits(nx, ny, nz) -> int:
     return lib.orc_mpm3_key_bits(nx, ny, nz)
+
+
+def mpm3_inactive_key(nx, ny, nz) -> int:
+    return lib.orc_mpm3_inactive_key(nx, ny, nz)
 
 
 class Mpm2:

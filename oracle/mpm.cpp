@@ -180,30 +180,47 @@ void orc_mpm3_step(int nx, int ny, int nz, int64_t count, float* const p[17], co
 
 // The 3D sort key: Stencil3's base cell, numbered brick-major (8x8x8-cell bricks, bricks
 // row-major, cells row-major within a brick). A bijection of the base cell, so a stable
-// sort by it groups each cell's particles and keeps a brick's particles together.
+// sort by it groups each cell's particles and keeps a brick's particles together. A slot without
+// mass (m <= 0: a tail slot, or one the level of detail emptied) takes the key one past the
+// largest base-cell key, so the sort moves every active slot ahead of every inactive one.
 static inline int brick_count(int n) { return (n - 2 + 7) / 8; }
 
+static int32_t base_key(int nx, int ny, int nz, const int32_t (&cell)[3]) {
+    const int bx = brick_count(nx), by = brick_count(ny);
+    (void)nz;
+    int32_t brick = ((cell[2] >> 3) * by + (cell[1] >> 3)) * bx + (cell[0] >> 3);
+    int32_t local = ((cell[2] & 7) * 8 + (cell[1] & 7)) * 8 + (cell[0] & 7);
+    return brick * 512 + local;
+}
+
+int32_t orc_mpm3_inactive_key(int nx, int ny, int nz) {
+    const int32_t last[3] = {nx - 3, ny - 3, nz - 3};
+    return base_key(nx, ny, nz, last) + 1;
+}
+
 int orc_mpm3_key_bits(int nx, int ny, int nz) {
-    int64_t keys = (int64_t)brick_count(nx) * brick_count(ny) * brick_count(nz) * 512;
+    const int64_t keys = (int64_t)orc_mpm3_inactive_key(nx, ny, nz) + 1;
     int bits = 0;
     while ((1LL << bits) < keys) bits++;
     return bits;
 }
 
 void orc_mpm3_keys(int nx, int ny, int nz, int64_t count, const float* x, const float* y, const float* z,
-                   int32_t* keys) {
-    const int bx = brick_count(nx), by = brick_count(ny);
+                   const float* m, int32_t* keys) {
     const float* at[3] = {x, y, z};
     const int size[3] = {nx, ny, nz};
+    const int32_t inactive = orc_mpm3_inactive_key(nx, ny, nz);
     for (int64_t q = 0; q < count; q++) {
+        if (!(m[q] > 0.0f)) {
+            keys[q] = inactive;
+            continue;
+        }
         int32_t cell[3];
         for (int a = 0; a < 3; a++) {
             float px = clampf(at[a][q], 0.5f, (float)size[a] - 1.5f);
             cell[a] = (int32_t)mn(px - 0.5f, (float)(size[a] - 3));
         }
-        int32_t brick = ((cell[2] >> 3) * by + (cell[1] >> 3)) * bx + (cell[0] >> 3);
-        int32_t local = ((cell[2] & 7) * 8 + (cell[1] & 7)) * 8 + (cell[0] & 7);
-        keys[q] = brick * 512 + local;
+        keys[q] = base_key(nx, ny, nz, cell);
     }
 }
 

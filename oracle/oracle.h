@@ -78,10 +78,13 @@ void orc_mpm3_advect(int nx, int ny, int nz, int64_t count, float* const p[17],
 void orc_mpm3_step(int nx, int ny, int nz, int64_t count, float* const p[17],
                    const int32_t params[7], float* const grid[7]);
 /* The 3D sort key: the stencil base cell (Flip3.Stencil3.of) renumbered brick-major,
- * 8x8x8-cell bricks. Returns the number of key bits. */
+ * 8x8x8-cell bricks; a slot with m <= 0 takes orc_mpm3_inactive_key, one past every cell's key.
+ * orc_mpm3_key_bits returns the number of key bits. The reference's 3D step has no sort: this
+ * is the key this build sorts by. */
+int32_t orc_mpm3_inactive_key(int nx, int ny, int nz);
 int orc_mpm3_key_bits(int nx, int ny, int nz);
 void orc_mpm3_keys(int nx, int ny, int nz, int64_t count, const float* x, const float* y,
-                   const float* z, int32_t* keys);
+                   const float* z, const float* m, int32_t* keys);
 
 /* 3D surface tension: Tension3.blur (core/particle/Tension3.java:69-107; scaled: the mass to a
  * colour), Tension3.stress (:122-147; t = txx tyy tzz txy txz tyz), Tension3.force (:156-196)

@@ -87,7 +87,9 @@ def sort3_case():
 
 def sort3_run(c):
     n = int(c["n"])
-    s = oracle.Mpm3(n, n, n, {"x": c["x0"], "y": c["y0"], "z": c["z0"]}, oracle.mpm3_params(1e-3, 0, 0, 0, 0, 1))
+    m = np.ones_like(c["x0"])  # every particle active
+    s = oracle.Mpm3(n, n, n, {"x": c["x0"], "y": c["y0"], "z": c["z0"], "m": m},
+                    oracle.mpm3_params(1e-3, 0, 0, 0, 0, 1))
     keys = s.keys()
     return dict(keys=keys, order=oracle.stable_order(keys))
 

@@ -84,10 +84,10 @@ def test_device_sort_fixture(gpu):
     rig.write("x", i["x0"])
     rig.write("y", i["y0"])
     rig.write("z", i["z0"])
-    rig.write("m", np.arange(count, dtype=np.float32))  # the particle's index rides along
+    rig.write("m", np.arange(1, count + 1, dtype=np.float32))  # index + 1 rides along (all active)
     rig.run(["sort"])
     np.testing.assert_array_equal(rig.sim.read_words("keys"), e["keys"][e["order"]])
-    np.testing.assert_array_equal(rig.read("m").astype(np.int64), e["order"])
+    np.testing.assert_array_equal(rig.read("m").astype(np.int64) - 1, e["order"])
 
 
 SCHEDULES = [dict(p2g=0, g2p=0), dict(p2g=1, g2p=1, run=4), dict(p2g=1, g2p=2, run=4)]

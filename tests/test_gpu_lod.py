@@ -30,21 +30,60 @@ class GpuLodRig:
             self.sim.run_pass(what)
 
 
+class SortingGpuLodRig(GpuLodRig):
+    """The device rig with a sort by cell before every lod and lod_target call: the slots move
+    (the inactive ones behind every active one) and the passes reach each group's slots through
+    the slot map. Particle fields are written and read in slot order."""
+
+    def _slots(self):
+        return self.sim.read_words("slot")
+
+    def write(self, name, values):
+        if name in oracle.FIELDS3:
+            values = np.asarray(values, np.float32)[self._slots()]
+        super().write(name, values)
+
+    def read(self, name):
+        v = super().read(name)
+        if name in oracle.FIELDS3:
+            out = np.empty_like(v)
+            out[self._slots()] = v
+            return out
+        return v
+
+    def run(self, what):
+        if what in ("lod", "lod_target"):
+            self.sim.run_pass("sort")
+            m = super().read("m")
+            active = int(np.count_nonzero(m > 0))
+            # the sort put every slot with mass first, and counted them
+            assert int(self.sim.read_words("active")[0]) == active
+            assert np.all(m[:active] > 0) and not np.any(m[active:] > 0)
+        super().run(what)
+
+
+RIGS = [GpuLodRig, SortingGpuLodRig]
+
+
+@pytest.mark.parametrize("rig", RIGS, ids=["slots-in-place", "sorted"])
 @pytest.mark.parametrize("case", cases.CASES, ids=[c.__name__ for c in cases.CASES])
-def test_lod_case_on_the_device(gpu, case):
-    print(f"[lod] {case.__name__}", flush=True)
-    case(GpuLodRig)
+def test_lod_case_on_the_device(gpu, case, rig):
+    print(f"[lod] {case.__name__} {rig.__name__}", flush=True)
+    case(rig)
 
 
-def test_lod_target_on_the_device(gpu):
-    cases.case_lod_target(lambda slots: GpuLodRig(cases.N, slots))
+@pytest.mark.parametrize("rig", RIGS, ids=["slots-in-place", "sorted"])
+def test_lod_target_on_the_device(gpu, rig):
+    cases.case_lod_target(lambda slots: rig(cases.N, slots))
 
 
-def test_every_call_is_the_oracles(gpu):
+@pytest.mark.parametrize("rig", RIGS, ids=["slots-in-place", "sorted"])
+def test_every_call_is_the_oracles(gpu, rig):
     """Thinning to 1, then 5, then filling to 8: after every call, every field, stay and level
-    bit for bit the oracle's (and a fresh slot set down at the group's mean the same)."""
-    print("[lod] bit-exact sequence", flush=True)
-    dev, ref = GpuLodRig(), cases.OracleLodRig()
+    bit for bit the oracle's (and a fresh slot set down at the group's mean the same); sorted,
+    the slots have moved before every call."""
+    print(f"[lod] bit-exact sequence {rig.__name__}", flush=True)
+    dev, ref = rig(), cases.OracleLodRig()
     for rig in (dev, ref):
         cases.seed(rig)
     for want, calls in ((1, 40), (5, 25), (8, 40)):

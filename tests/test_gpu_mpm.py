@@ -184,8 +184,10 @@ def test_fast_advect_close_3d(gpu):
 # ------------------------------------------------------------------ sort
 
 def test_sort_3d_bit_exact(gpu):
-    """The device sort of shuffled particles: keys ascending, and the permutation the host's stable
-    sort by the same brick key gives, every field bit for bit."""
+    """The device sort of shuffled particles, a fifth of them without mass: keys ascending, and
+    the permutation the host's stable sort by the same brick key gives (the massless slots last,
+    in slot order), every field bit for bit; the slot map is that permutation and the active
+    count is the number of slots with mass."""
     n = 48
     x, y, z = scenes.jittered_block3((2, 2, 2), (40, 30, 20), 2, scenes.JavaRandomFloats(21))
     count = len(x)
@@ -195,9 +197,13 @@ def test_sort_3d_bit_exact(gpu):
     for name in oracle.FIELDS3[3:]:
         f[name] = rng.standard_normal(count).astype(np.float32)
     f["m"] = np.abs(f["m"]) + np.float32(0.1)
+    dead = rng.random(count) < 0.2
+    f["m"][dead] = 0.0
+    f["m"][np.flatnonzero(dead)[:7]] = -0.5
     ref = oracle.Mpm3(n, n, n, f, oracle.mpm3_params(1e-3, 0, 0, 0, 0, 1))
     rig = maker3()(n, n, n, count)
     cases.load3(rig, f)
+    np.testing.assert_array_equal(rig.sim.read_words("slot"), np.arange(count))
     rig.run(["sort"])
     keys = rig.sim.read_words("keys")
     perm = ref.sort()
@@ -206,6 +212,68 @@ def test_sort_3d_bit_exact(gpu):
     for name in oracle.FIELDS3:
         np.testing.assert_array_equal(rig.read(name).view(np.int32), ref.p[name].view(np.int32), err_msg=name)
     assert perm.shape == (count,)
+    np.testing.assert_array_equal(rig.sim.read_words("slot"), perm)
+    active = int(np.count_nonzero(~dead))
+    assert int(rig.sim.read_words("active")[0]) == active
+    np.testing.assert_array_equal(perm[active:], np.flatnonzero(dead))
+
+
+def test_inactive_slots_skipped_after_sort_3d(gpu):
+    """A block with 40% of its slots emptied, sorted every 5 steps: the sort moves the empty slots
+    behind the rest and P2G and G2P stop at the last active one. Twenty steps track the oracle
+    (which visits every slot and skips the massless ones) slot for slot within 1e-5, and the
+    empty slots come through untouched, bit for bit."""
+    n = 32
+    x, y, z = scenes.jittered_block3((3, 3, 3), (12, 14, 10), 8, scenes.JavaRandomFloats(9))
+    count = len(x)
+    rng = np.random.default_rng(4)
+    m = np.full(count, 0.125, np.float32)
+    dead = rng.random(count) < 0.4
+    m[dead] = 0.0
+    u = rng.standard_normal(count).astype(np.float32)
+    params = oracle.mpm3_params(5e-4, 0, -200.0, 0, 2000.0, 1)
+    f = {"x": x, "y": y, "z": z, "m": m, "u": u}
+    ref = oracle.Mpm3(n, n, n, f, params)
+    rig = maker3()(n, n, n, count)
+    for name, v in (("x", x), ("y", y), ("z", z), ("m", m), ("u", u), ("j", np.ones(count, np.float32))):
+        rig.write(name, v)
+    rig.params(params)
+    for k in range(20):
+        if k % 5 == 0:
+            rig.run(["sort"])
+            assert int(rig.sim.read_words("active")[0]) == count - int(dead.sum())
+        ref.step()
+        rig.run("step")
+    slots = rig.sim.read_words("slot")
+
+    def by_slot(name):
+        v = rig.read(name)
+        out = np.empty_like(v)
+        out[slots] = v
+        return out
+
+    for name in ("x", "y", "z"):
+        np.testing.assert_allclose(by_slot(name), ref.p[name], rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(by_slot("j"), ref.p["j"], rtol=1e-4, atol=1e-5)
+    # the oracle never touches a massless slot either
+    for name in oracle.FIELDS3:
+        np.testing.assert_array_equal(by_slot(name)[dead].view(np.int32), ref.p[name][dead].view(np.int32),
+                                      err_msg=name)
+    mass = float(np.sum(m, dtype=np.float64))
+    assert abs(float(np.sum(rig.read("gm"), dtype=np.float64)) - mass) <= 1e-6 * mass
+
+
+def test_slot_writes_are_checked(gpu):
+    """The slot map takes only a permutation of the slots; the active count is read-only."""
+    rig = maker3()(16, 16, 16, 64)
+    with pytest.raises(_native.InvalidArgument):
+        rig.sim.write("slot", np.zeros(64, np.int32))
+    with pytest.raises(_native.InvalidArgument):
+        rig.sim.write("slot", np.arange(1, 65, dtype=np.int32))
+    with pytest.raises(_native.InvalidArgument):
+        rig.sim.write("active", np.zeros(1, np.int32))
+    rig.sim.write("slot", np.arange(64, dtype=np.int32)[::-1].copy())
+    np.testing.assert_array_equal(rig.sim.read_words("slot"), np.arange(64)[::-1])
 
 
 def _sorter2(n, p, count):
@@ -273,7 +341,7 @@ def test_config_two_cubes_full_size(gpu):
     z = np.zeros(count, np.float32)
     sim.load((s["x"], s["y"], s["z"]), (z, z, z), s["m"], params)
     keys_host = np.empty(count, np.int32)
-    oracle.lib.orc_mpm3_keys(256, 256, 256, count, s["x"], s["y"], s["z"], keys_host)
+    oracle.lib.orc_mpm3_keys(256, 256, 256, count, s["x"], s["y"], s["z"], s["m"], keys_host)
     perm = np.argsort(keys_host, kind="stable")
     sim.sort()
     np.testing.assert_array_equal(sim.sim.read_words("keys"), keys_host[perm])

@@ -70,13 +70,43 @@ def test_brick_keys_are_a_bijection_of_the_base_cell():
     x = (cx.ravel() + 0.75).astype(np.float32)
     y = (cy.ravel() + 0.75).astype(np.float32)
     z = (cz.ravel() + 0.75).astype(np.float32)
-    s = oracle.Mpm3(n, n, n, {"x": x, "y": y, "z": z}, oracle.mpm3_params(1e-3, 0, 0, 0, 0, 1))
+    m = np.ones_like(x)
+    s = oracle.Mpm3(n, n, n, {"x": x, "y": y, "z": z, "m": m}, oracle.mpm3_params(1e-3, 0, 0, 0, 0, 1))
     keys = s.keys()
     assert len(np.unique(keys)) == len(keys)
     bits = oracle.mpm3_key_bits(n, n, n)
     assert keys.max() < (1 << bits)
+    # the inactive key is one past the largest cell's (the base cell n - 3 on every axis)
+    assert oracle.mpm3_inactive_key(n, n, n) == keys.max() + 1 < (1 << bits)
     # cells within one 8x8x8 brick share its 512-key block
     base = ((z - 0.5).astype(np.int64), (y - 0.5).astype(np.int64), (x - 0.5).astype(np.int64))
     brick = (base[0] // 8) * 9 + (base[1] // 8) * 3 + base[2] // 8
     np.testing.assert_array_equal(keys // 512, brick)
     assert oracle.mpm3_key_bits(256, 256, 256) == 24
+
+
+@pytest.mark.parametrize("n", [20, 256, 258])
+def test_inactive_slots_sort_last_in_slot_order(n):
+    """A slot without mass (m <= 0, or NaN) keys past every cell; the stable sort then puts the
+    active slots first, by cell, and the inactive ones after them in their slot order. The key
+    bits grow only when the last brick is full (n - 2 a multiple of 8)."""
+    rng = np.random.default_rng(n)
+    count = 5000
+    x, y, z = (rng.uniform(0, n, count).astype(np.float32) for _ in range(3))
+    m = rng.uniform(0.1, 1, count).astype(np.float32)
+    dead = rng.random(count) < 0.3
+    m[dead] = 0.0
+    m[np.flatnonzero(dead)[:5]] = -1.0
+    m[np.flatnonzero(dead)[5:10]] = np.nan
+    s = oracle.Mpm3(n, n, n, {"x": x, "y": y, "z": z, "m": m}, oracle.mpm3_params(1e-3, 0, 0, 0, 0, 1))
+    keys = s.keys()
+    inactive = oracle.mpm3_inactive_key(n, n, n)
+    np.testing.assert_array_equal(keys == inactive, dead)
+    assert keys[~dead].max() < inactive < (1 << oracle.mpm3_key_bits(n, n, n))
+    order = oracle.stable_order(keys)
+    active = int((~dead).sum())
+    assert not dead[order[:active]].any()
+    np.testing.assert_array_equal(order[active:], np.flatnonzero(dead))
+    full = (n - 2) % 8 == 0
+    cells = ((n - 2 + 7) // 8) ** 3 * 512
+    assert oracle.mpm3_key_bits(n, n, n) == int(np.ceil(np.log2(cells + (1 if full else 0))))

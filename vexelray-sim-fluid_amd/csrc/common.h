@@ -74,6 +74,13 @@ struct Mpm3Args {
     // Clear and grid visit, in each row of nodes, only the span of bricks flagged in either.
     uint8_t* touched;
     uint8_t* live;
+    // The slot each particle was loaded as (permuted with the particles by a sort), the position
+    // of each slot (home[slot[p]] = p), and the particles from the first whose key is the
+    // inactive one (m <= 0) on: P2G and G2P skip them; equal to count unless the last sort
+    // found inactive slots and nothing has given mass to a slot since.
+    int32_t* slot;
+    int32_t* home;
+    int32_t* active;
     // With tension: per brick, bit 0 = within one brick of a touched or live brick at the last
     // region computation, bit 1 = bit 0 then or at the computation before (the blur and stress
     // passes visit the span of bit 1).
@@ -91,6 +98,15 @@ int mpm3_grid(const Launch& l, const Mpm3Args& a);
 int mpm3_advect(const Launch& l, const Mpm3Args& a);
 int mpm3_advect_tiled(const Launch& l, const Mpm3Args& a, int run, bool fast);
 int mpm3_keys(const Launch& l, const Mpm3Args& a, int32_t* keys, int32_t* index);
+// The key of a slot with m <= 0: one past every base cell's, so the sort puts those slots last.
+int32_t mpm3_inactive_key(int nx, int ny, int nz);
+// After the permute of a sort (`a` over the permuted fields): home from slot, and active from
+// the sorted keys.
+int mpm3_after_sort(const Launch& l, const Mpm3Args& a, const int32_t* keys);
+// Every slot at its own position (slot = home = identity) and active = count.
+int mpm3_reset_slots(const Launch& l, const Mpm3Args& a);
+// home from slot (after the host wrote slot), and active = count.
+int mpm3_rebuild_home(const Launch& l, const Mpm3Args& a);
 // After a grid pass that did not go through mpm3_grid: touched bricks become live, touched clears.
 int mpm3_flags(const Launch& l, const Mpm3Args& a);
 // 3D surface tension (core/particle/Tension3.java): blur (scaled: gm to a colour), stress

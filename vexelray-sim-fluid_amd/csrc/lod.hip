@@ -1,7 +1,8 @@
 // The 3D level of detail for gfx950: Flip3.lod (core/particle/Flip3.java:420-586) and
 // Flip3.lodTarget (:600-641), one thread a group of GROUP = 8 consecutive slots, in the
 // reference's binary32 operations and order (so the CPU restatement's results, bit for bit).
-// A group's 8 slots are 8 consecutive floats of each field: a wave reads 2 KB contiguous per field.
+// A group is the 8 slots loaded consecutively; a sort moves slots, so a slot's fields are read and
+// written at its position home[slot] (the identity until the first sort).
 #include "common.h"
 
 namespace vxr {
@@ -16,14 +17,19 @@ constexpr int kMass = 6, kVolume = 7;
 __global__ __launch_bounds__(kBlock) void lod_kernel(Mpm3Args a, int64_t groups, float* __restrict__ level,
                                                      const float* __restrict__ target, float* __restrict__ stay) {
     const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    // slots may gain mass: until the next sort, P2G and G2P visit every particle
+    if (g == 0) *a.active = (int32_t)a.count;
     if (g >= groups) return;
     const int size[3] = {a.nx, a.ny, a.nz};
     const int64_t base = g * kGroup;
+    int64_t at[kGroup];
+#pragma unroll
+    for (int s = 0; s < kGroup; s++) at[s] = a.home[base + s];
     float m[kGroup], keep[kGroup];
     float total = 0.0f, count = 0.0f;
 #pragma unroll
     for (int s = 0; s < kGroup; s++) {
-        m[s] = a.p[kMass][base + s];
+        m[s] = a.p[kMass][at[s]];
         keep[s] = stay[base + s];
         total = total + m[s];
         count = count + keep[s];
@@ -46,7 +52,7 @@ __global__ __launch_bounds__(kBlock) void lod_kernel(Mpm3Args a, int64_t groups,
         if (f == kMass) continue;
         float acc = 0.0f;
 #pragma unroll
-        for (int s = 0; s < kGroup; s++) acc = acc + m[s] * a.p[f][base + s];
+        for (int s = 0; s < kGroup; s++) acc = acc + m[s] * a.p[f][at[s]];
         mean[f] = acc * inv;
     }
     // too many meant to stay: the ones nearest the mean position stop, one at a time
@@ -57,7 +63,7 @@ __global__ __launch_bounds__(kBlock) void lod_kernel(Mpm3Args a, int64_t groups,
         near[s] = 0.0f;
 #pragma unroll
         for (int ax = 0; ax < 3; ax++) {
-            const float d = a.p[ax][base + s] - mean[ax];
+            const float d = a.p[ax][at[s]] - mean[ax];
             near[s] = near[s] + d * d;
         }
     }
@@ -88,18 +94,18 @@ __global__ __launch_bounds__(kBlock) void lod_kernel(Mpm3Args a, int64_t groups,
         float offset[3];
 #pragma unroll
         for (int ax = 0; ax < 3; ax++) offset[ax] = ((float)((s >> ax) & 1) - 0.5f) * 2.0f * kJoin;
-        const int64_t at = base + s;
+        const int64_t q = at[s];
 #pragma unroll
         for (int ax = 0; ax < 3; ax++) {
-            a.p[ax][at] = clampf(mean[ax] + offset[ax], kWall, (float)(size[ax] - 1) - kWall);
+            a.p[ax][q] = clampf(mean[ax] + offset[ax], kWall, (float)(size[ax] - 1) - kWall);
             float vel = mean[3 + ax];
 #pragma unroll
             for (int e = 0; e < 3; e++) vel = vel + mean[8 + 3 * ax + e] * offset[e];
-            a.p[3 + ax][at] = vel;
+            a.p[3 + ax][q] = vel;
         }
-        a.p[kVolume][at] = mean[kVolume];
+        a.p[kVolume][q] = mean[kVolume];
 #pragma unroll
-        for (int e = 0; e < 9; e++) a.p[8 + e][at] = mean[8 + e];
+        for (int e = 0; e < 9; e++) a.p[8 + e][q] = mean[8 + e];
     }
     // the shares those meant to stay now have, who has too much and who too little
     float stayers = 0.0f;
@@ -125,14 +131,14 @@ __global__ __launch_bounds__(kBlock) void lod_kernel(Mpm3Args a, int64_t groups,
             if (f == kMass) continue;
             float given = 0.0f;
 #pragma unroll
-            for (int s = 0; s < kGroup; s++) given = given + extra[s] * a.p[f][base + s];
+            for (int s = 0; s < kGroup; s++) given = given + extra[s] * a.p[f][at[s]];
             given = given * inverse;
 #pragma unroll
             for (int s = 0; s < kGroup; s++)
-                if (gain[s] > 0.0f) a.p[f][base + s] = (m[s] * a.p[f][base + s] + gain[s] * given) / (m[s] + gain[s]);
+                if (gain[s] > 0.0f) a.p[f][at[s]] = (m[s] * a.p[f][at[s]] + gain[s] * given) / (m[s] + gain[s]);
         }
 #pragma unroll
-        for (int s = 0; s < kGroup; s++) a.p[kMass][base + s] = (m[s] + gain[s]) - rate * extra[s];
+        for (int s = 0; s < kGroup; s++) a.p[kMass][at[s]] = (m[s] + gain[s]) - rate * extra[s];
     }
 #pragma unroll
     for (int s = 0; s < kGroup; s++) stay[base + s] = keep[s];
@@ -145,11 +151,11 @@ __global__ __launch_bounds__(kBlock) void lod_target_kernel(Mpm3Args a, int64_t 
                                                             const float* __restrict__ view) {
     const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (g >= groups) return;
-    const int64_t slot = g * kGroup;
+    const int64_t first = a.home[g * kGroup];  // the group's first slot
     float d2 = 0.0f;
 #pragma unroll
     for (int ax = 0; ax < 3; ax++) {
-        const float d = a.p[ax][slot] - view[ax];
+        const float d = a.p[ax][first] - view[ax];
         d2 = d2 + d * d;
     }
     const float near = view[3];

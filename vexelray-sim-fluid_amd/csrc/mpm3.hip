@@ -13,6 +13,7 @@
 //
 // The grid update and G2P keep the reference's arithmetic order exactly, so from the same
 // grid they reproduce the CPU restatement bit for bit; G2P reads its nodes from an LDS tile.
+#include <algorithm>
 #include <climits>
 
 #include "common.h"
@@ -185,7 +186,7 @@ __device__ __forceinline__ void atomic_particle(const Mpm3Args& a, const Axis (&
 
 __global__ __launch_bounds__(kBlock) void p2g3_atomic_kernel(Mpm3Args a) {
     const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (q >= a.count) return;
+    if (q >= *a.active) return;  // the inactive slots a sort moved to the end
     if (!(a.p[6][q] > 0.0f)) return;
     const float dt = bits_f(a.params[DT]), bulk = bits_f(a.params[BULK]), rho0 = bits_f(a.params[RHO0]);
     Particle p;
@@ -487,6 +488,7 @@ __global__ __launch_bounds__(kWave) void p2g3_wave_kernel(Mpm3Args a) {
     __shared__ float4 tile[kWaveTile];
     __shared__ int claim[kWaveTile];
     const int lane = threadIdx.x;
+    if ((int64_t)blockIdx.x * kWave * R >= *a.active) return;  // a wave of inactive slots
     const int64_t first = ((int64_t)blockIdx.x * kWave + lane) * R;  // first + R <= capacity
     float f[17][R];
 #pragma unroll
@@ -755,7 +757,7 @@ __device__ __forceinline__ void g2p_fast(const Mpm3Args& a, int64_t q, const flo
 
 __global__ __launch_bounds__(kBlock) void g2p3_kernel(Mpm3Args a) {
     const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (q >= a.count || !(a.p[6][q] > 0.0f)) return;
+    if (q >= *a.active || !(a.p[6][q] > 0.0f)) return;
     const float pos[3] = {a.p[0][q], a.p[1][q], a.p[2][q]};
     Axis ax[3] = {axis(pos[0], a.nx), axis(pos[1], a.ny), axis(pos[2], a.nz)};
     const int64_t base = ((int64_t)ax[2].cell * a.ny + ax[1].cell) * a.nx + ax[0].cell;
@@ -775,6 +777,7 @@ __global__ __launch_bounds__(kBlock) void g2p3_tiled_kernel(Mpm3Args a) {
     __shared__ float tile[3 * kG2PTile];
     __shared__ int red[6 * (kBlock / kWave)];
     const int64_t chunk = (int64_t)blockIdx.x * kBlock * R;
+    if (chunk >= *a.active) return;  // a block of inactive slots
     float pos[R][3];
     bool live[R];
     const int size[3] = {a.nx, a.ny, a.nz};
@@ -837,11 +840,14 @@ __host__ __device__ __forceinline__ int brick_count(int n) { return (n - 2 + 7) 
 // Stencil3's base cell numbered brick-major: 8x8x8-cell bricks row-major, cells row-major
 // inside a brick. Padding particles take the largest key so a stable sort leaves them last.
 __global__ __launch_bounds__(kBlock) void keys3_kernel(Mpm3Args a, int32_t* __restrict__ keys,
-                                                       int32_t* __restrict__ index, int32_t pad_key) {
+                                                       int32_t* __restrict__ index, int32_t pad_key,
+                                                       int32_t inactive_key) {
     const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (q >= a.capacity) return;
     int32_t key = pad_key;
-    if (q < a.count) {
+    if (q < a.count && !(a.p[6][q] > 0.0f)) {
+        key = inactive_key;
+    } else if (q < a.count) {
         const int cx = base_cell(a.p[0][q], a.nx), cy = base_cell(a.p[1][q], a.ny), cz = base_cell(a.p[2][q], a.nz);
         const int bx = brick_count(a.nx), by = brick_count(a.ny);
         const int brick = ((cz >> 3) * by + (cy >> 3)) * bx + (cx >> 3);
@@ -852,15 +858,69 @@ __global__ __launch_bounds__(kBlock) void keys3_kernel(Mpm3Args a, int32_t* __re
     index[q] = (int32_t)q;
 }
 
+// home[slot[p]] = p over the sorted particles, and active = the first p whose key is the
+// inactive one (count when there is none): the one p where the keys cross it writes it, so
+// nothing has to be cleared first.
+__global__ __launch_bounds__(kBlock) void after_sort_kernel(Mpm3Args a, const int32_t* __restrict__ keys,
+                                                            int32_t inactive_key) {
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= a.count) return;
+    a.home[a.slot[p]] = (int32_t)p;
+    const bool here = keys[p] >= inactive_key;
+    const bool before = p > 0 && keys[p - 1] >= inactive_key;
+    if (here && !before) *a.active = (int32_t)p;
+    if (!here && p == a.count - 1) *a.active = (int32_t)a.count;
+}
+
+__global__ __launch_bounds__(kBlock) void slots_kernel(Mpm3Args a, int identity) {
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p == 0) *a.active = (int32_t)a.count;
+    if (p >= a.capacity) return;
+    if (identity) {
+        a.slot[p] = (int32_t)p;
+        a.home[p] = (int32_t)p;
+    } else if (p < a.count) {
+        a.home[a.slot[p]] = (int32_t)p;
+    }
+}
+
 inline unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
 }  // namespace
 
+int32_t mpm3_inactive_key(int nx, int ny, int nz) {
+    // the key of base cell (n - 3) on every axis, plus one
+    const int c[3] = {nx - 3, ny - 3, nz - 3};
+    const int bx = (nx - 2 + 7) / 8, by = (ny - 2 + 7) / 8;
+    const int brick = ((c[2] >> 3) * by + (c[1] >> 3)) * bx + (c[0] >> 3);
+    const int local = ((c[2] & 7) * 8 + (c[1] & 7)) * 8 + (c[0] & 7);
+    return brick * 512 + local + 1;
+}
+
 int mpm3_key_bits(int nx, int ny, int nz) {
-    const int64_t keys = (int64_t)((nx - 2 + 7) / 8) * ((ny - 2 + 7) / 8) * ((nz - 2 + 7) / 8) * 512;
+    const int64_t keys = (int64_t)mpm3_inactive_key(nx, ny, nz) + 1;
     int bits = 0;
     while ((1LL << bits) < keys) bits++;
     return bits;
+}
+
+int mpm3_after_sort(const Launch& l, const Mpm3Args& a, const int32_t* keys) {
+    if (a.count == 0) return 0;
+    hipLaunchKernelGGL(after_sort_kernel, dim3(blocks_for(a.count, kBlock)), dim3(kBlock), 0, l.stream, a, keys,
+                       mpm3_inactive_key(a.nx, a.ny, a.nz));
+    return check_launch("mpm3_after_sort");
+}
+
+int mpm3_reset_slots(const Launch& l, const Mpm3Args& a) {
+    hipLaunchKernelGGL(slots_kernel, dim3(blocks_for(std::max<int64_t>(a.capacity, 1), kBlock)), dim3(kBlock), 0,
+                       l.stream, a, 1);
+    return check_launch("mpm3_reset_slots");
+}
+
+int mpm3_rebuild_home(const Launch& l, const Mpm3Args& a) {
+    hipLaunchKernelGGL(slots_kernel, dim3(blocks_for(std::max<int64_t>(a.capacity, 1), kBlock)), dim3(kBlock), 0,
+                       l.stream, a, 0);
+    return check_launch("mpm3_rebuild_home");
 }
 
 int mpm3_clear(const Launch& l, const Mpm3Args& a) {
@@ -938,7 +998,7 @@ int mpm3_keys(const Launch& l, const Mpm3Args& a, int32_t* keys, int32_t* index)
     const int32_t pad = (int32_t)((1LL << mpm3_key_bits(a.nx, a.ny, a.nz)) - 1);
     timed_begin(l, "keys3");
     hipLaunchKernelGGL(keys3_kernel, dim3(blocks_for(a.capacity, kBlock)), dim3(kBlock), 0, l.stream, a, keys, index,
-                       pad);
+                       pad, mpm3_inactive_key(a.nx, a.ny, a.nz));
     timed_end(l, "keys3");
     return check_launch("keys3");
 }

@@ -148,6 +148,8 @@ struct vxr_sim {
     int32_t* starts = nullptr;  // 2D: cells + 1
     int64_t nodes = 0;
     float *level = nullptr, *target = nullptr, *stay = nullptr, *view = nullptr;  // 3D level of detail
+    int32_t* home = nullptr;                    // 3D: the position of each slot (p[17] is the slot of each particle)
+    int32_t* active = nullptr;                  // 3D: particles before the first inactive one
     bool tension = false;                       // 3D surface tension (VXR_FLAG_TENSION)
     float* colour[2] = {};                      // c0, c1: the blurred colour, in turn
     float* stress[6] = {};                      // txx tyy tzz txy txz tyz
@@ -254,6 +256,14 @@ bool lookup(vxr_sim* s, const std::string& name, Buf* b) {
             *b = Buf{name == "level" ? s->level : s->target, groups, groups};
             return true;
         }
+        if (three && name == "slot") {
+            *b = Buf{s->p[17], s->count, s->capacity};
+            return true;
+        }
+        if (three && name == "active") {
+            *b = Buf{s->active, 1, 1};
+            return true;
+        }
         if (three && name == "stay") {
             *b = Buf{s->stay, s->count, s->capacity};
             return true;
@@ -309,6 +319,9 @@ vxr::Mpm3Args mpm3_args(vxr_sim* s) {
     a.touched = s->touched;
     a.live = s->live;
     a.near = s->near;
+    a.slot = (int32_t*)s->p[17];
+    a.home = s->home;
+    a.active = s->active;
     return a;
 }
 
@@ -385,10 +398,11 @@ int sort_pass(vxr_sim* s) {
         while ((1LL << bits) < keys) bits++;
     }
     if ((rc = vxr::radix_sort(l, s->keys, s->index, s->count, bits, (int)opt(s, "radix_bits"), s->scratch))) return rc;
-    const int nf = three ? 17 : 10;
+    const int nf = three ? 18 : 10;  // 3D: the 17 fields and the slot
     if ((rc = vxr::permute_fields(l, s->p.data(), s->p_alt.data(), nf, s->index, s->count))) return rc;
     std::swap(s->p, s->p_alt);
     s->flipped = !s->flipped;
+    if (three && (rc = vxr::mpm3_after_sort(l, mpm3_args(s), s->keys))) return rc;
     if (!three) {
         const int32_t cells = (s->nx - 1) * (s->ny - 1);
         if ((rc = vxr::mpm2_starts(l, s->keys, s->count, cells, s->starts))) return rc;
@@ -658,6 +672,8 @@ int clear_all(vxr_sim* s) {
         for (float* f : s->g) HIP_TRY(hipMemsetAsync(f, 0, round_up(s->nodes, 4) * 4, s->stream));
         HIP_TRY(hipMemsetAsync(s->params, 0, s->param_words * 4, s->stream));
         if (int rc = mark_all_bricks(s)) return rc;
+        if (s->kind == VXR_KIND_MPM3)
+            if (int rc = vxr::mpm3_reset_slots(s->launch(), mpm3_args(s))) return rc;
         if (s->tension) {
             // what the full passes give a grid without mass: +0 colour and diagonal stress, -0
             // off-diagonal stress
@@ -857,7 +873,7 @@ int vxr_create(int kind, const int32_t dims[3], int64_t particles, uint32_t flag
         if (!rc) rc = alloc(s, 5, &ptr), s->params = (int32_t*)ptr;
     } else {
         const bool three = kind == VXR_KIND_MPM3;
-        const int nf = three ? 17 : 10, ng = three ? 7 : 5;
+        const int nf = three ? 18 : 10, ng = three ? 7 : 5;  // 3D: the 17 fields and the slot
         for (int f = 0; f < nf && !rc; ++f) {
             rc = alloc(s, s->capacity, &ptr);
             s->p.push_back((float*)ptr);
@@ -884,6 +900,10 @@ int vxr_create(int kind, const int32_t dims[3], int64_t particles, uint32_t flag
             if (!rc) rc = alloc(s, groups, &ptr), s->target = (float*)ptr;
             if (!rc) rc = alloc(s, s->capacity, &ptr), s->stay = (float*)ptr;
             if (!rc) rc = alloc(s, 8, &ptr), s->view = (float*)ptr;
+        }
+        if (!rc && three) {
+            rc = alloc(s, s->capacity, &ptr), s->home = (int32_t*)ptr;
+            if (!rc) rc = alloc(s, 1, &ptr), s->active = (int32_t*)ptr;
         }
         if (!rc && three) {
             s->bricks = vxr::brick_count3(s->nx, s->ny, s->nz);
@@ -1019,10 +1039,28 @@ int vxr_write(vxr_sim* s, const char* name, const void* words, int64_t n) {
     const bool prefix_ok = std::string(name) == "params";
     if (n != b.len && !(prefix_ok && n >= 0 && n <= b.len))
         return fail(VXR_EINVAL, "buffer '%s' holds %lld words, got %lld", name, (long long)b.len, (long long)n);
+    const bool three = s->kind == VXR_KIND_MPM3;
+    if (three && b.ptr == (void*)s->active) return fail(VXR_EINVAL, "buffer 'active' is read-only");
+    if (three && b.ptr == (void*)s->p[17]) {
+        // the slots must be a permutation of 0 .. count - 1: the level of detail indexes through them
+        const int32_t* v = static_cast<const int32_t*>(words);
+        std::vector<uint8_t> seen((size_t)n, 0);
+        for (int64_t i = 0; i < n; ++i) {
+            if (v[i] < 0 || v[i] >= n || seen[(size_t)v[i]])
+                return fail(VXR_EINVAL, "'slot' must be a permutation of 0..%lld; word %lld is %d", (long long)n - 1,
+                            (long long)i, v[i]);
+            seen[(size_t)v[i]] = 1;
+        }
+    }
     HIP_TRY(hipMemcpyAsync(b.ptr, words, n * 4, hipMemcpyHostToDevice, s->stream));
-    if (s->kind == VXR_KIND_MPM3 && name[0] == 'g' && name[1] != 0)  // a grid field: sweep it all next time
+    if (three && name[0] == 'g' && name[1] != 0)  // a grid field: sweep it all next time
         if (int rc = mark_all_bricks(s)) return rc;
-    if (s->tension && (b.ptr == s->colour[0] || b.ptr == s->colour[1] || name[0] == 't')) {
+    if (three && (b.ptr == (void*)s->p[17] || b.ptr == (void*)s->p[6]))
+        // new slots, or new masses: home from the slots, and P2G and G2P visit every particle
+        if (int rc = vxr::mpm3_rebuild_home(s->launch(), mpm3_args(s))) return rc;
+    bool tension_buffer = false;
+    for (float* f : s->stress) tension_buffer = tension_buffer || (f && b.ptr == (void*)f);
+    if (s->tension && (b.ptr == s->colour[0] || b.ptr == s->colour[1] || tension_buffer)) {
         // tension buffers: unknown content, so the next passes run dense
         if (b.ptr == s->colour[0]) s->spread[0] = -1;
         if (b.ptr == s->colour[1]) s->spread[1] = -1;

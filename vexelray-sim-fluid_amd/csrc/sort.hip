@@ -188,13 +188,13 @@ int permute_fields(const Launch& l, float* const* in, float* const* out, int fie
     if (n <= 0) return 0;
     const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
     timed_begin(l, "permute");
-    if (fields == 17) {
-        FieldPtrs<17> f;
-        for (int k = 0; k < 17; ++k) {
+    if (fields == 18) {
+        FieldPtrs<18> f;
+        for (int k = 0; k < 18; ++k) {
             f.in[k] = in[k];
             f.out[k] = out[k];
         }
-        hipLaunchKernelGGL(permute_kernel<17>, dim3(blocks), dim3(kBlock), 0, l.stream, f, index, n);
+        hipLaunchKernelGGL(permute_kernel<18>, dim3(blocks), dim3(kBlock), 0, l.stream, f, index, n);
     } else if (fields == 10) {
         FieldPtrs<10> f;
         for (int k = 0; k < 10; ++k) {

@@ -177,6 +177,8 @@ class ParticleSimulation3(_Runner):
         live = m[m > 0]
         self._nominal = live.min() if live.size else np.float32(np.finfo(np.float32).max)
         self._write("j", np.ones(n, np.float32))
+        # every slot at its own position again: the level of detail's groups are the slots as loaded
+        self.sim.write("slot", np.arange(n, dtype=np.int32))
         for c in AFFINE3:
             self._write(c, np.zeros(n, np.float32))
         self.params(words)
@@ -192,7 +194,9 @@ class ParticleSimulation3(_Runner):
 
     def refine(self, eye, near, floor) -> None:
         """Moves every group of 8 slots one step toward the level its distance from `eye` asks for
-        (:346-349): lodTarget, then lod. Groups are the slots as loaded, so do not sort as well."""
+        (:346-349): lodTarget, then lod. Groups are the slots as loaded; a sort moves slots, and
+        the passes reach a group's slots wherever the sort put them. The next sort moves the slots
+        left without mass behind every active one, and P2G and G2P then stop at the last active."""
         self._write("view", np.array([eye[0], eye[1], eye[2], near, floor], np.float32))
         self.sim.run(_native.GROUP_LOD)
 
@@ -202,6 +206,10 @@ class ParticleSimulation3(_Runner):
 
     def grid(self):
         return [self._read(g) for g in ("gm", "gmu", "gmv", "gmw")]
+
+    def slots(self):
+        """The slot each particle position holds (the identity until a sort)."""
+        return self.sim.read_words("slot")
 
     def compression(self):
         """J of the particles holding at least half the nominal mass (:312-333)."""
@@ -214,9 +222,6 @@ class ParticleSimulation3(_Runner):
 
     def positions(self):
         return [self._read(v) for v in "xyz"]
-
-    def active(self) -> int:
-        return int(np.count_nonzero(self._read("m") > 0))
 
 
 class ParticleBatch:
