@@ -57,10 +57,12 @@ def kernel_table(sim, names, units, bytes_override=None):
     return out
 
 
-def bench_mpm3(steps, warmup, p2g, g2p, run, tension=False):
+def bench_mpm3(steps, warmup, p2g, g2p, run, tension=False, lod_level=None):
     """The 3D substep on the two-cubes scene; with ``tension`` Flip3Step's tension step (six
     blurs, the stress and the grid with the capillary force), sigma = 0.01 B 8, the scale
-    Tension3Test gives a drop of radius 8 nodes."""
+    Tension3Test gives a drop of radius 8 nodes. With ``lod_level`` every cell's group of 8 is
+    first thinned to that many slots by 60 lod calls (Flip3.lod), so the sorts move the emptied
+    slots behind the rest and P2G and G2P stop at the last active one."""
     from vexhip import particles, scenes
 
     s = scenes.mpm3_two_cubes()
@@ -73,6 +75,9 @@ def bench_mpm3(steps, warmup, p2g, g2p, run, tension=False):
     sim.sim.set_option("run", run)
     z = np.zeros(count, np.float32)
     sim.load((s["x"], s["y"], s["z"]), (z, z, z), s["m"], words)
+    if lod_level is not None:
+        sim.sim.write("target", np.full(count // 8, np.float32(lod_level)))
+        sim.sim.run_pass("lod", 60)
     sim.advance(warmup, sort_every=SORT_EVERY)
     sim.finish()
     sim.sim.timing(True)
@@ -87,6 +92,10 @@ def bench_mpm3(steps, warmup, p2g, g2p, run, tension=False):
     table = kernel_table(sim.sim, names, units)
     sim.sim.timing(False)
     mass = float(np.sum(sim.grid()[0], dtype=np.float64))
+    if lod_level is not None:
+        active = sim.active()
+        sim.close()
+        return active, t1 - t0, table, mass / float(np.sum(s["m"], dtype=np.float64)) - 1.0
     sim.close()
     return count, t1 - t0, table, mass / float(np.sum(s["m"], dtype=np.float64)) - 1.0
 
@@ -308,6 +317,7 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--tension-steps", type=int, default=50)
+    ap.add_argument("--no-lod", action="store_true")
     ap.add_argument("--no-tension", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (counter calibration) and exit")
@@ -357,6 +367,17 @@ def main():
                             "workload": "the same 3D scene stepped as Flip3Step's tension step (clear, P2G, six "
                                         "blurs, stress, grid with tension, G2P), sigma = 0.01*B*8, sort every 10",
                             "kernels": ttable, "grid_mass_drift": tdrift}
+    if not args.no_lod:
+        line["lod3"] = {"workload": "the same 3D scene after 60 Flip3.lod calls thin every cell's 8 slots to "
+                                    "`level`; the sort every 10 substeps moves the emptied slots last and P2G/G2P "
+                                    "stop at the active count", "levels": {}}
+        for level in (4, 1):
+            print(f"[bench] 3D MPM thinned to {level} a cell, {args.tension_steps} steps", file=sys.stderr, flush=True)
+            lcount, lsecs, ltable, _ = bench_mpm3(args.tension_steps, 10, args.p2g, args.g2p, args.run, lod_level=level)
+            line["lod3"]["levels"][str(level)] = {
+                "active_particles": lcount, "ms_per_step": round(lsecs / args.tension_steps * 1e3, 4),
+                "active_particle_substeps_per_s": round(lcount * args.tension_steps / lsecs, 1),
+                "p2g_us": ltable.get("p2g3_seg", {}).get("avg_us"), "g2p_us": ltable.get("g2p3_fast", {}).get("avg_us")}
     if not args.no_hll:
         print(f"[bench] HLL 4096^2, {args.hll_steps} steps", file=sys.stderr, flush=True)
         rate, htable = bench_hll(args.hll_steps)

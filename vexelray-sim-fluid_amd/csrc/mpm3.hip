@@ -332,39 +332,70 @@ __device__ __forceinline__ Box wave_box(int (&lo)[3], int (&hi)[3], int cap) {
 }
 
 constexpr int kWaveTile = 1024;  // nodes of a wave's LDS tile: 1024 x (16 B + 4 B claim) = 20 KB
-constexpr int kMaxPasses = 16;   // slabs a wave's box may take through the tile
+constexpr int kMaxPasses = 16;   // bricks a wave takes through the tile before going direct
 
-// A slab of a wave's box: base cells [slo, shi] of the split axis; its nodes lo .. lo + dims - 1.
+// The tile's nodes: lo .. lo + dims - 1 (a box of base cells and the two nodes past its top).
 struct Slab {
-    int lo[3], dims[3], slo, shi, nodes;  // slo..shi: the split axis's base cells
+    int lo[3], dims[3], nodes;
 };
 
-// v[i] without indexing a local array at run time (that would put it in scratch memory)
-__device__ __forceinline__ int pick(const int (&v)[3], int i) { return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]); }
-
-__device__ __forceinline__ Slab slab(const Box& box, const int (&nodes_of)[3], int split, int width, int pass,
-                                     bool sliced) {
+__device__ __forceinline__ Slab tile_of(const Box& box) {
     Slab sl;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        sl.lo[i] = box.lo[i];
-        sl.dims[i] = nodes_of[i];
-    }
-    sl.slo = pick(box.lo, split);
-    sl.shi = pick(box.hi, split);
-    if (sliced) {
-        sl.slo += pass * width;
-        sl.shi = min(sl.slo + width - 1, sl.shi);
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            if (i == split) {
-                sl.lo[i] = sl.slo;
-                sl.dims[i] = sl.shi - sl.slo + 3;
-            }
-        }
-    }
-    sl.nodes = sl.dims[0] * sl.dims[1] * sl.dims[2];
+    sl.lo[0] = box.lo[0];
+    sl.lo[1] = box.lo[1];
+    sl.lo[2] = box.lo[2];
+    sl.dims[0] = box.ex;
+    sl.dims[1] = box.ey;
+    sl.dims[2] = box.ez;
+    sl.nodes = box.ex * box.ey * box.ez;
     return sl;
+}
+
+// The 8x8x8-cell brick of base cell c, numbered as the sort key numbers bricks.
+__device__ __forceinline__ int brick_of_cell(const int (&c)[3], int nx, int ny) {
+    const int bx = (nx - 2 + 7) / 8, by = (ny - 2 + 7) / 8;
+    return ((c[2] >> 3) * by + (c[1] >> 3)) * bx + (c[0] >> 3);
+}
+
+// The least of v[] above `after` (INT_MAX if none), in this lane; wave_min takes it over the wave.
+template <int R>
+__device__ __forceinline__ int min_of(const int (&v)[R], int after) {
+    int m = INT_MAX;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (v[r] > after) m = min(m, v[r]);
+    return m;
+}
+
+template <int R>
+__device__ __forceinline__ int max_of(const int (&v)[R]) {
+    int m = INT_MIN;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (v[r] != INT_MAX) m = max(m, v[r]);
+    return m;
+}
+
+// v[r] for a run-time r without indexing a local array (which would put it in scratch memory).
+template <int R>
+__device__ __forceinline__ int pick_run(const int (&v)[R], int r) {
+    int out = v[0];
+#pragma unroll
+    for (int k = 1; k < R; ++k)
+        if (r == k) out = v[k];
+    return out;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+    return v;
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+    return v;
 }
 
 __device__ __forceinline__ void clear_tile(float4* tile, const Slab& sl) {
@@ -439,13 +470,20 @@ __device__ __forceinline__ void flush_run(float4* tile, int* claim, const Slab& 
     }
 }
 
+// The bricks a lone particle's stencil reaches, marked by this lane alone.
+__device__ __forceinline__ void mark_one(const Mpm3Args& a, const Axis (&ax)[3]) {
+    const int lo[3] = {ax[0].cell, ax[1].cell, ax[2].cell};
+    const int hi[3] = {lo[0] + 2, lo[1] + 2, lo[2] + 2};
+    mark_bricks(a, lo, hi, 0, 1);
+}
+
 // One particle into this lane's run: its stencil, a flush when its base changes, its amounts.
-__device__ __forceinline__ void take(const Mpm3Args& a, float4* tile, int* claim, const Slab& sl, int split, float x,
-                                     float y, float z, float m, float j, const float (&vel)[3], const float (&c)[9],
-                                     int& run_base, float (&acc)[108]) {
+// `in_pass`: the particle has mass and lies in the brick this pass takes.
+__device__ __forceinline__ void take(const Mpm3Args& a, float4* tile, int* claim, const Slab& sl, bool in_pass,
+                                     float x, float y, float z, float m, float j, const float (&vel)[3],
+                                     const float (&c)[9], int& run_base, float (&acc)[108]) {
     Axis ax[3] = {axis(x, a.nx), axis(y, a.ny), axis(z, a.nz)};
-    const int cs = split == 0 ? ax[0].cell : (split == 1 ? ax[1].cell : ax[2].cell);
-    const bool live = m > 0.0f && cs >= sl.slo && cs <= sl.shi;
+    const bool live = in_pass;
     const int base = ((ax[2].cell - sl.lo[2]) * sl.dims[1] + (ax[1].cell - sl.lo[1])) * sl.dims[0] +
                      (ax[0].cell - sl.lo[0]);
     const bool change = live && base != run_base;
@@ -456,6 +494,18 @@ __device__ __forceinline__ void take(const Mpm3Args& a, float4* tile, int* claim
         const float s = (4.0f * (dt * (m / rho0))) * (bulk * (1.0f - j));
         p2g_fold(ax, m, vel, c, s, acc);
     }
+}
+
+// The tile for the base cells [lo, hi] of the wave's lanes (reduced over the wave): its bricks
+// marked, its nodes cleared.
+__device__ __forceinline__ Slab enter_tile(const Mpm3Args& a, float4* tile, int (&lo)[3], int (&hi)[3]) {
+    const Box box = wave_box(lo, hi, kWaveTile);
+    const int nlo[3] = {box.lo[0], box.lo[1], box.lo[2]};
+    const int nhi[3] = {box.hi[0] + 2, box.hi[1] + 2, box.hi[2] + 2};
+    mark_bricks(a, nlo, nhi, threadIdx.x, kWave);
+    const Slab sl = tile_of(box);
+    clear_tile(tile, sl);
+    return sl;
 }
 
 // The tile's nodes into the grid: one global float atomic per touched node and field.
@@ -494,65 +544,34 @@ __global__ __launch_bounds__(kWave) void p2g3_wave_kernel(Mpm3Args a) {
 #pragma unroll
     for (int k = 0; k < 17; ++k) load_run<R>(a.p[k], first, f[k]);
     const int size[3] = {a.nx, a.ny, a.nz};
-    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    // each particle's brick (the sort's 8x8x8 blocks of base cells); INT_MAX for a massless one
+    int bk[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        if (!(f[6][r] > 0.0f)) continue;
+        int c[3];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            const int c = base_cell(f[i][r], size[i]);
-            lo[i] = min(lo[i], c);
-            hi[i] = max(hi[i], c);
-        }
+        for (int i = 0; i < 3; ++i) c[i] = base_cell(f[i][r], size[i]);
+        bk[r] = f[6][r] > 0.0f ? brick_of_cell(c, a.nx, a.ny) : INT_MAX;
     }
-    const Box box = wave_box(lo, hi, kWaveTile);
-    if (box.empty) return;
-    {
-        const int nlo[3] = {box.lo[0], box.lo[1], box.lo[2]};
-        const int nhi[3] = {box.hi[0] + 2, box.hi[1] + 2, box.hi[2] + 2};
-        mark_bricks(a, nlo, nhi, lane, kWave);
-    }
-    // A box larger than the tile goes through it in slabs across its longest axis, each slab
-    // taking the particles whose base cell lies in it; past kMaxPasses slabs the runs go to
-    // global memory directly.
-    int cells[3], nodes_of[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        cells[i] = box.hi[i] - box.lo[i] + 1;
-        nodes_of[i] = cells[i] + 2;
-    }
-    const int split = cells[0] >= cells[1] && cells[0] >= cells[2] ? 0 : (cells[1] >= cells[2] ? 1 : 2);
-    const int cross = nodes_of[0] * nodes_of[1] * nodes_of[2] / pick(nodes_of, split);
-    const int width = kWaveTile / cross - 2;  // cells of the split axis a slab takes
-    const int passes = width >= 1 ? (pick(cells, split) + width - 1) / width : INT_MAX;
-    const bool use_tile = passes <= kMaxPasses;
-
-    const float dt = bits_f(a.params[DT]), bulk = bits_f(a.params[BULK]), rho0 = bits_f(a.params[RHO0]);
-    if (!use_tile) {
-        // particles spread too thin for the tile (unsorted input): the reference's direct atomics
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const float m = f[6][r];
-            if (!(m > 0.0f)) continue;
-            Axis ax[3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i) ax[i] = axis(f[i][r], size[i]);
-            const float vel[3] = {f[3][r], f[4][r], f[5][r]};
-            float c[9];
-#pragma unroll
-            for (int i = 0; i < 9; ++i) c[i] = f[8 + i][r];
-            const float s = (4.0f * (dt * (m / rho0))) * (bulk * (1.0f - f[7][r]));
-            atomic_particle(a, ax, m, vel, c, s);
-        }
-        return;
-    }
+    const int first_brick = wave_min(min_of(bk, INT_MIN));
+    if (first_brick == INT_MAX) return;
     float acc[108];
 #pragma unroll
     for (int i = 0; i < 108; ++i) acc[i] = 0.0f;
-    if (passes == 1) {
-        // the common case: one slab, the particles from the registers they were loaded into
-        const Slab sl = slab(box, nodes_of, split, width, 0, false);
-        clear_tile(tile, sl);
+    if (wave_max(max_of(bk)) == first_brick) {
+        // the common case: one brick, the particles from the registers they were loaded into
+        int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (bk[r] == INT_MAX) continue;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const int c = base_cell(f[i][r], size[i]);
+                lo[i] = min(lo[i], c);
+                hi[i] = max(hi[i], c);
+            }
+        }
+        const Slab sl = enter_tile(a, tile, lo, hi);
         int run_base = -1;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -560,28 +579,64 @@ __global__ __launch_bounds__(kWave) void p2g3_wave_kernel(Mpm3Args a) {
             float c[9];
 #pragma unroll
             for (int i = 0; i < 9; ++i) c[i] = f[8 + i][r];
-            take(a, tile, claim, sl, split, f[0][r], f[1][r], f[2][r], f[6][r], f[7][r], vel, c, run_base, acc);
+            take(a, tile, claim, sl, bk[r] != INT_MAX, f[0][r], f[1][r], f[2][r], f[6][r], f[7][r], vel, c, run_base,
+                 acc);
         }
         flush_run(tile, claim, sl, run_base, run_base >= 0, acc);
         leave_tile(a, tile, sl);
         return;
     }
-    // several slabs: each pass re-reads its particles (from cache), nothing held across passes
-    for (int pass = 0; pass < passes; ++pass) {
-        const Slab sl = slab(box, nodes_of, split, width, pass, true);
-        clear_tile(tile, sl);
+    // Several bricks (a wave straddling a brick boundary, or unsorted input): one pass through
+    // the tile per brick, lowest first, each re-reading its particles (from cache); the stencils
+    // of one brick's particles cover at most 10 x 10 x 10 nodes, so they always fit. Past
+    // kMaxPasses bricks, the rest go by the reference's direct atomics.
+    int cur = first_brick;
+    for (int pass = 0; pass < kMaxPasses && cur != INT_MAX; ++pass) {
+        int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (bk[r] != cur) continue;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const int c = base_cell(a.p[i][first + r], size[i]);
+                lo[i] = min(lo[i], c);
+                hi[i] = max(hi[i], c);
+            }
+        }
+        const Slab sl = enter_tile(a, tile, lo, hi);
         int run_base = -1;
+#pragma unroll
         for (int r = 0; r < R; ++r) {
             const int64_t q = first + r;
             const float vel[3] = {a.p[3][q], a.p[4][q], a.p[5][q]};
             float c[9];
 #pragma unroll
             for (int i = 0; i < 9; ++i) c[i] = a.p[8 + i][q];
-            take(a, tile, claim, sl, split, a.p[0][q], a.p[1][q], a.p[2][q], a.p[6][q], a.p[7][q], vel, c, run_base,
-                 acc);
+            take(a, tile, claim, sl, bk[r] == cur, a.p[0][q], a.p[1][q], a.p[2][q], a.p[6][q], a.p[7][q], vel, c,
+                 run_base, acc);
         }
         flush_run(tile, claim, sl, run_base, run_base >= 0, acc);
         leave_tile(a, tile, sl);
+        cur = wave_min(min_of(bk, cur));
+    }
+    if (cur == INT_MAX) return;
+    const float dt = bits_f(a.params[DT]), bulk = bits_f(a.params[BULK]), rho0 = bits_f(a.params[RHO0]);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int64_t q = first + r;
+        const int b = bk[r];
+        if (b < cur || b == INT_MAX) continue;
+        const float m = a.p[6][q];
+        Axis ax[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) ax[i] = axis(a.p[i][q], size[i]);
+        const float vel[3] = {a.p[3][q], a.p[4][q], a.p[5][q]};
+        float c[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) c[i] = a.p[8 + i][q];
+        const float sv = (4.0f * (dt * (m / rho0))) * (bulk * (1.0f - a.p[7][q]));
+        mark_one(a, ax);
+        atomic_particle(a, ax, m, vel, c, sv);
     }
 }
 
