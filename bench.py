@@ -29,7 +29,7 @@ BYTES = {
     "g2p3_fast": ("particle", 84), "g2p3_tiled": ("particle", 84), "g2p3": ("particle", 84),
     "grid3": ("node", 28), "mpm3_clear": ("node", 16),
     "keys3": ("particle", 20), "radix_hist": ("element", 4), "radix_scatter": ("element", 16),
-    "radix_scan": ("none", 0), "permute": ("particle", 140),
+    "radix_scan": ("none", 0), "permute": ("particle", 148),
     "hll_step": ("cell", 24),
     # 3D tension, dense over the grid: a blur reads one colour (or gm) and writes one; the stress
     # reads the colour and writes six components; the grid with tension reads four sums, the

@@ -29,8 +29,10 @@ _SHORT = {
     "conv3x3_wgrad_kernel": "conv3x3_wgrad", "wgrad_reduce_kernel": "wgrad_reduce", "hll64_kernel": "hll64",
     "adam_kernel": "adam", "loss_kernel": "loss", "unpack_kernel": "unpack", "pack_kernel": "pack",
     "flip_kernel": "flip", "dam_bumps_kernel": "dam_bumps", "keys2_kernel": "keys2", "g2p2_kernel": "g2p2",
-    "starts_kernel": "starts2", "grid_tension_kernel": "grid3_tension", "blur_kernel": "tension3_blur",
-    "stress_kernel": "tension3_stress",
+    "starts_kernel": "starts2", "grid_tension_rows_kernel": "grid3_tension", "blur_march_kernel": "tension3_blur",
+    "stress_rows_kernel": "tension3_stress", "region_kernel": "tension3_region", "lod_target_kernel": "lod3_target",
+    "lod_kernel": "lod3", "after_sort_kernel": "mpm3_after_sort", "slots_kernel": "mpm3_slots",
+    "fill_zero_kernel": "hll_fill_zero", "f64_to_f32_kernel": "f64_to_f32", "batch2_flags": "batch2_flags",
 }
 
 
