@@ -95,9 +95,13 @@ int vxr_destroy(vxr_sim* sim);
 /* Writes `n` 32-bit words into the named buffer, synchronously, from host memory the call
  * only borrows (PassRunner.write / ResidentBuffer.write). `n` must equal the buffer's
  * length; "params" also takes a shorter prefix. Names: MPM3 x y z u v w m j c00..c22 gm gmu
- * gmv gmw gu gv gw params keys level target stay view; MPM2 x y u v m j c00 c01 c10 c11 gm gmu gmv gu gv params keys
+ * gmv gmw gu gv gw params keys level target stay view slot active (and with VXR_FLAG_TENSION
+ * c0 c1 txx tyy tzz txy txz tyz); MPM2 x y u v m j c00 c01 c10 c11 gm gmu gmv gu gv params keys
  * starts; SWE h hu hv params speed budget clamped (the current slot of each rotating set,
- * as gui/PatchSimulation.java:147-213 resolves it). */
+ * as gui/PatchSimulation.java:147-213 resolves it). MPM3 "slot" (int32) is the slot each
+ * particle position holds -- the identity until a sort permutes it with the particles -- and a
+ * write of it must be a permutation of 0..count-1; "active" (int32, read-only) is the number of
+ * leading particles the last sort found with mass (count when unknown). */
 int vxr_write(vxr_sim* sim, const char* name, const void* words, int64_t n);
 
 /* Reads `n` words of the named buffer after every queued pass has finished (a readback,
@@ -121,8 +125,9 @@ int vxr_advance(vxr_sim* sim, int count, int sort_every, int64_t first_step);
  * step, as test/particle/Flip3Test.java:102 runs step().subList(0, 2)); MPM3 also "lod_target" and
  * "lod" (core/particle/Flip3.java:420-641) over count / 8 groups of 8 consecutive slots, with the
  * buffers "level", "target" (a word a group), "stay" (a word a slot) and "view" (eye x y z, near,
- * floor). The groups are the slots as loaded: a sort regroups particles, so a simulation that
- * thins and fills does not sort (the reference's 3D step has no sort). */
+ * floor). The groups are the slots as loaded; after a sort the passes find a slot's particle
+ * through the slot map, and the sort moves the slots without mass behind every active one, where
+ * P2G and G2P no longer visit them (until a lod pass or a write of "m" may give them mass). */
 int vxr_run_pass(vxr_sim* sim, const char* pass, int repeat);
 
 /* SWE only: queues `count` steps with the reference's buffer rotation -- state k%2 -> (k+1)%2,

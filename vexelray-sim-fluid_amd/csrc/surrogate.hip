@@ -96,15 +96,56 @@ __global__ __launch_bounds__(kBlock) void conv3x3_kernel(const __bf16* __restric
     }
 }
 
-// The same convolution with its operands staged: a block stays resident over tiles of 2 rows x 64
-// pixels; each tile's input with its one-pixel halo (4 x 66 pixels, every channel) goes to LDS
-// once and serves all 9 taps and all waves, while each wave keeps the weights of its output
-// channels in registers for the whole launch. Waves split a tile 2 x 2 (pixels x channels) for
-// 64 output channels, 4 x 1 for 16. Pixel stride in LDS is cin + 8 channels, so the 16 lanes
-// reading 16 consecutive pixels hit distinct banks.
 constexpr int kRows = 2;
 constexpr int kCols = 64;
 
+// 16 bytes of zeros in global memory: the source of the halo slots outside the image and of the
+// pad slot after each pixel's channels, so every lane of a staging load has an address.
+__device__ __align__(16) const unsigned kZero16[4] = {0u, 0u, 0u, 0u};
+
+// The compiler does not see the LDS stores of a global-to-LDS load: LDS reads after the wait
+// must not be moved above it or served from values it assumed.
+__device__ __forceinline__ void lds_fence() { asm volatile("" ::: "memory"); }
+
+// Stages input tile `tt` into `buf` with global-to-LDS loads (global_load_lds_dwordx4): the tile
+// is kTileH x kTileW pixels of kCin / 8 16-byte chunks and one pad chunk, chunk s of the tile at
+// buf + 16 s; each load instruction fills 64 consecutive chunks, lane l the l-th, and the 4 waves
+// take every fourth instruction. Nothing passes through registers, so every load of the tile is
+// in flight at once; the caller waits (vmcnt) and synchronises before reading.
+template <int kCin>
+__device__ __forceinline__ void stage_tile(const __bf16* __restrict__ in, int images, int H, int W, int64_t tt,
+                                           __bf16* buf) {
+    constexpr int kSlots = kCin / 8 + 1;
+    constexpr int kTileW = kCols + 2, kTileH = kRows + 2;
+    constexpr int kChunks = kTileH * kTileW * kSlots;
+    constexpr int kLoads = (kChunks + 63) / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tiles_x = (W + kCols - 1) / kCols, tiles_y = (H + kRows - 1) / kRows;
+    const int n = (int)(tt / ((int64_t)tiles_y * tiles_x));
+    const int rem = (int)(tt % ((int64_t)tiles_y * tiles_x));
+    const int y0 = (rem / tiles_x) * kRows, x0 = (rem % tiles_x) * kCols;
+    const int64_t img = (int64_t)n * H * W;
+    for (int j = wave; j < kLoads; j += kBlock / 64) {
+        const int e = 64 * j + lane;
+        if (e < kChunks) {
+            const int pix = e / kSlots, cg = e % kSlots;
+            const int ty = pix / kTileW, tx = pix % kTileW;
+            const int yy = y0 + ty - 1, xx = x0 + tx - 1;
+            const void* src = kZero16;
+            if (cg < kCin / 8 && yy >= 0 && yy < H && xx >= 0 && xx < W)
+                src = in + (img + (int64_t)yy * W + xx) * kCin + 8 * cg;
+            __builtin_amdgcn_global_load_lds(src, buf + 64 * 8 * j, 16, 0, 0);
+        }
+    }
+}
+
+// The staged implicit-GEMM 3x3 convolution. Resident blocks walk 2 x 64-pixel tiles; each tile's
+// input with its one-pixel halo (4 x 66 pixels, every channel) is staged into LDS by
+// global-to-LDS loads, double-buffered so the next tile's loads are in flight while this one
+// computes, and serves all 9 taps and all waves, while each wave keeps the weights of its output
+// channels in registers for the whole launch. Waves split a tile 2 x 2 (pixels x channels) for
+// 64 output channels, 4 x 1 for 16. Pixel stride in LDS is cin + 8 channels, so the 16 lanes
+// reading 16 consecutive pixels hit distinct banks.
 template <int kCin, int kCoTiles>
 __global__ __launch_bounds__(kBlock, 2) void conv3x3_lds_kernel(const __bf16* __restrict__ in, int images, int H,
                                                                 int W, const __bf16* __restrict__ w,
@@ -119,11 +160,16 @@ __global__ __launch_bounds__(kBlock, 2) void conv3x3_lds_kernel(const __bf16* __
     constexpr int kMT = (kRows * kCols / 16) / kWavesM;  // 16-pixel tiles a wave computes
     constexpr int kK = 9 * kCin;
     constexpr int kSteps = (kK + 31) / 32;
-    __shared__ __align__(16) __bf16 tile[kTileH * kTileW * kStride];
+    constexpr int kTileElems = kTileH * kTileW * kStride;
+    __shared__ __align__(16) __bf16 staged[2 * kTileElems];
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wn = wave % kWavesN, wm = wave / kWavesN;
     const int i = lane & 15, kg = lane >> 4;
+
+    const int tiles_x = (W + kCols - 1) / kCols, tiles_y = (H + kRows - 1) / kRows;
+    const int64_t tiles = (int64_t)images * tiles_y * tiles_x;
+    if ((int64_t)blockIdx.x < tiles) stage_tile<kCin>(in, images, H, W, blockIdx.x, staged);
 
     // this wave's weights: B[k][n] for its kNT channel tiles, every K step
     bf16x8 b[kNT][kSteps];
@@ -137,24 +183,18 @@ __global__ __launch_bounds__(kBlock, 2) void conv3x3_lds_kernel(const __bf16* __
             b[t][s] = tap < 9 ? *(const bf16x8*)(w + ((int64_t)co * 9 + tap) * kCin + ci) : zero8();
         }
     }
-    const int tiles_x = (W + kCols - 1) / kCols, tiles_y = (H + kRows - 1) / kRows;
-    const int64_t tiles = (int64_t)images * tiles_y * tiles_x;
-    for (int64_t tt = blockIdx.x; tt < tiles; tt += gridDim.x) {
+    int cur = 0;
+    for (int64_t tt = blockIdx.x; tt < tiles; tt += gridDim.x, cur ^= 1) {
         const int n = (int)(tt / ((int64_t)tiles_y * tiles_x));
         const int rem = (int)(tt % ((int64_t)tiles_y * tiles_x));
         const int y0 = (rem / tiles_x) * kRows, x0 = (rem % tiles_x) * kCols;
         const int64_t img = (int64_t)n * H * W;
-        // stage the halo tile: 16 bytes a thread at a time
-        constexpr int kChunks = kTileH * kTileW * (kCin / 8);
-        for (int e = threadIdx.x; e < kChunks; e += kBlock) {
-            const int pix = e / (kCin / 8), cg = e % (kCin / 8);
-            const int ty = pix / kTileW, tx = pix % kTileW;
-            const int yy = y0 + ty - 1, xx = x0 + tx - 1;
-            bf16x8 v = zero8();
-            if (yy >= 0 && yy < H && xx >= 0 && xx < W) v = *(const bf16x8*)(in + (img + (int64_t)yy * W + xx) * kCin + 8 * cg);
-            *(bf16x8*)&tile[pix * kStride + 8 * cg] = v;
-        }
+        // this tile's loads have landed in every wave, and every wave is done with the other buffer
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
+        lds_fence();
+        if (tt + gridDim.x < tiles) stage_tile<kCin>(in, images, H, W, tt + gridDim.x, staged + (cur ^ 1) * kTileElems);
+        const __bf16* tile = staged + cur * kTileElems;
         f32x4 acc[kMT][kNT];
 #pragma unroll
         for (int m = 0; m < kMT; ++m)
@@ -201,7 +241,6 @@ __global__ __launch_bounds__(kBlock, 2) void conv3x3_lds_kernel(const __bf16* __
                 }
             }
         }
-        __syncthreads();  // the tile is restaged for the next one
     }
 }
 
@@ -491,6 +530,28 @@ int sfail(const char* msg) {
 
 // The surrogate's kernels as plain entry points over device pointers on a caller's stream. They
 // return 0, 1 (bad argument) or 3 (HIP error); vxs_last_error says which.
+// The staged convolution over resident blocks only (as many as the device holds at once), so
+// each block loads its weights once and walks tiles; the count is read once per instantiation.
+template <int kCin, int kCoTiles>
+static int launch_conv(hipStream_t s, int64_t tiles, const __bf16* x, int images, int H, int W, const __bf16* k,
+                       const float* bias, void* out, int out_ch, const __bf16* mk, int f) {
+    static int resident = 0;
+    if (!resident) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, conv3x3_lds_kernel<kCin, kCoTiles>, kBlock, 0) !=
+                hipSuccess ||
+            cus < 1 || per_cu < 1)
+            return 1;
+        resident = cus * per_cu;
+    }
+    const unsigned g = (unsigned)(tiles < resident ? tiles : resident);
+    hipLaunchKernelGGL((conv3x3_lds_kernel<kCin, kCoTiles>), dim3(g), dim3(kBlock), 0, s, x, images, H, W, k, bias,
+                       out, out_ch, mk, f);
+    return 0;
+}
+
 extern "C" {
 
 const char* vxs_last_error(void) { return g_serr; }
@@ -508,18 +569,19 @@ int vxs_conv3x3(void* stream, const void* in, int images, int H, int W, int cin,
     const __bf16* mk = (const __bf16*)mask;
     if (!(flags & 16)) {  // the staged kernel, resident blocks over 2 x 64 tiles
         const int64_t tiles = (int64_t)images * ((H + kRows - 1) / kRows) * ((W + kCols - 1) / kCols);
-        const unsigned g = (unsigned)(tiles < 2048 ? tiles : 2048);
         const int f = flags & 15;
+        int rc = 1;
         if (cin == 16 && cout == 64)
-            hipLaunchKernelGGL((conv3x3_lds_kernel<16, 4>), dim3(g), dim3(kBlock), 0, s, x, images, H, W, k, bias, out, out_ch, mk, f);
+            rc = launch_conv<16, 4>(s, tiles, x, images, H, W, k, bias, out, out_ch, mk, f);
         else if (cin == 64 && cout == 64)
-            hipLaunchKernelGGL((conv3x3_lds_kernel<64, 4>), dim3(g), dim3(kBlock), 0, s, x, images, H, W, k, bias, out, out_ch, mk, f);
+            rc = launch_conv<64, 4>(s, tiles, x, images, H, W, k, bias, out, out_ch, mk, f);
         else if (cin == 64 && cout == 16)
-            hipLaunchKernelGGL((conv3x3_lds_kernel<64, 1>), dim3(g), dim3(kBlock), 0, s, x, images, H, W, k, bias, out, out_ch, mk, f);
+            rc = launch_conv<64, 1>(s, tiles, x, images, H, W, k, bias, out, out_ch, mk, f);
         else if (cin == 16 && cout == 16)
-            hipLaunchKernelGGL((conv3x3_lds_kernel<16, 1>), dim3(g), dim3(kBlock), 0, s, x, images, H, W, k, bias, out, out_ch, mk, f);
+            rc = launch_conv<16, 1>(s, tiles, x, images, H, W, k, bias, out, out_ch, mk, f);
         else
             return sfail("conv3x3: channels (cin, cout) must be 16 or 64 each");
+        if (rc) return sfail("conv3x3: cannot size the launch");
         return serr("conv3x3");
     }
     const dim3 grid((unsigned)(H * ((W + kTileX - 1) / kTileX)), (unsigned)images);
