@@ -28,11 +28,14 @@ def _conv_ref(x_nhwc, w_ctc, bias, relu):
     return y.permute(0, 2, 3, 1).contiguous()
 
 
+@pytest.mark.parametrize("shape", [(2, 37, 70), (1, 21, 200)], ids=["edges", "interior"])
 @pytest.mark.parametrize("cin,cout,relu,f32", [(16, 64, True, False), (64, 64, True, False), (64, 16, False, True),
                                                (16, 64, False, True)])
-def test_conv_matches_torch(gpu, cin, cout, relu, f32):
+def test_conv_matches_torch(gpu, cin, cout, relu, f32, shape):
+    """Both staged paths: 37 x 70 has every 2 x 64 tile on an image edge (rows not a multiple
+    of the tile either); 21 x 200 has tiles clear of every edge, staged without bounds checks."""
     gen = np.random.default_rng(cin + cout)
-    n, h, w = 2, 37, 70  # a row not a multiple of the 64-pixel tile
+    n, h, w = shape
     x = _bf((n, h, w, cin), gen)
     k = _bf((cout, 9, cin), gen, 0.1)
     b = torch.from_numpy(gen.standard_normal(cout).astype(np.float32))

@@ -11,12 +11,15 @@
 // loading 8 consecutive channels of one tap (16 bytes). Zero padding at the image edges, so the
 // input gradient is the same kernel run with the taps flipped and the channel roles swapped. The
 // weight gradient is a GEMM over pixels, staged through LDS transposed.
+#include <climits>
+
 #include "common.h"
 
 namespace vxr {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kBlock = 256;
@@ -103,39 +106,71 @@ constexpr int kCols = 64;
 // pad slot after each pixel's channels, so every lane of a staging load has an address.
 __device__ __align__(16) const unsigned kZero16[4] = {0u, 0u, 0u, 0u};
 
+// Where the lanes of a store with nothing to store (a pixel past the image edge) write, so that
+// every store instruction of the plain epilogue is issued.
+__device__ __align__(16) __bf16 kSink[64 * 4];
+
+// s_waitcnt with vmcnt = n and the other counters not waited on (gfx9 encoding: vmcnt in bits
+// 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
+template <int n>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(n >= 0 && n < 64, "vmcnt holds 6 bits");
+    __builtin_amdgcn_s_waitcnt((n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
 // The compiler does not see the LDS stores of a global-to-LDS load: LDS reads after the wait
 // must not be moved above it or served from values it assumed.
 __device__ __forceinline__ void lds_fence() { asm volatile("" ::: "memory"); }
 
 // Stages input tile `tt` into `buf` with global-to-LDS loads (global_load_lds_dwordx4): the tile
-// is kTileH x kTileW pixels of kCin / 8 16-byte chunks and one pad chunk, chunk s of the tile at
-// buf + 16 s; each load instruction fills 64 consecutive chunks, lane l the l-th, and the 4 waves
-// take every fourth instruction. Nothing passes through registers, so every load of the tile is
-// in flight at once; the caller waits (vmcnt) and synchronises before reading.
+// is kTileH x kTileW pixels of kCin / 8 16-byte chunks and one pad chunk, chunk e of the tile at
+// buf + 16 e; each load instruction fills 64 consecutive chunks, lane l the l-th, and wave w
+// takes instructions w, w + 4, ... Nothing passes through registers, so every load of the tile
+// is in flight at once; the caller waits (vmcnt) and synchronises before reading. A lane's chunk
+// (pixel row ty, column tx, slot cg) moves by 256 chunks from one of its instructions to the
+// next, kept up to date by additions rather than divided out each time.
 template <int kCin>
-__device__ __forceinline__ void stage_tile(const __bf16* __restrict__ in, int images, int H, int W, int64_t tt,
-                                           __bf16* buf) {
+__device__ __forceinline__ void stage_tile(const __bf16* __restrict__ in, int H, int W, int64_t tt, __bf16* buf) {
     constexpr int kSlots = kCin / 8 + 1;
     constexpr int kTileW = kCols + 2, kTileH = kRows + 2;
     constexpr int kChunks = kTileH * kTileW * kSlots;
     constexpr int kLoads = (kChunks + 63) / 64;
+    constexpr int kPerWave = (kLoads + 3) / 4;
+    constexpr int kStepPix = 256 / kSlots, kStepSlot = 256 % kSlots;  // 256 chunks
+    static_assert(kStepPix + 1 < 2 * kTileW, "two wraps of the column at most");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int tiles_x = (W + kCols - 1) / kCols, tiles_y = (H + kRows - 1) / kRows;
     const int n = (int)(tt / ((int64_t)tiles_y * tiles_x));
     const int rem = (int)(tt % ((int64_t)tiles_y * tiles_x));
     const int y0 = (rem / tiles_x) * kRows, x0 = (rem % tiles_x) * kCols;
-    const int64_t img = (int64_t)n * H * W;
-    for (int j = wave; j < kLoads; j += kBlock / 64) {
-        const int e = 64 * j + lane;
-        if (e < kChunks) {
-            const int pix = e / kSlots, cg = e % kSlots;
-            const int ty = pix / kTileW, tx = pix % kTileW;
-            const int yy = y0 + ty - 1, xx = x0 + tx - 1;
+    const __bf16* base = in + ((int64_t)n * H * W + (int64_t)y0 * W + x0) * kCin;
+    const bool inside = y0 >= 1 && y0 + kRows < H && x0 >= 1 && x0 + kCols < W;
+    const int e0 = 64 * wave + lane;
+    int pix = e0 / kSlots, cg = e0 % kSlots;
+    int ty = pix / kTileW, tx = pix % kTileW;
+#pragma unroll 1
+    for (int jj = 0; jj < kPerWave; ++jj) {
+        const int j = wave + 4 * jj;
+        if (j < kLoads && 64 * j + lane < kChunks) {
             const void* src = kZero16;
-            if (cg < kCin / 8 && yy >= 0 && yy < H && xx >= 0 && xx < W)
-                src = in + (img + (int64_t)yy * W + xx) * kCin + 8 * cg;
+            if (cg < kCin / 8) {
+                const int yy = y0 + ty - 1, xx = x0 + tx - 1;
+                if (inside || (yy >= 0 && yy < H && xx >= 0 && xx < W))
+                    src = base + ((ty - 1) * W + (tx - 1)) * kCin + 8 * cg;
+            }
             __builtin_amdgcn_global_load_lds(src, buf + 64 * 8 * j, 16, 0, 0);
         }
+        // the next instruction of this wave: 256 chunks on
+        cg += kStepSlot;
+        const int carry = cg >= kSlots ? 1 : 0;
+        cg -= carry * kSlots;
+        tx += kStepPix + carry;
+#pragma unroll
+        for (int w = 0; w < 2; ++w)
+            if (tx >= kTileW) {
+                tx -= kTileW;
+                ++ty;
+            }
     }
 }
 
@@ -169,7 +204,7 @@ __global__ __launch_bounds__(kBlock, 2) void conv3x3_lds_kernel(const __bf16* __
 
     const int tiles_x = (W + kCols - 1) / kCols, tiles_y = (H + kRows - 1) / kRows;
     const int64_t tiles = (int64_t)images * tiles_y * tiles_x;
-    if ((int64_t)blockIdx.x < tiles) stage_tile<kCin>(in, images, H, W, blockIdx.x, staged);
+    if ((int64_t)blockIdx.x < tiles) stage_tile<kCin>(in, H, W, blockIdx.x, staged);
 
     // this wave's weights: B[k][n] for its kNT channel tiles, every K step
     bf16x8 b[kNT][kSteps];
@@ -183,17 +218,39 @@ __global__ __launch_bounds__(kBlock, 2) void conv3x3_lds_kernel(const __bf16* __
             b[t][s] = tap < 9 ? *(const bf16x8*)(w + ((int64_t)co * 9 + tap) * kCin + ci) : zero8();
         }
     }
+    // The plain epilogue (bf16 out, no mask, no accumulation, every channel) issues exactly
+    // kNT * kMT stores a tile and nothing else touches memory after a tile's staging loads, so
+    // waiting until at most that many vector memory operations are outstanding waits for the
+    // loads (they complete in issue order) without waiting for the previous tile's stores.
+    const bool plain = !(flags & (kMask | kOutF32 | kAccumulate)) && out_ch == 16 * kCoTiles;
+    // the biases of this lane's output channels, read once (a load in the tile loop would make
+    // the wait for it wait for the next tile's staging loads too)
+    float bias_r[kNT][4];
+#pragma unroll
+    for (int t = 0; t < kNT; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int co = 16 * (wn * kNT + t) + 4 * kg + q;
+            bias_r[t][q] = bias && co < out_ch ? bias[co] : 0.0f;
+        }
+    // the weights and biases are in registers before the tile loop: no wait inside it is for them
+    // (a wait there for a load issued before the loop would also wait for the next tile's staging)
+    __builtin_amdgcn_s_waitcnt(0);
     int cur = 0;
     for (int64_t tt = blockIdx.x; tt < tiles; tt += gridDim.x, cur ^= 1) {
         const int n = (int)(tt / ((int64_t)tiles_y * tiles_x));
         const int rem = (int)(tt % ((int64_t)tiles_y * tiles_x));
         const int y0 = (rem / tiles_x) * kRows, x0 = (rem % tiles_x) * kCols;
         const int64_t img = (int64_t)n * H * W;
-        // this tile's loads have landed in every wave, and every wave is done with the other buffer
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
+        // this tile's loads have landed in every wave, and every wave is done with the other buffer.
+        // A bare barrier: only LDS is shared here (__syncthreads' release fence would also wait
+        // for the previous tile's global stores, which no wave reads).
+        if (plain && tt != blockIdx.x) wait_vmcnt<kNT * kMT>();
+        else __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wave's LDS reads are done
+        __builtin_amdgcn_s_barrier();
         lds_fence();
-        if (tt + gridDim.x < tiles) stage_tile<kCin>(in, images, H, W, tt + gridDim.x, staged + (cur ^ 1) * kTileElems);
+        if (tt + gridDim.x < tiles) stage_tile<kCin>(in, H, W, tt + gridDim.x, staged + (cur ^ 1) * kTileElems);
         const __bf16* tile = staged + cur * kTileElems;
         f32x4 acc[kMT][kNT];
 #pragma unroll
@@ -211,32 +268,93 @@ __global__ __launch_bounds__(kBlock, 2) void conv3x3_lds_kernel(const __bf16* __
                 const int r = mt / (kCols / 16), c = 16 * (mt % (kCols / 16)) + i;
                 const bf16x8 a = tap < 9 ? *(const bf16x8*)&tile[((r + dy) * kTileW + c + dx) * kStride + ci] : zero8();
 #pragma unroll
-                for (int t = 0; t < kNT; ++t) acc[m][t] = mfma(a, b[t][s], acc[m][t]);
+                for (int t = 0; t < kNT; ++t) acc[m][t] = mfma(b[t][s], a, acc[m][t]);
             }
         }
-        // lane holds C[4 kg + q][i] of each (m, t): pixel (row of mt, col 16 (mt % 4) + 4 kg + q), channel
+        // weights as the row operand: lane holds C[4 kg + q][i] of each (m, t) = channel
+        // 16 (wn kNT + t) + 4 kg + q of pixel (row of mt, col 16 (mt % 4) + i), so its 4 values
+        // are 4 consecutive channels of one pixel: one 8-byte (bf16) or 16-byte (FP32) access
+        if (plain) {
+#pragma unroll
+            for (int t = 0; t < kNT; ++t) {
+                const int co = 16 * (wn * kNT + t) + 4 * kg;
+#pragma unroll
+                for (int m = 0; m < kMT; ++m) {
+                    const int mt = wm * kMT + m;
+                    const int yy = y0 + mt / (kCols / 16), xx = x0 + 16 * (mt % (kCols / 16)) + i;
+                    bf16x4 r;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        float v = acc[m][t][q] + bias_r[t][q];
+                        if (flags & kRelu) v = v > 0.0f ? v : 0.0f;
+                        r[q] = (__bf16)v;
+                    }
+                    __bf16* o = yy < H && xx < W ? (__bf16*)out + (img + (int64_t)yy * W + xx) * out_ch + co
+                                                 : kSink + 4 * lane;
+                    *(bf16x4*)o = r;
+                }
+            }
+            continue;
+        }
 #pragma unroll
         for (int t = 0; t < kNT; ++t) {
-            const int co = 16 * (wn * kNT + t) + i;
+            const int co = 16 * (wn * kNT + t) + 4 * kg;
             if (co >= out_ch) continue;
-            const float bv = bias ? bias[co] : 0.0f;
+            const float* bv = bias_r[t];
 #pragma unroll
             for (int m = 0; m < kMT; ++m) {
                 const int mt = wm * kMT + m;
-                const int yy = y0 + mt / (kCols / 16);
+                const int yy = y0 + mt / (kCols / 16), xx = x0 + 16 * (mt % (kCols / 16)) + i;
+                if (yy >= H || xx >= W) continue;
+                const int64_t at = (img + (int64_t)yy * W + xx) * out_ch + co;
+                float v[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int xx = x0 + 16 * (mt % (kCols / 16)) + 4 * kg + q;
-                    if (yy >= H || xx >= W) continue;
-                    const int64_t at = (img + (int64_t)yy * W + xx) * out_ch + co;
-                    float v = acc[m][t][q] + bv;
-                    if ((flags & kMask) && !((float)mask[at] > 0.0f)) v = 0.0f;
-                    if (flags & kRelu) v = v > 0.0f ? v : 0.0f;
-                    if (flags & kOutF32) {
-                        float* o = (float*)out;
-                        o[at] = (flags & kAccumulate) ? o[at] + v : v;
+                for (int q = 0; q < 4; ++q) v[q] = acc[m][t][q] + bv[q];
+                const bool whole = (out_ch & 3) == 0;  // 4 channels in range and the access aligned
+                if (flags & kMask) {
+                    if (whole) {
+                        const bf16x4 mk = *(const bf16x4*)(mask + at);
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            if (!((float)mk[q] > 0.0f)) v[q] = 0.0f;
                     } else {
-                        ((__bf16*)out)[at] = (__bf16)v;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            if (co + q < out_ch && !((float)mask[at + q] > 0.0f)) v[q] = 0.0f;
+                    }
+                }
+                if (flags & kRelu) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) v[q] = v[q] > 0.0f ? v[q] : 0.0f;
+                }
+                if (flags & kOutF32) {
+                    float* o = (float*)out + at;
+                    if (whole) {
+                        float4 r = make_float4(v[0], v[1], v[2], v[3]);
+                        if (flags & kAccumulate) {
+                            const float4 p = *(const float4*)o;
+                            r.x += p.x;
+                            r.y += p.y;
+                            r.z += p.z;
+                            r.w += p.w;
+                        }
+                        *(float4*)o = r;
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            if (co + q < out_ch) o[q] = (flags & kAccumulate) ? o[q] + v[q] : v[q];
+                    }
+                } else {
+                    __bf16* o = (__bf16*)out + at;
+                    if (whole) {
+                        bf16x4 r;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) r[q] = (__bf16)v[q];
+                        *(bf16x4*)o = r;
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            if (co + q < out_ch) o[q] = (__bf16)v[q];
                     }
                 }
             }
