@@ -7,8 +7,9 @@
 //
 // Activations are NHWC bf16; weights bf16 [cout][tap][cin] with an FP32 master copy; every
 // product accumulates in FP32 on the matrix cores (v_mfma_f32_16x16x32_bf16). A 3x3 convolution
-// is an implicit GEMM: M = pixels, N = output channels, K = 9 taps x input channels, each lane
-// loading 8 consecutive channels of one tap (16 bytes). Zero padding at the image edges, so the
+// is an implicit GEMM over K = 9 taps x input channels, each lane loading 8 consecutive channels
+// of one tap (16 bytes); the staged kernel takes the weights as the row operand (output channels)
+// and the pixels as the column operand, the unstaged one the other way round. Zero padding at the image edges, so the
 // input gradient is the same kernel run with the taps flipped and the channel roles swapped. The
 // weight gradient is a GEMM over pixels, staged through LDS transposed.
 #include <climits>
