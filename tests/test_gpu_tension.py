@@ -278,3 +278,31 @@ def test_a_drop_on_the_floor_comes_to_rest(gpu):
           f"{steps} steps", flush=True)
     assert abs(rise) < 5, rise
     assert energy < 120, energy
+
+
+def test_tension_steps_replayed_as_graphs(gpu):
+    """vxr_advance with graph replay (option graph = 1) on a tension simulation: the same steps as
+    eager advancing, within P2G's float-atomic reassociation; the first segment after the load
+    runs eagerly (the region is rebuilt), the rest replay captured segments."""
+    n = 32
+    f = _blob((n, n, n), (10, 11, 9), (20, 21, 19), 8, 44)
+    count = len(f["x"])
+    words = particles.params3(5e-4, 0, -100.0, 0, 2000.0, 1.0, 200.0)
+    runs = []
+    for graph in (0, 1):
+        sim = particles.ParticleSimulation3(n, n, n, count, tension=True)
+        zero = np.zeros(count, np.float32)
+        sim.load((f["x"], f["y"], f["z"]), (zero, zero, zero), f["m"], words)
+        sim.sim.set_option("graph", graph)
+        sim.advance(25, sort_every=10)
+        sim.advance(15, sort_every=10)
+        slots = sim.slots()
+        pos = []
+        for v in sim.positions():
+            out = np.empty_like(v)
+            out[slots] = v
+            pos.append(out)
+        runs.append(pos)
+        sim.close()
+    for a, b in zip(*runs):
+        np.testing.assert_allclose(b, a, rtol=1e-5, atol=2e-5)

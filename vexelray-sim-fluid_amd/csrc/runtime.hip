@@ -505,6 +505,10 @@ int run_segment(vxr_sim* s, bool sort, int n) {
 // no memset node is recorded.
 int replay_segment(vxr_sim* s, bool sort, int n) {
     if (!opt(s, "graph") || s->timer) return run_segment(s, sort, n);
+    // The tension passes' region bookkeeping is host state, read while a segment is recorded: a
+    // segment recorded while it is stale would replay whole-grid passes for good. Run eagerly
+    // until a step has rebuilt the region.
+    if (s->tension && s->stale) return run_segment(s, sort, n);
     const auto key = std::make_tuple((int)sort, n, (int)s->flipped);
     auto it = s->graphs.find(key);
     if (it == s->graphs.end()) {
