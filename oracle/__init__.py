@@ -59,6 +59,7 @@ def _load() -> ctypes.CDLL:
         "orc_mpm3_advect": (None, [_c_i, _c_i, _c_i, _c_i64, _vp, _i32p, _f32p, _f32p, _f32p]),
         "orc_mpm3_step": (None, [_c_i, _c_i, _c_i, _c_i64, _vp, _i32p, _vp]),
         "orc_mpm3_key_bits": (_c_i, [_c_i, _c_i, _c_i]),
+        "orc_mpm3_step_omp": (None, [_c_i, _c_i, _c_i, _c_i64, _vp, _i32p, _vp, _c_i]),
         "orc_mpm3_keys": (None, [_c_i, _c_i, _c_i, _c_i64, _f32p, _f32p, _f32p, _f32p, _i32p]),
         "orc_mpm3_inactive_key": (ctypes.c_int32, [_c_i, _c_i, _c_i]),
         "orc_mpm2_clear": (None, [_c_i, _c_i, _f32p, _f32p, _f32p]),
@@ -342,6 +343,13 @@ class Mpm3:
         grid = _ptrs([self.g[f] for f in GRID3])
         for _ in range(n):
             lib.orc_mpm3_step(*self.n, self.count, self._pp(), self.params, grid)
+
+    def step_omp(self, n=1, threads=4):
+        """Plain steps on `threads` host threads (the CPU baseline's step; its P2G sums in another
+        order than step's)."""
+        grid = _ptrs([self.g[f] for f in GRID3])
+        for _ in range(n):
+            lib.orc_mpm3_step_omp(*self.n, self.count, self._pp(), self.params, grid, int(threads))
 
     def keys(self):
         out = np.empty(self.count, np.int32)

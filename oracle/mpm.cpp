@@ -4,7 +4,10 @@
 // the reference's (Scatter.java:48-53), and this one is fixed. Test infrastructure only.
 #include "oracle.h"
 
+#include <omp.h>
+
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -60,6 +63,89 @@ inline void keep_inside(float* at, float* vel, int size) {
 enum { DT3 = 0, GX3 = 1, GY3 = 2, GZ3 = 3, BULK3 = 4, RHO03 = 5 };     // Flip3.java:58-64
 enum { DT2 = 0, GX2 = 1, GY2 = 2, BULK2 = 3, RHO02 = 4 };              // Flip.java:102-106
 
+// One particle's 108 P2G amounts (Flip3.java:217-243), handed to add(node, field, amount) in the
+// reference's order: node = base_of(base cell) + dz plane + dy row + dx in the caller's layout.
+// Nothing for a particle without mass.
+template <typename Base, typename Add>
+inline void scatter3_one(int nx, int ny, int nz, float* const p[17], int64_t q, float dt, float bulk, float rho0,
+                         Base base_of, int64_t row, int64_t plane, Add add) {
+    const float m = p[6][q];
+    if (!(m > 0.0f)) return;
+    const int size[3] = {nx, ny, nz};
+    int32_t cell[3];
+    float off[3], w[3][3];
+    for (int a = 0; a < 3; a++) axis(p[a][q], size[a], &cell[a], &off[a], w[a]);
+    const float vel[3] = {p[3][q], p[4][q], p[5][q]};
+    float c[9];
+    for (int a = 0; a < 9; a++) c[a] = p[8 + a][q];
+    const float s = (4.0f * (dt * (m / rho0))) * (bulk * (1.0f - p[7][q]));
+    const int64_t base = base_of(cell);
+    for (int k = 0; k < 27; k++) {
+        const float wgt = w[0][k % 3] * (w[1][(k / 3) % 3] * w[2][k / 9]);
+        const float d[3] = {(float)(k % 3) - off[0], (float)((k / 3) % 3) - off[1], (float)(k / 9) - off[2]};
+        const float wm = wgt * m;
+        const int64_t node = base + (k / 9) * plane + ((k / 3) % 3) * row + k % 3;
+        add(node, 0, wm);
+        for (int a = 0; a < 3; a++) {
+            const float at3 = vel[a] + (c[3 * a] * d[0] + (c[3 * a + 1] * d[1] + c[3 * a + 2] * d[2]));
+            const float push = s * d[a];
+            add(node, 1 + a, wm * at3 + wgt * push);
+        }
+    }
+}
+
+// Flip3.grid at one node (Flip3.java:267-311, the plain branch).
+inline void grid3_node(int nx, int ny, int nz, int64_t node, float dt, const float (&g)[3], const float* gm,
+                       const float* const mom[3], float* const out[3]) {
+    const int size[3] = {nx, ny, nz};
+    const int64_t at[3] = {node % nx, (node / nx) % ny, node / ((int64_t)nx * ny)};
+    const float mass = gm[node];
+    float vel[3] = {0.0f, 0.0f, 0.0f};
+    if (mass > kEmpty) {
+        for (int a = 0; a < 3; a++) vel[a] = mom[a][node] / mass + dt * g[a];
+    }
+    for (int a = 0; a < 3; a++) {
+        if (!(at[a] > 1)) vel[a] = mx(vel[a], 0.0f);
+        if (at[a] > size[a] - 3) vel[a] = mn(vel[a], 0.0f);
+        out[a][node] = vel[a];
+    }
+}
+
+// Flip3.advect for one particle (Flip3.java:329-383, the plain branch).
+inline void advect3_one(int nx, int ny, int nz, float* const p[17], int64_t q, float dt, const float* const vel_in[3]) {
+    if (!(p[6][q] > 0.0f)) return;
+    // local copies: the particle stores below cannot be taken to change them
+    const float* const grid[3] = {vel_in[0], vel_in[1], vel_in[2]};
+    const int size[3] = {nx, ny, nz};
+    int32_t cell[3];
+    float off[3], w[3][3];
+    for (int a = 0; a < 3; a++) axis(p[a][q], size[a], &cell[a], &off[a], w[a]);
+    const int64_t base = ((int64_t)cell[2] * ny + cell[1]) * nx + cell[0];
+    float vel[3] = {0.0f, 0.0f, 0.0f};
+    float sum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 27; k++) {
+        const float wgt = w[0][k % 3] * (w[1][(k / 3) % 3] * w[2][k / 9]);
+        const float d[3] = {(float)(k % 3) - off[0], (float)((k / 3) % 3) - off[1], (float)(k / 9) - off[2]};
+        const int64_t node = base + (int64_t)(k / 9) * nx * ny + ((k / 3) % 3) * nx + k % 3;
+        for (int a = 0; a < 3; a++) {
+            const float na = wgt * grid[a][node];
+            vel[a] = vel[a] + na;
+            for (int e = 0; e < 3; e++) sum[3 * a + e] = sum[3 * a + e] + na * d[e];
+        }
+    }
+    float c[9];
+    for (int a = 0; a < 9; a++) c[a] = 4.0f * sum[a];
+    for (int a = 0; a < 3; a++) {
+        float pos = p[a][q] + dt * vel[a];
+        keep_inside(&pos, &vel[a], size[a]);
+        p[a][q] = pos;
+        p[3 + a][q] = vel[a];
+    }
+    for (int a = 0; a < 9; a++) p[8 + a][q] = c[a];
+    const float j = p[7][q] * (1.0f + dt * (c[0] + (c[4] + c[8])));
+    p[7][q] = clampf(j, 1.0f / kJBound, kJBound);
+}
+
 }  // namespace
 
 extern "C" {
@@ -81,31 +167,10 @@ void orc_mpm3_scatter(int nx, int ny, int nz, int64_t count, float* const p[17],
     const float dt = bits_to_float(params[DT3]);
     const float bulk = bits_to_float(params[BULK3]);
     const float rho0 = bits_to_float(params[RHO03]);
-    const int size[3] = {nx, ny, nz};
-    for (int64_t q = 0; q < count; q++) {
-        const float m = p[6][q];
-        if (!(m > 0.0f)) continue;
-        int32_t cell[3];
-        float off[3], w[3][3];
-        for (int a = 0; a < 3; a++) axis(p[a][q], size[a], &cell[a], &off[a], w[a]);
-        const int64_t base = ((int64_t)cell[2] * ny + cell[1]) * nx + cell[0];
-        const float vel[3] = {p[3][q], p[4][q], p[5][q]};
-        float c[9];
-        for (int a = 0; a < 9; a++) c[a] = p[8 + a][q];
-        const float s = (4.0f * (dt * (m / rho0))) * (bulk * (1.0f - p[7][q]));
-        for (int k = 0; k < 27; k++) {
-            const float wgt = w[0][k % 3] * (w[1][(k / 3) % 3] * w[2][k / 9]);
-            const float d[3] = {(float)(k % 3) - off[0], (float)((k / 3) % 3) - off[1], (float)(k / 9) - off[2]};
-            const float wm = wgt * m;
-            const int64_t node = base + (int64_t)(k / 9) * nx * ny + ((k / 3) % 3) * nx + k % 3;
-            grid[0][node] += wm;
-            for (int a = 0; a < 3; a++) {
-                const float at3 = vel[a] + (c[3 * a] * d[0] + (c[3 * a + 1] * d[1] + c[3 * a + 2] * d[2]));
-                const float push = s * d[a];
-                grid[1 + a][node] += wm * at3 + wgt * push;
-            }
-        }
-    }
+    const auto base_of = [&](const int32_t (&cell)[3]) { return ((int64_t)cell[2] * ny + cell[1]) * nx + cell[0]; };
+    for (int64_t q = 0; q < count; q++)
+        scatter3_one(nx, ny, nz, p, q, dt, bulk, rho0, base_of, nx, (int64_t)nx * ny,
+                     [&](int64_t node, int f, float v) { grid[f][node] += v; });
 }
 
 // Flip3.grid, Flip3.java:267-311 (the plain branch)
@@ -113,61 +178,18 @@ void orc_mpm3_grid(int nx, int ny, int nz, const int32_t params[7], const float*
                    const float* gmv, const float* gmw, float* gu, float* gv, float* gw) {
     const float* mom[3] = {gmu, gmv, gmw};
     float* out[3] = {gu, gv, gw};
-    const int size[3] = {nx, ny, nz};
     const float dt = bits_to_float(params[DT3]);
     const float g[3] = {bits_to_float(params[GX3]), bits_to_float(params[GY3]), bits_to_float(params[GZ3])};
     const int64_t nodes = (int64_t)nx * ny * nz;
-    for (int64_t node = 0; node < nodes; node++) {
-        const int64_t at[3] = {node % nx, (node / nx) % ny, node / ((int64_t)nx * ny)};
-        const float mass = gm[node];
-        float vel[3] = {0.0f, 0.0f, 0.0f};
-        if (mass > kEmpty) {
-            for (int a = 0; a < 3; a++) vel[a] = mom[a][node] / mass + dt * g[a];
-        }
-        for (int a = 0; a < 3; a++) {
-            if (!(at[a] > 1)) vel[a] = mx(vel[a], 0.0f);
-            if (at[a] > size[a] - 3) vel[a] = mn(vel[a], 0.0f);
-            out[a][node] = vel[a];
-        }
-    }
+    for (int64_t node = 0; node < nodes; node++) grid3_node(nx, ny, nz, node, dt, g, gm, mom, out);
 }
 
 // Flip3.advect, Flip3.java:329-383 (the plain branch)
 void orc_mpm3_advect(int nx, int ny, int nz, int64_t count, float* const p[17], const int32_t params[7],
                      const float* gu, const float* gv, const float* gw) {
     const float* grid[3] = {gu, gv, gw};
-    const int size[3] = {nx, ny, nz};
     const float dt = bits_to_float(params[DT3]);
-    for (int64_t q = 0; q < count; q++) {
-        if (!(p[6][q] > 0.0f)) continue;
-        int32_t cell[3];
-        float off[3], w[3][3];
-        for (int a = 0; a < 3; a++) axis(p[a][q], size[a], &cell[a], &off[a], w[a]);
-        const int64_t base = ((int64_t)cell[2] * ny + cell[1]) * nx + cell[0];
-        float vel[3] = {0.0f, 0.0f, 0.0f};
-        float sum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int k = 0; k < 27; k++) {
-            const float wgt = w[0][k % 3] * (w[1][(k / 3) % 3] * w[2][k / 9]);
-            const float d[3] = {(float)(k % 3) - off[0], (float)((k / 3) % 3) - off[1], (float)(k / 9) - off[2]};
-            const int64_t node = base + (int64_t)(k / 9) * nx * ny + ((k / 3) % 3) * nx + k % 3;
-            for (int a = 0; a < 3; a++) {
-                const float na = wgt * grid[a][node];
-                vel[a] = vel[a] + na;
-                for (int e = 0; e < 3; e++) sum[3 * a + e] = sum[3 * a + e] + na * d[e];
-            }
-        }
-        float c[9];
-        for (int a = 0; a < 9; a++) c[a] = 4.0f * sum[a];
-        for (int a = 0; a < 3; a++) {
-            float pos = p[a][q] + dt * vel[a];
-            keep_inside(&pos, &vel[a], size[a]);
-            p[a][q] = pos;
-            p[3 + a][q] = vel[a];
-        }
-        for (int a = 0; a < 9; a++) p[8 + a][q] = c[a];
-        const float j = p[7][q] * (1.0f + dt * (c[0] + (c[4] + c[8])));
-        p[7][q] = clampf(j, 1.0f / kJBound, kJBound);
-    }
+    for (int64_t q = 0; q < count; q++) advect3_one(nx, ny, nz, p, q, dt, grid);
 }
 
 void orc_mpm3_step(int nx, int ny, int nz, int64_t count, float* const p[17], const int32_t params[7],
@@ -176,6 +198,63 @@ void orc_mpm3_step(int nx, int ny, int nz, int64_t count, float* const p[17], co
     orc_mpm3_scatter(nx, ny, nz, count, p, params, grid[0], grid[1], grid[2], grid[3]);
     orc_mpm3_grid(nx, ny, nz, params, grid[0], grid[1], grid[2], grid[3], grid[4], grid[5], grid[6]);
     orc_mpm3_advect(nx, ny, nz, count, p, params, grid[4], grid[5], grid[6]);
+}
+
+// The same substep on `threads` host threads, for the CPU baseline only (not a restatement to
+// check against: its P2G sums in another order). Clear, grid update and G2P split the nodes and
+// particles evenly; P2G gives each thread a contiguous run of particles, scattered into a private
+// box of nodes around them and then added into the grid one thread at a time.
+void orc_mpm3_step_omp(int nx, int ny, int nz, int64_t count, float* const p[17], const int32_t params[7],
+                       float* const grid[7], int threads) {
+    const int64_t nodes = (int64_t)nx * ny * nz;
+    const float dt = bits_to_float(params[DT3]);
+    const float bulk = bits_to_float(params[BULK3]);
+    const float rho0 = bits_to_float(params[RHO03]);
+    const float g[3] = {bits_to_float(params[GX3]), bits_to_float(params[GY3]), bits_to_float(params[GZ3])};
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (int64_t i = 0; i < nodes; i++) grid[0][i] = grid[1][i] = grid[2][i] = grid[3][i] = 0.0f;
+#pragma omp parallel num_threads(threads)
+    {
+        const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+        const int64_t q0 = count * t / nt, q1 = count * (t + 1) / nt;
+        const int size[3] = {nx, ny, nz};
+        int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+        for (int64_t q = q0; q < q1; q++) {
+            if (!(p[6][q] > 0.0f)) continue;
+            for (int a = 0; a < 3; a++) {
+                int32_t cell;
+                float off, w[3];
+                axis(p[a][q], size[a], &cell, &off, w);
+                lo[a] = std::min(lo[a], (int)cell);
+                hi[a] = std::max(hi[a], (int)cell);
+            }
+        }
+        if (lo[0] <= hi[0]) {
+            const int ex = hi[0] - lo[0] + 3, ey = hi[1] - lo[1] + 3, ez = hi[2] - lo[2] + 3;
+            std::vector<float> box((size_t)ex * ey * ez * 4, 0.0f);
+            const auto base_of = [&](const int32_t (&cell)[3]) {
+                return ((int64_t)(cell[2] - lo[2]) * ey + cell[1] - lo[1]) * ex + cell[0] - lo[0];
+            };
+            for (int64_t q = q0; q < q1; q++)
+                scatter3_one(nx, ny, nz, p, q, dt, bulk, rho0, base_of, ex, (int64_t)ex * ey,
+                             [&](int64_t at, int f, float v) { box[4 * at + f] += v; });
+#pragma omp critical
+            for (int z = 0; z < ez; z++)
+                for (int y = 0; y < ey; y++)
+                    for (int x = 0; x < ex; x++) {
+                        const int64_t at = ((int64_t)z * ey + y) * ex + x;
+                        const int64_t node = ((int64_t)(lo[2] + z) * ny + lo[1] + y) * nx + lo[0] + x;
+                        for (int f = 0; f < 4; f++) grid[f][node] += box[4 * at + f];
+                    }
+        }
+    }
+    const float* mom[3] = {grid[1], grid[2], grid[3]};
+    float* out[3] = {grid[4], grid[5], grid[6]};
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (int64_t node = 0; node < nodes; node++) grid3_node(nx, ny, nz, node, dt, g, grid[0], mom, out);
+    const float* vel[3] = {grid[4], grid[5], grid[6]};
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (int64_t q = 0; q < count; q++) advect3_one(nx, ny, nz, p, q, dt, vel);
 }
 
 // The 3D sort key: Stencil3's base cell, numbered brick-major (8x8x8-cell bricks, bricks

@@ -77,6 +77,10 @@ void orc_mpm3_advect(int nx, int ny, int nz, int64_t count, float* const p[17],
 /* One whole step, as core/particle/Flip3Step.java:81-118 lists it. */
 void orc_mpm3_step(int nx, int ny, int nz, int64_t count, float* const p[17],
                    const int32_t params[7], float* const grid[7]);
+/* The same substep on `threads` host threads (OpenMP), for the CPU baseline only: its P2G adds
+ * per-thread boxes of sums into the grid, so its sums are reassociated. */
+void orc_mpm3_step_omp(int nx, int ny, int nz, int64_t count, float* const p[17],
+                       const int32_t params[7], float* const grid[7], int threads);
 /* The 3D sort key: the stencil base cell (Flip3.Stencil3.of) renumbered brick-major,
  * 8x8x8-cell bricks; a slot with m <= 0 takes orc_mpm3_inactive_key, one past every cell's key.
  * orc_mpm3_key_bits returns the number of key bits. The reference's 3D step has no sort: this
