@@ -823,13 +823,19 @@ __global__ __launch_bounds__(kBlock) void g2p3_kernel(Mpm3Args a) {
     });
 }
 
-constexpr int kG2PTile = 2560;  // nodes of the f32 LDS tile: 3 x 2560 x 4 B = 30 KB
+// The G2P tile: the node velocities u, v, w in three planes at fixed strides, so the 81 stencil
+// reads of a particle are one address and constant offsets (the immediates of ds_read_b32). It
+// holds boxes of up to 20 x 12 x 10 nodes: a block's 128 cells of sorted particles, also where
+// the block straddles two neighbouring bricks; a larger box (unsorted or thinned particles) reads
+// the grid.
+constexpr int kG2PX = 20, kG2PY = 12, kG2PZ = 10;
+constexpr int kG2PNodes = kG2PX * kG2PY * kG2PZ;  // 3 x 2400 x 4 B = 28.8 KB
 
 // A block takes a chunk of kBlock * R particles, lane-consecutive, with the chunk's node
 // velocities staged in LDS; every position is loaded once, up front, for the box and the move.
 template <int R, bool kFast>
 __global__ __launch_bounds__(kBlock) void g2p3_tiled_kernel(Mpm3Args a) {
-    __shared__ float tile[3 * kG2PTile];
+    __shared__ float tile[3 * kG2PNodes];
     __shared__ int red[6 * (kBlock / kWave)];
     const int64_t chunk = (int64_t)blockIdx.x * kBlock * R;
     if (chunk >= *a.active) return;  // a block of inactive slots
@@ -851,16 +857,18 @@ __global__ __launch_bounds__(kBlock) void g2p3_tiled_kernel(Mpm3Args a) {
             hi[i] = max(hi[i], c);
         }
     }
-    const Box box = reduce_box(lo, hi, red, kG2PTile);
+    const Box box = reduce_box(lo, hi, red, INT_MAX);
     if (box.empty) return;
-    const bool use_tile = box.nodes <= kG2PTile;
+    const bool use_tile = box.ex <= kG2PX && box.ey <= kG2PY && box.ez <= kG2PZ;
     if (use_tile) {
-        for (int t = threadIdx.x; t < box.nodes; t += kBlock) {
+        const int nodes = box.ex * box.ey * box.ez;
+        for (int t = threadIdx.x; t < nodes; t += kBlock) {
             const int lx = t % box.ex, ly = (t / box.ex) % box.ey, lz = t / (box.ex * box.ey);
             const int64_t node = ((int64_t)(box.lo[2] + lz) * a.ny + (box.lo[1] + ly)) * a.nx + (box.lo[0] + lx);
-            tile[t] = a.g[4][node];
-            tile[box.nodes + t] = a.g[5][node];
-            tile[2 * box.nodes + t] = a.g[6][node];
+            const int at = (lz * kG2PY + ly) * kG2PX + lx;
+            tile[at] = a.g[4][node];
+            tile[kG2PNodes + at] = a.g[5][node];
+            tile[2 * kG2PNodes + at] = a.g[6][node];
         }
     }
     __syncthreads();
@@ -872,11 +880,9 @@ __global__ __launch_bounds__(kBlock) void g2p3_tiled_kernel(Mpm3Args a) {
         const int64_t q = chunk + (int64_t)r * kBlock + threadIdx.x;
         Axis ax[3] = {axis(pos[r][0], a.nx), axis(pos[r][1], a.ny), axis(pos[r][2], a.nz)};
         if (use_tile) {
-            const int base = ((ax[2].cell - box.lo[2]) * box.ey + (ax[1].cell - box.lo[1])) * box.ex +
-                             (ax[0].cell - box.lo[0]);
-            auto at = [&](int k, int i) {
-                return tile[i * box.nodes + base + ((k / 9) * box.ey + (k / 3) % 3) * box.ex + k % 3];
-            };
+            const float* base = tile + ((ax[2].cell - box.lo[2]) * kG2PY + (ax[1].cell - box.lo[1])) * kG2PX +
+                                (ax[0].cell - box.lo[0]);
+            auto at = [&](int k, int i) { return base[i * kG2PNodes + ((k / 9) * kG2PY + (k / 3) % 3) * kG2PX + k % 3]; };
             if (kFast) g2p_fast(a, q, pos[r], ax, dt, at);
             else g2p_one(a, q, pos[r], ax, dt, at);
         } else {
