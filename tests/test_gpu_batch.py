@@ -71,6 +71,10 @@ def test_fixed_p2g_bit_exact_with_oracle(gpu, run):
     ref.steps = 1
     ref.advance(30, 10)
     _assert_fields(dev.read, ref.p, "30 steps")
+    # the whole grid of the last step: the chunks clear and grid skipped hold what the oracle's
+    # full passes give them
+    for g in ("gm", "gmu", "gmv", "gu", "gv"):
+        np.testing.assert_array_equal(dev.read(g).view(np.int32), ref.g[g].view(np.int32), err_msg=f"30 steps: {g}")
     dev.run(_native.GROUP_SORT)
     ref.sort()
     _assert_fields(dev.read, ref.p, "sorted")

@@ -156,6 +156,11 @@ struct Batch2Args {
     unsigned long long* q[3];  // fixed-point sums: mass (2^-40), momentum u, v (2^-32)
     const int32_t* params;
     int cell_bits;
+    // Per 256-node chunk: P2G sets `touched` for every chunk a row of its tile box reaches; the
+    // grid pass moves it to `live` (chunks whose sums and velocities may be nonzero) and clears
+    // it. Clear and grid visit only chunks flagged in either.
+    uint8_t* touched;
+    uint8_t* live;
 };
 // `blocks` > 0 caps the grid (each block then strides over chunks); 0 is a block per chunk.
 int batch2_clear(const Launch& l, const Batch2Args& a, int blocks);

@@ -333,6 +333,13 @@ __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a, int ru
         const int bx0 = box[0], by0 = box[1];
         const int w = box[2] - bx0 + 3, h = box[3] - by0 + 3;
         const bool tiled = box[2] >= 0 && w * h <= kTile2;
+        if (box[2] >= 0) {
+            // the chunks each node row of the box reaches (the box lies inside the scene's grid)
+            for (int r = threadIdx.x; r < h; r += kBlock) {
+                const int64_t row0 = sc.goff + (int64_t)(by0 + r) * sc.nx + bx0;
+                for (int64_t ch = row0 / kBlock; ch <= (row0 + w - 1) / kBlock; ++ch) a.touched[ch] = 1;
+            }
+        }
         if (tiled) {
             for (int i = threadIdx.x; i < w * h; i += kBlock) tile[0][i] = tile[1][i] = tile[2][i] = 0ull;
         }
@@ -397,18 +404,31 @@ __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a, int ru
     }
 }
 
+// Zeroes the sums of every chunk flagged touched or live; the others hold zero sums already.
 __global__ __launch_bounds__(kBlock) void batch2_clear_kernel(Batch2Args a) {
-    const int64_t n = a.gchunks * kBlock;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    for (int64_t c = blockIdx.x; c < a.gchunks; c += gridDim.x) {
+        if (!(a.touched[c] | a.live[c])) continue;
+        const int64_t i = c * kBlock + threadIdx.x;
         a.q[0][i] = 0ull;
         a.q[1][i] = 0ull;
         a.q[2][i] = 0ull;
     }
 }
 
-// The sums decoded into gm gmu gmv, then Flip.grid on them.
+// After a grid pass: live := touched, touched := 0.
+__global__ void batch2_flags_kernel(uint8_t* __restrict__ touched, uint8_t* __restrict__ live, int64_t chunks) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= chunks) return;
+    live[c] = touched[c];
+    touched[c] = 0;
+}
+
+// The sums decoded into gm gmu gmv, then Flip.grid on them, over the chunks flagged touched or
+// live. A live chunk P2G did not reach again holds zero sums, so it gets the zero velocity the
+// full pass would give it; a chunk in neither set already holds zero sums and velocities.
 __global__ __launch_bounds__(kBlock) void batch2_grid_kernel(Batch2Args a) {
     for (int64_t c = blockIdx.x; c < a.gchunks; c += gridDim.x) {
+        if (!(a.touched[c] | a.live[c])) continue;
         const int s = a.gscene[c];
         const Scene2 sc = a.scenes[s];
         const int64_t node = c * kBlock + threadIdx.x - sc.goff;
@@ -565,6 +585,8 @@ int batch2_grid(const Launch& l, const Batch2Args& a, int blocks) {
     if (!g) return 0;
     timed_begin(l, "batch2_grid");
     hipLaunchKernelGGL(batch2_grid_kernel, dim3(g), dim3(kBlock), 0, l.stream, a);
+    hipLaunchKernelGGL(batch2_flags_kernel, dim3((unsigned)((a.gchunks + 255) / 256)), dim3(256), 0, l.stream,
+                       a.touched, a.live, a.gchunks);
     timed_end(l, "batch2_grid");
     return check_launch("batch2_grid");
 }

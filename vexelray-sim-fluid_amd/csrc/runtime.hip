@@ -174,6 +174,7 @@ struct vxr_sim {
     vxr::Scene2* d_scenes = nullptr;
     int32_t *pscene = nullptr, *gscene = nullptr, *sscene = nullptr;
     int64_t pchunks = 0, gchunks = 0, schunks = 0;
+    uint8_t *btouched = nullptr, *blive = nullptr;  // 2D fixed-point path: chunk flags
     unsigned long long* q[3] = {};
     int cell_bits = 0, scene_bits = 0;
     int64_t starts_len = 0;
@@ -328,6 +329,10 @@ vxr::Mpm3Args mpm3_args(vxr_sim* s) {
 // Every brick flagged: the next clear and grid passes cover the whole grid (after the host wrote
 // grid fields, or cleared everything). Never inside a captured graph.
 int mark_all_bricks(vxr_sim* s) {
+    if (s->btouched) {
+        HIP_TRY(hipMemsetAsync(s->btouched, 1, s->gchunks, s->stream));
+        HIP_TRY(hipMemsetAsync(s->blive, 1, s->gchunks, s->stream));
+    }
     if (!s->touched) return 0;
     HIP_TRY(hipMemsetAsync(s->touched, 1, s->bricks, s->stream));
     HIP_TRY(hipMemsetAsync(s->live, 1, s->bricks, s->stream));
@@ -363,6 +368,8 @@ vxr::Batch2Args batch2_args(vxr_sim* s) {
     for (int f = 0; f < 3; ++f) a.q[f] = s->q[f];
     a.params = s->params;
     a.cell_bits = s->cell_bits;
+    a.touched = s->btouched;
+    a.live = s->blive;
     return a;
 }
 
@@ -774,6 +781,10 @@ int alloc_batch(vxr_sim* s) {
         if ((rc = alloc(s, s->gchunks * 256 * 2, &ptr))) return rc;
         s->q[f] = (unsigned long long*)ptr;
     }
+    if ((rc = alloc(s, (std::max<int64_t>(s->gchunks, 1) + 3) / 4, &ptr))) return rc;
+    s->btouched = (uint8_t*)ptr;
+    if ((rc = alloc(s, (std::max<int64_t>(s->gchunks, 1) + 3) / 4, &ptr))) return rc;
+    s->blive = (uint8_t*)ptr;
     HIP_TRY(hipMemcpy(s->d_scenes, s->scenes.data(), nscenes * sizeof(vxr::Scene2), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->pscene, pt.data(), pt.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->gscene, gt.data(), gt.size() * 4, hipMemcpyHostToDevice));
@@ -1057,7 +1068,7 @@ int vxr_write(vxr_sim* s, const char* name, const void* words, int64_t n) {
         }
     }
     HIP_TRY(hipMemcpyAsync(b.ptr, words, n * 4, hipMemcpyHostToDevice, s->stream));
-    if (three && name[0] == 'g' && name[1] != 0)  // a grid field: sweep it all next time
+    if ((three || s->btouched) && name[0] == 'g' && name[1] != 0)  // a grid field: sweep it all next time
         if (int rc = mark_all_bricks(s)) return rc;
     if (three && (b.ptr == (void*)s->p[17] || b.ptr == (void*)s->p[6]))
         // new slots, or new masses: home from the slots, and P2G and G2P visit every particle
@@ -1266,7 +1277,11 @@ int vxr_set_option(vxr_sim* s, const char* key, int64_t value) {
             if (std::string(key) == "p2g" && value != 2 && s->kind == VXR_KIND_MPM2_BATCH)
                 return fail(VXR_EINVAL, "a batch of scenes always scatters in fixed point (p2g 2)");
             drop_graphs(s);
+            const bool p2g_changed = std::string(key) == "p2g" && s->options[key] != value;
             s->options[key] = value;
+            // the float P2G paths clear and update the grid in full: the chunk flags start over
+            if (p2g_changed)
+                if (int rc = mark_all_bricks(s)) return rc;
             return VXR_OK;
         }
     }
