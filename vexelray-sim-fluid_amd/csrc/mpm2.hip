@@ -82,7 +82,11 @@ __device__ __forceinline__ void amounts(float* const* p, int64_t q, const Axis& 
 // Scaling by a power of two is exact, so this is the one rounding, the same on the CPU.
 constexpr int kMassBits = 40;
 constexpr int kMomentumBits = 32;
-__device__ __forceinline__ long long quantize(float a, float scale) { return __float2ll_rn(a * scale); }
+// The integer is held in a double: a * scale is exact (a power of two) and rintf rounds it to the
+// nearest integer, ties to even. Sums of such integers in double are exact while they stay below
+// 2^53 in magnitude (a run's sums of up to 4 particles' amounts, far below it), so a run folds
+// in four instructions an amount and converts to a 64-bit integer once, at its flush.
+__device__ __forceinline__ double quantize_d(float a, float scale) { return (double)rintf(a * scale); }
 __device__ __forceinline__ float dequantize(unsigned long long q, int bits) {
     return (float)ldexp((double)(long long)q, -bits);
 }
@@ -350,27 +354,29 @@ __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a, int ru
         unsigned long long* q2 = a.q[2] + sc.goff;
         for (int pass = 0; pass < passes; ++pass) {
             const int64_t first = c * kBatchAlign + ((int64_t)pass * kBlock + threadIdx.x) * run;
-            long long acc[27];
+            double acc[27];  // the run's fixed-point sums, exact integers in double
 #pragma unroll
-            for (int i = 0; i < 27; ++i) acc[i] = 0;
+            for (int i = 0; i < 27; ++i) acc[i] = 0.0;
             int rx = -1, ry = -1;
             auto flush = [&]() {
 #pragma unroll
                 for (int k = 0; k < 9; ++k) {
+                    const long long s0 = __double2ll_rn(acc[3 * k]), s1 = __double2ll_rn(acc[3 * k + 1]),
+                                    s2 = __double2ll_rn(acc[3 * k + 2]);
                     if (tiled) {
                         const int at = (ry - by0 + k / 3) * w + (rx - bx0 + k % 3);
-                        atomicAdd(&tile[0][at], (unsigned long long)acc[3 * k]);
-                        atomicAdd(&tile[1][at], (unsigned long long)acc[3 * k + 1]);
-                        atomicAdd(&tile[2][at], (unsigned long long)acc[3 * k + 2]);
+                        atomicAdd(&tile[0][at], (unsigned long long)s0);
+                        atomicAdd(&tile[1][at], (unsigned long long)s1);
+                        atomicAdd(&tile[2][at], (unsigned long long)s2);
                     } else {
                         const int64_t node = (int64_t)(ry + k / 3) * sc.nx + rx + k % 3;
-                        add_fixed(q0 + node, acc[3 * k]);
-                        add_fixed(q1 + node, acc[3 * k + 1]);
-                        add_fixed(q2 + node, acc[3 * k + 2]);
+                        add_fixed(q0 + node, s0);
+                        add_fixed(q1 + node, s1);
+                        add_fixed(q2 + node, s2);
                     }
                 }
 #pragma unroll
-                for (int i = 0; i < 27; ++i) acc[i] = 0;
+                for (int i = 0; i < 27; ++i) acc[i] = 0.0;
             };
             for (int r = 0; r < run; ++r) {
                 const int64_t q = first + r;
@@ -382,9 +388,9 @@ __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a, int ru
                     ry = ay.cell;
                 }
                 amounts(a.p, q, ax, ay, dt, bulk, rho0, [&](int k, float m, float mu, float mv) {
-                    acc[3 * k] += quantize(m, mscale);
-                    acc[3 * k + 1] += quantize(mu, pscale);
-                    acc[3 * k + 2] += quantize(mv, pscale);
+                    acc[3 * k] += quantize_d(m, mscale);
+                    acc[3 * k + 1] += quantize_d(mu, pscale);
+                    acc[3 * k + 2] += quantize_d(mv, pscale);
                 });
             }
             if (rx >= 0) flush();
