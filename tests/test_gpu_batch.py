@@ -119,6 +119,38 @@ def test_batch_scenes_equal_single_runs(gpu):
     batch.close()
 
 
+def _clusters(n, count, seed, size=5.0):
+    """count particles in two small clusters at opposite corners of an n x n grid, so that the
+    sorted keys jump over many chunks of 256 cells and the last particle sits far from the end."""
+    r = np.random.default_rng(seed)
+    half = count // 2
+    lo = r.uniform(1.0, 1.0 + size, (half, 2))
+    hi = r.uniform(0.6 * (n - 1), 0.7 * (n - 1), (count - half, 2))
+    xy = np.concatenate([lo, hi]).astype(np.float32)
+    return {"x": xy[:, 0].copy(), "y": xy[:, 1].copy(), "u": np.zeros(count, np.float32),
+            "v": np.zeros(count, np.float32), "m": np.full(count, 0.25, np.float32)}
+
+
+def test_batch_starts_sparse_scenes(gpu):
+    """Every scene's cellStarts after a sort against the oracle's, on scenes whose particles leave
+    long runs of empty chunks between them and after them: one particle, exactly kBatchAlign
+    particles, a scene of one chunk, and one whose first chunk holds more sorted keys than the
+    LDS window of 2048 (its cells' bounds are searched in global memory)."""
+    spec = [(256, 1024, 31, 5.0), (64, 1, 32, 5.0), (512, 3000, 33, 5.0), (8, 40, 34, 5.0), (128, 2048, 35, 5.0),
+            (64, 6000, 36, 1.5)]
+    sizes = [(n, n) for n, _, _, _ in spec]
+    sc = [_clusters(n, c, seed, size) for n, c, seed, size in spec]
+    print(f"[batch] sparse starts over {len(spec)} scenes", flush=True)
+    batch = particles.ParticleBatch(sizes, [c for _, c, _, _ in spec])
+    batch.load(sc, [_params(n) for n, _, _, _ in spec])
+    batch.sim.run(_native.GROUP_SORT)
+    for k, (n, _, _, _) in enumerate(spec):
+        got = batch.scene(k, names=("x", "y"))
+        np.testing.assert_array_equal(batch.scene_starts(k), oracle.mpm2_cell_starts(got["x"], got["y"], n, n),
+                                      err_msg=f"scene {k}")
+    batch.close()
+
+
 def test_graph_replays_match_eager(gpu):
     """The same batch stepped eagerly and by replayed graphs, several calls of different lengths
     (so graphs with and without a leading sort, of several step counts, each replayed more than

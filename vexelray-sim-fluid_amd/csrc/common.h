@@ -161,6 +161,12 @@ struct Batch2Args {
     // it. Clear and grid visit only chunks flagged in either.
     uint8_t* touched;
     uint8_t* live;
+    // Per starts chunk: its first cell's key, its scene's first particle, how many of its cells
+    // the scene has (bit 16 set on the scene's last chunk) and the scene's particles' end, set
+    // with the layout; and the first of its scene's sorted particles whose key is at least the
+    // first cell's, which the sort writes before the starts.
+    const int32_t* sinfo;  // 4 words a chunk
+    int32_t* bounds;
 };
 // `blocks` > 0 caps the grid (each block then strides over chunks); 0 is a block per chunk.
 int batch2_clear(const Launch& l, const Batch2Args& a, int blocks);

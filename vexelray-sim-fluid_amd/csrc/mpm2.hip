@@ -477,22 +477,109 @@ __global__ __launch_bounds__(kBlock) void batch2_keys_kernel(Batch2Args a, int32
 }
 
 // starts[soff + c] = the first of the scene's sorted particles in cell c or after it, counted
-// from the scene's first particle (Scatter.cellStarts, Scatter.java:484-502).
+// from the scene's first particle (Scatter.cellStarts, Scatter.java:484-502). Two passes over
+// chunks of kBlock cells. The first writes each chunk's bound (its scene's first particle whose
+// key is at least the chunk's first cell's): a particle whose chunk differs from its
+// predecessor's bounds every chunk from the one past the predecessor's to its own, and the
+// scene's last particle bounds the chunks after its own at the scene's end. The second takes
+// kStartsGroup chunks to a block: the first particle of each run of equal keys between a chunk's
+// bound and the next chunk's records its index in an LDS table by cell, and a cell's start is
+// the least index recorded at that cell or after it (the run count past the last), a suffix
+// minimum across the block. Both passes are chains of dependent loads, so each wave takes
+// several independent items to keep more loads in flight.
+constexpr int kStartsGroup = 4;
+
+__global__ __launch_bounds__(kBlock) void batch2_bounds_kernel(Batch2Args a, const int32_t* __restrict__ keys) {
+    constexpr int kPer = kBatchAlign / kBlock;
+    const uint32_t mask = (1u << a.cell_bits) - 1u;
+    for (int64_t pc = blockIdx.x; pc < a.pchunks; pc += gridDim.x) {
+        const Scene2 sc = a.scenes[a.pscene[pc]];  // a particle chunk lies in one scene
+        const int64_t c0 = sc.soff / kBlock;
+        const int64_t base = pc * kBatchAlign + threadIdx.x;
+        int32_t key[kPer], before[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int64_t q = base + j * kBlock;
+            const bool real = q - sc.poff < sc.count;
+            key[j] = real ? keys[q] : 0;
+            before[j] = real && q > sc.poff ? keys[q - 1] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int64_t q = base + j * kBlock;
+            const int64_t i = q - sc.poff;
+            if (i >= sc.count) continue;
+            const int64_t mine = (int64_t)(((uint32_t)key[j] & mask) / kBlock);
+            const int64_t prev = i > 0 ? (int64_t)(((uint32_t)before[j] & mask) / kBlock) : -1;
+            for (int64_t k = prev + 1; k <= mine; ++k) a.bounds[c0 + k] = (int32_t)q;
+            if (i == sc.count - 1) {
+                const int64_t chunks = ((int64_t)sc.cells + kBlock) / kBlock;  // cells + 1 starts
+                for (int64_t k = mine + 1; k < chunks; ++k) a.bounds[c0 + k] = (int32_t)(sc.poff + sc.count);
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void batch2_starts_kernel(Batch2Args a, const int32_t* __restrict__ keys,
                                                                int32_t* __restrict__ starts) {
-    for (int64_t c = blockIdx.x; c < a.schunks; c += gridDim.x) {
-        const int s = a.sscene[c];
-        const Scene2 sc = a.scenes[s];
-        const int64_t cell = c * kBlock + threadIdx.x - sc.soff;
-        if (cell > sc.cells) continue;
-        const int32_t target = (int32_t)(((uint32_t)s << a.cell_bits) | (uint32_t)cell);
-        int64_t lo = sc.poff, hi = sc.poff + sc.count;
-        while (lo < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (keys[mid] < target) lo = mid + 1;
-            else hi = mid;
+    constexpr int kWaves = kBlock / 64;
+    __shared__ int32_t head[kStartsGroup][kBlock];
+    __shared__ int32_t wave_min[kStartsGroup][kWaves];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int64_t groups = (a.schunks + kStartsGroup - 1) / kStartsGroup;
+    for (int64_t g = blockIdx.x; g < groups; g += gridDim.x) {
+        int lo[kStartsGroup], at[kStartsGroup + 1], rel[kStartsGroup], key0[kStartsGroup], valid[kStartsGroup];
+        at[0] = 0;
+#pragma unroll
+        for (int j = 0; j < kStartsGroup; ++j) {
+            const int64_t c = g * kStartsGroup + j;
+            int n = 0;
+            if (c < a.schunks) {
+                const int4 info = reinterpret_cast<const int4*>(a.sinfo)[c];
+                const bool last = (info.z >> 16) & 1;
+                const bool none = info.w == info.y;  // a scene without particles has no bounds written
+                lo[j] = none ? info.y : a.bounds[c];
+                n = (last || none ? info.w : a.bounds[c + 1]) - lo[j];
+                rel[j] = lo[j] - info.y;
+                key0[j] = info.x;
+                valid[j] = info.z & 0xffff;
+            } else {
+                lo[j] = rel[j] = key0[j] = valid[j] = 0;
+            }
+            at[j + 1] = at[j] + n;
+            head[j][t] = n;
         }
-        starts[sc.soff + cell] = (int32_t)(lo - sc.poff);
+        __syncthreads();
+        // the group's keys as one run of at[kStartsGroup]: index idx lies in the last chunk whose
+        // offset is at most idx (empty chunks share their successor's offset)
+        for (int idx = t; idx < at[kStartsGroup]; idx += kBlock) {
+            int j = 0, from = lo[0], off = 0, k0 = key0[0];
+#pragma unroll
+            for (int jj = 1; jj < kStartsGroup; ++jj)
+                if (idx >= at[jj]) j = jj, from = lo[jj], off = at[jj], k0 = key0[jj];
+            const int i = idx - off;
+            const int32_t key = keys[from + i];
+            if (i == 0 || keys[from + i - 1] != key) head[j][key - k0] = i;
+        }
+        __syncthreads();
+        int v[kStartsGroup];
+#pragma unroll
+        for (int j = 0; j < kStartsGroup; ++j) {
+            v[j] = head[j][t];
+            // suffix minimum over the wave: a lane past the last reads its own value
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) v[j] = min(v[j], __shfl_down(v[j], d));
+            if (lane == 0) wave_min[j][wave] = v[j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kStartsGroup; ++j) {
+#pragma unroll
+            for (int w = 1; w < kWaves; ++w)
+                if (w > wave) v[j] = min(v[j], wave_min[j][w]);
+            if (t < valid[j]) starts[(g * kStartsGroup + j) * kBlock + t] = rel[j] + v[j];
+        }
+        __syncthreads();  // the next group rewrites head and wave_min
     }
 }
 
@@ -619,7 +706,9 @@ int batch2_starts(const Launch& l, const Batch2Args& a, const int32_t* keys, int
     const unsigned g = grid_of(a.schunks, blocks);
     if (!g) return 0;
     timed_begin(l, "batch2_starts");
-    hipLaunchKernelGGL(batch2_starts_kernel, dim3(g), dim3(kBlock), 0, l.stream, a, keys, starts);
+    hipLaunchKernelGGL(batch2_bounds_kernel, dim3(grid_of(a.pchunks, blocks)), dim3(kBlock), 0, l.stream, a, keys);
+    hipLaunchKernelGGL(batch2_starts_kernel, dim3(grid_of((a.schunks + kStartsGroup - 1) / kStartsGroup, blocks)),
+                       dim3(kBlock), 0, l.stream, a, keys, starts);
     timed_end(l, "batch2_starts");
     return check_launch("batch2_starts");
 }

@@ -175,6 +175,8 @@ struct vxr_sim {
     int32_t *pscene = nullptr, *gscene = nullptr, *sscene = nullptr;
     int64_t pchunks = 0, gchunks = 0, schunks = 0;
     uint8_t *btouched = nullptr, *blive = nullptr;  // 2D fixed-point path: chunk flags
+    int32_t* sinfo = nullptr;    // batch: per starts chunk, what the layout fixes (4 words)
+    int32_t* sbounds = nullptr;  // batch: per starts chunk, its first particle (the sort's scratch)
     unsigned long long* q[3] = {};
     int cell_bits = 0, scene_bits = 0;
     int64_t starts_len = 0;
@@ -370,6 +372,8 @@ vxr::Batch2Args batch2_args(vxr_sim* s) {
     a.cell_bits = s->cell_bits;
     a.touched = s->btouched;
     a.live = s->blive;
+    a.sinfo = s->sinfo;
+    a.bounds = s->sbounds;
     return a;
 }
 
@@ -785,6 +789,23 @@ int alloc_batch(vxr_sim* s) {
     s->btouched = (uint8_t*)ptr;
     if ((rc = alloc(s, (std::max<int64_t>(s->gchunks, 1) + 3) / 4, &ptr))) return rc;
     s->blive = (uint8_t*)ptr;
+    if ((rc = alloc(s, std::max<int64_t>(s->schunks, 1), &ptr))) return rc;
+    s->sbounds = (int32_t*)ptr;
+    if ((rc = alloc(s, std::max<int64_t>(s->schunks, 1) * 4, &ptr))) return rc;
+    s->sinfo = (int32_t*)ptr;
+    std::vector<int32_t> info(std::max<int64_t>(s->schunks, 1) * 4, 0);
+    for (int64_t k = 0; k < nscenes; ++k) {
+        const vxr::Scene2& sc = s->scenes[k];
+        for (int64_t first = 0; first <= sc.cells; first += 256) {
+            int32_t* w = &info[(sc.soff + first) / 256 * 4];
+            const bool last = first + 256 > sc.cells;
+            w[0] = (int32_t)(((uint32_t)k << s->cell_bits) | (uint32_t)first);
+            w[1] = (int32_t)sc.poff;
+            w[2] = (int32_t)(last ? sc.cells + 1 - first : 256) | (last ? 1 << 16 : 0);
+            w[3] = (int32_t)(sc.poff + sc.count);
+        }
+    }
+    HIP_TRY(hipMemcpy(s->sinfo, info.data(), info.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->d_scenes, s->scenes.data(), nscenes * sizeof(vxr::Scene2), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->pscene, pt.data(), pt.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->gscene, gt.data(), gt.size() * 4, hipMemcpyHostToDevice));
