@@ -186,20 +186,44 @@ __device__ __forceinline__ void keep_inside(float& at, float& vel, int size) {
     }
 }
 
-// Flip.advectBody, Flip.java:494-545 (not relaxing), for particle q of one scene.
-__device__ __forceinline__ void g2p2_particle(float* const* p, int64_t q, int nx, int ny, const float* gu,
-                                              const float* gv, const int32_t* params) {
-    const Axis ax = axis(p[0][q], nx), ay = axis(p[1][q], ny);
-    const int64_t base = (int64_t)ay.cell * nx + ax.cell;
+// What G2P reads of a particle and its grid: the position, J, and the 9 stencil nodes' velocities.
+struct G2p2In {
+    float x, y, j;
+    Axis ax, ay;
+    float gu[9], gv[9];
+};
+
+__device__ __forceinline__ void g2p2_gather(float* const* p, int64_t q, int nx, int ny, const float* gu,
+                                            const float* gv, G2p2In& in) {
+    in.x = p[0][q];
+    in.y = p[1][q];
+    in.j = p[5][q];
+    in.ax = axis(in.x, nx);
+    in.ay = axis(in.y, ny);
+    const int64_t base = (int64_t)in.ay.cell * nx + in.ax.cell;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int64_t node = base + (k / 3) * nx + k % 3;
+        in.gu[k] = gu[node];
+        in.gv[k] = gv[node];
+    }
+}
+
+// Flip.advectBody, Flip.java:494-545 (not relaxing), for particle q of one scene, from what
+// g2p2_gather read. Reads and stores are split so a thread can have several particles' reads
+// in flight before its first store (a read cannot be moved past a store that may alias it).
+__device__ __forceinline__ void g2p2_finish(float* const* p, int64_t q, int nx, int ny, const int32_t* params,
+                                            const G2p2In& in) {
+    const Axis& ax = in.ax;
+    const Axis& ay = in.ay;
     float u = 0, w = 0, b00 = 0, b01 = 0, b10 = 0, b11 = 0;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const float wgt = ax.w[k % 3] * ay.w[k / 3];
         const float dx = (float)(k % 3) - ax.off;
         const float dy = (float)(k / 3) - ay.off;
-        const int64_t node = base + (k / 3) * nx + k % 3;
-        const float ui = wgt * gu[node];
-        const float vi = wgt * gv[node];
+        const float ui = wgt * in.gu[k];
+        const float vi = wgt * in.gv[k];
         u = u + ui;
         w = w + vi;
         b00 = b00 + ui * dx;
@@ -209,8 +233,8 @@ __device__ __forceinline__ void g2p2_particle(float* const* p, int64_t q, int nx
     }
     const float c00 = 4.0f * b00, c01 = 4.0f * b01, c10 = 4.0f * b10, c11 = 4.0f * b11;
     const float dt = bits_f(params[DT]);
-    float x = p[0][q] + dt * u;
-    float y = p[1][q] + dt * w;
+    float x = in.x + dt * u;
+    float y = in.y + dt * w;
     keep_inside(x, u, nx);
     keep_inside(y, w, ny);
     p[0][q] = x;
@@ -221,8 +245,15 @@ __device__ __forceinline__ void g2p2_particle(float* const* p, int64_t q, int nx
     p[7][q] = c01;
     p[8][q] = c10;
     p[9][q] = c11;
-    const float j = p[5][q] * (1.0f + dt * (c00 + c11));
+    const float j = in.j * (1.0f + dt * (c00 + c11));
     p[5][q] = clampf(j, 1.0f / kJBound, kJBound);
+}
+
+__device__ __forceinline__ void g2p2_particle(float* const* p, int64_t q, int nx, int ny, const float* gu,
+                                              const float* gv, const int32_t* params) {
+    G2p2In in;
+    g2p2_gather(p, q, nx, ny, gu, gv, in);
+    g2p2_finish(p, q, nx, ny, params, in);
 }
 
 __global__ __launch_bounds__(kBlock) void g2p2_kernel(Mpm2Args a) {
@@ -450,14 +481,26 @@ __global__ __launch_bounds__(kBlock) void batch2_grid_kernel(Batch2Args a) {
     }
 }
 
+// A block a chunk of kBatchAlign particles (one scene's), a thread kBatchAlign / kBlock of them:
+// every read of the thread's particles in flight, then their sums and stores.
 __global__ __launch_bounds__(kBlock) void batch2_g2p_kernel(Batch2Args a) {
-    const int64_t chunks = a.pchunks * (kBatchAlign / kBlock);
-    for (int64_t c = blockIdx.x; c < chunks; c += gridDim.x) {
-        const int64_t q = c * kBlock + threadIdx.x;
-        const int s = a.pscene[q / kBatchAlign];
+    constexpr int kPer = kBatchAlign / kBlock;
+    for (int64_t pc = blockIdx.x; pc < a.pchunks; pc += gridDim.x) {
+        const int s = a.pscene[pc];
         const Scene2 sc = a.scenes[s];
-        if (q - sc.poff >= sc.count) continue;
-        g2p2_particle(a.p, q, sc.nx, sc.ny, a.g[3] + sc.goff, a.g[4] + sc.goff, a.params + 28 * (int64_t)s);
+        const float* gu = a.g[3] + sc.goff;
+        const float* gv = a.g[4] + sc.goff;
+        G2p2In in[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int64_t q = pc * kBatchAlign + k * kBlock + threadIdx.x;
+            if (q - sc.poff < sc.count) g2p2_gather(a.p, q, sc.nx, sc.ny, gu, gv, in[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int64_t q = pc * kBatchAlign + k * kBlock + threadIdx.x;
+            if (q - sc.poff < sc.count) g2p2_finish(a.p, q, sc.nx, sc.ny, a.params + 28 * (int64_t)s, in[k]);
+        }
     }
 }
 
@@ -685,7 +728,7 @@ int batch2_grid(const Launch& l, const Batch2Args& a, int blocks) {
 }
 
 int batch2_advect(const Launch& l, const Batch2Args& a, int blocks) {
-    const unsigned g = grid_of(a.pchunks * (kBatchAlign / kBlock), blocks);
+    const unsigned g = grid_of(a.pchunks, blocks);
     if (!g) return 0;
     timed_begin(l, "batch2_g2p");
     hipLaunchKernelGGL(batch2_g2p_kernel, dim3(g), dim3(kBlock), 0, l.stream, a);

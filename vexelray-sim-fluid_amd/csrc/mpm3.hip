@@ -724,9 +724,11 @@ __device__ __forceinline__ void keep_inside(float& at, float& vel, int size) {
     }
 }
 
-// The end of Flip3.advect (:361-377): C = 4 sum, the move, the walls, J.
-__device__ __forceinline__ void g2p_finish(const Mpm3Args& a, int64_t q, const float (&pos0)[3], float (&vel)[3],
-                                           const float (&sum)[9], float dt) {
+// The end of Flip3.advect (:361-377): C = 4 sum, the move, the walls, J. `j0` is the particle's
+// J, read by the caller with its position (a read here would sit behind the stores above it,
+// which the compiler cannot move it past, and wait out a whole memory latency per particle).
+__device__ __forceinline__ void g2p_finish(const Mpm3Args& a, int64_t q, const float (&pos0)[3], float j0,
+                                           float (&vel)[3], const float (&sum)[9], float dt) {
     float c[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) c[i] = 4.0f * sum[i];
@@ -740,15 +742,15 @@ __device__ __forceinline__ void g2p_finish(const Mpm3Args& a, int64_t q, const f
     }
 #pragma unroll
     for (int i = 0; i < 9; ++i) a.p[8 + i][q] = c[i];
-    const float j = a.p[7][q] * (1.0f + dt * (c[0] + (c[4] + c[8])));
+    const float j = j0 * (1.0f + dt * (c[0] + (c[4] + c[8])));
     a.p[7][q] = clampf(j, 1.0f / kJBound, kJBound);
 }
 
 // One particle's G2P in the reference's order (Flip3.java:349-360), node velocities read
 // through `node_vel(k, axis)`: the CPU restatement's bits.
 template <typename NodeVel>
-__device__ __forceinline__ void g2p_one(const Mpm3Args& a, int64_t q, const float (&pos0)[3], const Axis (&ax)[3],
-                                        float dt, NodeVel node_vel) {
+__device__ __forceinline__ void g2p_one(const Mpm3Args& a, int64_t q, const float (&pos0)[3], float j0,
+                                        const Axis (&ax)[3], float dt, NodeVel node_vel) {
     float vel[3] = {0.0f, 0.0f, 0.0f};
     float sum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -763,15 +765,15 @@ __device__ __forceinline__ void g2p_one(const Mpm3Args& a, int64_t q, const floa
             for (int e = 0; e < 3; ++e) sum[3 * i + e] = sum[3 * i + e] + na * d[e];
         }
     }
-    g2p_finish(a, q, pos0, vel, sum, dt);
+    g2p_finish(a, q, pos0, j0, vel, sum, dt);
 }
 
 // The same sums contracted one axis at a time with fused multiply-adds: x first (the weight
 // and the weight times the offset), then y, then z -- a third of the operations, each sum
 // within a few ulps of the reference's.
 template <typename NodeVel>
-__device__ __forceinline__ void g2p_fast(const Mpm3Args& a, int64_t q, const float (&pos0)[3], const Axis (&ax)[3],
-                                         float dt, NodeVel node_vel) {
+__device__ __forceinline__ void g2p_fast(const Mpm3Args& a, int64_t q, const float (&pos0)[3], float j0,
+                                         const Axis (&ax)[3], float dt, NodeVel node_vel) {
     float wd[3][3];
 #pragma unroll
     for (int e = 0; e < 3; ++e)
@@ -807,18 +809,19 @@ __device__ __forceinline__ void g2p_fast(const Mpm3Args& a, int64_t q, const flo
         sum[3 * c + 1] = s1;
         sum[3 * c + 2] = s2;
     }
-    g2p_finish(a, q, pos0, vel, sum, dt);
+    g2p_finish(a, q, pos0, j0, vel, sum, dt);
 }
 
 __global__ __launch_bounds__(kBlock) void g2p3_kernel(Mpm3Args a) {
     const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (q >= *a.active || !(a.p[6][q] > 0.0f)) return;
     const float pos[3] = {a.p[0][q], a.p[1][q], a.p[2][q]};
+    const float j0 = a.p[7][q];
     Axis ax[3] = {axis(pos[0], a.nx), axis(pos[1], a.ny), axis(pos[2], a.nz)};
     const int64_t base = ((int64_t)ax[2].cell * a.ny + ax[1].cell) * a.nx + ax[0].cell;
     const int64_t plane = (int64_t)a.nx * a.ny;
     const float dt = bits_f(a.params[DT]);
-    g2p_one(a, q, pos, ax, dt, [&](int k, int i) {
+    g2p_one(a, q, pos, j0, ax, dt, [&](int k, int i) {
         return a.g[4 + i][base + (k / 9) * plane + ((k / 3) % 3) * a.nx + k % 3];
     });
 }
@@ -839,7 +842,7 @@ __global__ __launch_bounds__(kBlock) void g2p3_tiled_kernel(Mpm3Args a) {
     __shared__ int red[6 * (kBlock / kWave)];
     const int64_t chunk = (int64_t)blockIdx.x * kBlock * R;
     if (chunk >= *a.active) return;  // a block of inactive slots
-    float pos[R][3];
+    float pos[R][3], jr[R];
     bool live[R];
     const int size[3] = {a.nx, a.ny, a.nz};
     int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
@@ -848,6 +851,7 @@ __global__ __launch_bounds__(kBlock) void g2p3_tiled_kernel(Mpm3Args a) {
         const int64_t q = chunk + (int64_t)r * kBlock + threadIdx.x;  // < capacity
 #pragma unroll
         for (int i = 0; i < 3; ++i) pos[r][i] = a.p[i][q];
+        jr[r] = a.p[7][q];
         live[r] = q < a.count && a.p[6][q] > 0.0f;
         if (!live[r]) continue;
 #pragma unroll
@@ -861,14 +865,38 @@ __global__ __launch_bounds__(kBlock) void g2p3_tiled_kernel(Mpm3Args a) {
     if (box.empty) return;
     const bool use_tile = box.ex <= kG2PX && box.ey <= kG2PY && box.ez <= kG2PZ;
     if (use_tile) {
+        // every load of the thread's nodes in flight before the first LDS write. A node's
+        // (x, y, z) in the box from its number t by float reciprocals: t < 2^12, so t + 1/2
+        // lies at least 1/(2 ex) from a multiple of ex and the rounded product floors exactly.
+        constexpr int kStage = (kG2PNodes + kBlock - 1) / kBlock;
         const int nodes = box.ex * box.ey * box.ez;
-        for (int t = threadIdx.x; t < nodes; t += kBlock) {
-            const int lx = t % box.ex, ly = (t / box.ex) % box.ey, lz = t / (box.ex * box.ey);
-            const int64_t node = ((int64_t)(box.lo[2] + lz) * a.ny + (box.lo[1] + ly)) * a.nx + (box.lo[0] + lx);
-            const int at = (lz * kG2PY + ly) * kG2PX + lx;
-            tile[at] = a.g[4][node];
-            tile[kG2PNodes + at] = a.g[5][node];
-            tile[2 * kG2PNodes + at] = a.g[6][node];
+        const float inv_ex = 1.0f / (float)box.ex, inv_ey = 1.0f / (float)box.ey;
+        const int64_t plane = (int64_t)a.nx * a.ny;
+        float gu[kStage], gv[kStage], gw[kStage];
+        int at[kStage];
+#pragma unroll
+        for (int k = 0; k < kStage; ++k) {
+            const int t = threadIdx.x + k * kBlock;
+            at[k] = -1;
+            if (t < nodes) {
+                const int row = (int)(((float)t + 0.5f) * inv_ex);
+                const int lx = t - row * box.ex;
+                const int lz = (int)(((float)row + 0.5f) * inv_ey);
+                const int ly = row - lz * box.ey;
+                const int64_t node =
+                    (int64_t)(box.lo[2] + lz) * plane + (int64_t)(box.lo[1] + ly) * a.nx + (box.lo[0] + lx);
+                at[k] = (lz * kG2PY + ly) * kG2PX + lx;
+                gu[k] = a.g[4][node];
+                gv[k] = a.g[5][node];
+                gw[k] = a.g[6][node];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kStage; ++k) {
+            if (at[k] < 0) continue;
+            tile[at[k]] = gu[k];
+            tile[kG2PNodes + at[k]] = gv[k];
+            tile[2 * kG2PNodes + at[k]] = gw[k];
         }
     }
     __syncthreads();
@@ -883,13 +911,13 @@ __global__ __launch_bounds__(kBlock) void g2p3_tiled_kernel(Mpm3Args a) {
             const float* base = tile + ((ax[2].cell - box.lo[2]) * kG2PY + (ax[1].cell - box.lo[1])) * kG2PX +
                                 (ax[0].cell - box.lo[0]);
             auto at = [&](int k, int i) { return base[i * kG2PNodes + ((k / 9) * kG2PY + (k / 3) % 3) * kG2PX + k % 3]; };
-            if (kFast) g2p_fast(a, q, pos[r], ax, dt, at);
-            else g2p_one(a, q, pos[r], ax, dt, at);
+            if (kFast) g2p_fast(a, q, pos[r], jr[r], ax, dt, at);
+            else g2p_one(a, q, pos[r], jr[r], ax, dt, at);
         } else {
             const int64_t base = ((int64_t)ax[2].cell * a.ny + ax[1].cell) * a.nx + ax[0].cell;
             auto at = [&](int k, int i) { return a.g[4 + i][base + (k / 9) * plane + ((k / 3) % 3) * a.nx + k % 3]; };
-            if (kFast) g2p_fast(a, q, pos[r], ax, dt, at);
-            else g2p_one(a, q, pos[r], ax, dt, at);
+            if (kFast) g2p_fast(a, q, pos[r], jr[r], ax, dt, at);
+            else g2p_one(a, q, pos[r], jr[r], ax, dt, at);
         }
     }
 }
