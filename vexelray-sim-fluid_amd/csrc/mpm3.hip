@@ -281,6 +281,14 @@ __device__ __forceinline__ void p2g_fold(const Axis (&ax)[3], float m, const flo
         for (int b = 0; b < 3; ++b) B[a][b] = a == b ? fmaf(m, c[3 * a + b], s) : m * c[3 * a + b];
         E[a] = fmaf(m, vel[a], -fmaf(B[a][0], ax[0].off, fmaf(B[a][1], ax[1].off, B[a][2] * ax[2].off)));
     }
+    // n B_a for a node offset n of 0, 1 or 2 is nothing, B_a or B_a + B_a: written out, since
+    // the compiler must keep a multiplication by 0.0 (it is -0 or NaN for some B).
+    auto plus = [](float t, float b, int n) { return n == 0 ? t : (n == 1 ? t + b : t + (b + b)); };
+    float Ej[3][3];  // E + j B_1
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) Ej[j][a] = plus(E[a], B[a][1], j);
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
 #pragma unroll
@@ -288,14 +296,14 @@ __device__ __forceinline__ void p2g_fold(const Axis (&ax)[3], float m, const flo
             const float wyz = ax[1].w[j] * ax[2].w[l];
             float T[3];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) T[a] = E[a] + (float)j * B[a][1] + (float)l * B[a][2];
+            for (int a = 0; a < 3; ++a) T[a] = plus(Ej[j][a], B[a][2], l);
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 const int k = i + 3 * j + 9 * l;
                 const float wgt = ax[0].w[i] * wyz;
                 acc[4 * k] = fmaf(wgt, m, acc[4 * k]);
 #pragma unroll
-                for (int a = 0; a < 3; ++a) acc[4 * k + 1 + a] = fmaf(wgt, T[a] + (float)i * B[a][0], acc[4 * k + 1 + a]);
+                for (int a = 0; a < 3; ++a) acc[4 * k + 1 + a] = fmaf(wgt, plus(T[a], B[a][0], i), acc[4 * k + 1 + a]);
             }
         }
     }
