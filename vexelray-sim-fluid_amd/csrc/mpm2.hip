@@ -506,16 +506,29 @@ __global__ __launch_bounds__(kBlock) void batch2_g2p_kernel(Batch2Args a) {
 
 // key = scene << cell_bits | Scatter.cell; a pad particle takes its scene's cell count, so it
 // sorts after the scene's particles and before the next scene's.
+// A block a chunk of kBatchAlign particles (one scene's), a thread kBatchAlign / kBlock of them,
+// every position read before the first key is written.
 __global__ __launch_bounds__(kBlock) void batch2_keys_kernel(Batch2Args a, int32_t* __restrict__ keys,
                                                              int32_t* __restrict__ index) {
-    const int64_t chunks = a.pchunks * (kBatchAlign / kBlock);
-    for (int64_t c = blockIdx.x; c < chunks; c += gridDim.x) {
-        const int64_t q = c * kBlock + threadIdx.x;
-        const int s = a.pscene[q / kBatchAlign];
+    constexpr int kPer = kBatchAlign / kBlock;
+    for (int64_t pc = blockIdx.x; pc < a.pchunks; pc += gridDim.x) {
+        const int s = a.pscene[pc];
         const Scene2 sc = a.scenes[s];
-        const int32_t cell = q - sc.poff < sc.count ? cell2(a.p[0][q], a.p[1][q], sc.nx, sc.ny) : sc.cells;
-        keys[q] = (int32_t)(((uint32_t)s << a.cell_bits) | (uint32_t)cell);
-        index[q] = (int32_t)q;
+        float x[kPer], y[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int64_t q = pc * kBatchAlign + k * kBlock + threadIdx.x;
+            const bool real = q - sc.poff < sc.count;
+            x[k] = real ? a.p[0][q] : 0.0f;
+            y[k] = real ? a.p[1][q] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int64_t q = pc * kBatchAlign + k * kBlock + threadIdx.x;
+            const int32_t cell = q - sc.poff < sc.count ? cell2(x[k], y[k], sc.nx, sc.ny) : sc.cells;
+            keys[q] = (int32_t)(((uint32_t)s << a.cell_bits) | (uint32_t)cell);
+            index[q] = (int32_t)q;
+        }
     }
 }
 
@@ -737,7 +750,7 @@ int batch2_advect(const Launch& l, const Batch2Args& a, int blocks) {
 }
 
 int batch2_keys(const Launch& l, const Batch2Args& a, int32_t* keys, int32_t* index, int blocks) {
-    const unsigned g = grid_of(a.pchunks * (kBatchAlign / kBlock), blocks);
+    const unsigned g = grid_of(a.pchunks, blocks);
     if (!g) return 0;
     timed_begin(l, "batch2_keys");
     hipLaunchKernelGGL(batch2_keys_kernel, dim3(g), dim3(kBlock), 0, l.stream, a, keys, index);

@@ -56,41 +56,49 @@ __global__ __launch_bounds__(kBlock) void scan_rows_kernel(int32_t* __restrict__
     if (threadIdx.x == kBlock - 1) totals[blockIdx.x] = part[kBlock - 1];
 }
 
+// Each wave takes a contiguous quarter of the block's elements, kPerWave of them, and ranks them
+// stably with ballots (the lanes whose digits agree on every bit) and a running count per digit
+// in LDS that only this wave touches, so the waves need no barrier between their rounds. The
+// block's elements then go through LDS in digit order -- a digit's elements in input order:
+// earlier waves first, then by rank in the wave -- and leave in that order, each to its digit's
+// place past all elements of smaller digits and the digit's elements in earlier blocks (the
+// scanned histogram). Neighbouring threads write neighbouring addresses within a digit.
+constexpr int kWaves = kBlock / kWave;
+constexpr int kPerWave = kPerBlock / kWaves;
+constexpr int kRounds = kPerWave / kWave;
+
 __global__ __launch_bounds__(kBlock) void scatter_kernel(const int32_t* __restrict__ keys_in,
                                                          const int32_t* __restrict__ vals_in,
                                                          int32_t* __restrict__ keys_out,
                                                          int32_t* __restrict__ vals_out, int64_t n, int shift,
                                                          int mask, int digit_bits, const int32_t* __restrict__ hist,
                                                          const int32_t* __restrict__ totals, int blocks) {
-    __shared__ int base[kMaxDigits];
-    __shared__ int wave_count[kBlock / kWave][kMaxDigits];
+    __shared__ int count[kWaves][kMaxDigits];  // a wave's running count per digit, then its local base
+    __shared__ int2 scan[kBlock];              // (all blocks, this block) per digit, scanned
+    __shared__ int out_of[kMaxDigits];         // global place of a digit's first element, less its local one
+    __shared__ int32_t sorted_keys[kPerBlock], sorted_vals[kPerBlock];
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wave = tid >> 6;
     const int digits = mask + 1;
-
-    // base[d] = (all elements with a smaller digit) + (elements of digit d in earlier blocks)
-    if (tid == 0) {
-        int run = 0;
-        for (int d = 0; d < digits; ++d) {
-            base[d] = run;
-            run += totals[d];
-        }
-    }
-    __syncthreads();
-    for (int d = tid; d < digits; d += kBlock) base[d] += hist[(int64_t)d * blocks + blockIdx.x];
-    for (int d = tid; d < (kBlock / kWave) * kMaxDigits; d += kBlock) (&wave_count[0][0])[d] = 0;
+    for (int d = tid; d < kWaves * kMaxDigits; d += kBlock) (&count[0][0])[d] = 0;
     __syncthreads();
 
     const uint64_t below = (1ull << lane) - 1ull;
     const int64_t begin = (int64_t)blockIdx.x * kPerBlock;
-    for (int it = 0; it < kPerBlock / kBlock; ++it) {
-        const int64_t i = begin + (int64_t)it * kBlock + tid;
-        const bool valid = i < n;
-        const int32_t key = valid ? keys_in[i] : 0;
-        const int32_t val = valid ? vals_in[i] : 0;
-        const int d = (key >> shift) & mask;
-        // lanes holding the same digit: agree on every digit bit, and valid
+    const int64_t first = begin + (int64_t)wave * kPerWave;
+    int32_t key[kRounds], val[kRounds];
+    int pos[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const int64_t i = first + r * kWave + lane;
+        key[r] = i < n ? keys_in[i] : 0;
+        val[r] = i < n ? vals_in[i] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const bool valid = first + r * kWave + lane < n;
+        const int d = (key[r] >> shift) & mask;
         uint64_t same = __ballot(valid);
         for (int b = 0; b < digit_bits; ++b) {
             const bool bit = (d >> b) & 1;
@@ -98,25 +106,56 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(const int32_t* __restri
             same &= bit ? ones : ~ones;
         }
         const int rank = __popcll(same & below);
-        if (valid && rank == 0) wave_count[wave][d] = __popcll(same);
+        // every lane of the digit reads the count before the first of them moves it on: one
+        // wave's LDS operations happen in program order
+        pos[r] = (valid ? count[wave][d] : 0) + rank;
+        asm volatile("" ::: "memory");  // the reads above stay before the write below
+        if (valid && rank == 0) count[wave][d] += __popcll(same);
+        asm volatile("" ::: "memory");
+    }
+    __syncthreads();
+    // per digit: the count over all blocks (totals) and over this block, both scanned
+    int2 own = make_int2(0, 0);
+    if (tid < digits) {
+        own.x = totals[tid];
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) own.y += count[w][tid];
+    }
+    scan[tid] = own;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        const int2 v = tid >= off ? scan[tid - off] : make_int2(0, 0);
         __syncthreads();
-        if (valid) {
-            int pos = base[d] + rank;
-            for (int w = 0; w < wave; ++w) pos += wave_count[w][d];
-            keys_out[pos] = key;
-            vals_out[pos] = val;
+        scan[tid].x += v.x;
+        scan[tid].y += v.y;
+        __syncthreads();
+    }
+    if (tid < digits) {
+        const int local = scan[tid].y - own.y;
+        out_of[tid] = scan[tid].x - own.x + hist[(int64_t)tid * blocks + blockIdx.x] - local;
+        int run = local;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const int c = count[w][tid];
+            count[w][tid] = run;
+            run += c;
         }
-        __syncthreads();
-        for (int dd = tid; dd < digits; dd += kBlock) {
-            int add = 0;
-            for (int w = 0; w < kBlock / kWave; ++w) {
-                add += wave_count[w][dd];
-                wave_count[w][dd] = 0;
-            }
-            base[dd] += add;
-        }
-        __syncthreads();
-        if (begin + (int64_t)(it + 1) * kBlock >= n) break;  // uniform: the rest of the block is past the end
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        if (first + r * kWave + lane >= n) continue;
+        const int at = count[wave][(key[r] >> shift) & mask] + pos[r];
+        sorted_keys[at] = key[r];
+        sorted_vals[at] = val[r];
+    }
+    __syncthreads();
+    const int here = (int)min((int64_t)kPerBlock, n - begin);
+    for (int i = tid; i < here; i += kBlock) {
+        const int32_t k = sorted_keys[i];
+        const int at = out_of[(k >> shift) & mask] + i;
+        keys_out[at] = k;
+        vals_out[at] = sorted_vals[i];
     }
 }
 
