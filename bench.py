@@ -160,12 +160,12 @@ def bench_batch2(steps, scenes=256, sort_every=SORT_EVERY, graph=True, tune=True
                         "64^2..512^2, water columns at 4 per cell, fixed-point P2G, sort every 10"}
 
 
-def bench_small2(steps, graph):
-    """The 2D reference-sized scene alone: 8,192 particles (a 32x64-cell column at 4 a cell) on
-    128^2, the launch-bound case that graphs address."""
+def bench_small2(steps, graph, ppc=4):
+    """The 2D reference-sized scene alone: a 32x64-cell column on 128^2 at `ppc` a cell -- 8,192
+    particles at 4, the launch-bound case that graphs address, and 65,536 at 32."""
     from vexhip import particles, scenes as sc_mod
 
-    s = sc_mod.mpm2_dam_block()
+    s = sc_mod.mpm2_dam_block(ppc=ppc)
     count = len(s["x"])
     h = 1.0 / 127
     words = particles.params2(1e-4, 0, -9.8 / h, 10.0 / h ** 2, 1.0)
@@ -407,6 +407,10 @@ def main():
         print("[bench] 2D 8,192 particles on 128^2, eager and graph", file=sys.stderr, flush=True)
         line["small2d"] = {"eager": bench_small2(500, False), "graph": bench_small2(500, True),
                            "workload": "2D MLS-MPM, 8,192 particles, 128^2, fixed-point P2G, sort every 10"}
+        print("[bench] 2D 65,536 particles on 128^2 (32 a cell), graph", file=sys.stderr, flush=True)
+        line["small2d_65536"] = {"graph": bench_small2(500, True, ppc=32),
+                                 "workload": "2D MLS-MPM, 65,536 particles (the 32x64-cell column at 32 a cell), "
+                                             "128^2, fixed-point P2G, sort every 10"}
     if not args.no_surrogate:
         print(f"[bench] surrogate: train {args.surrogate_iters} iterations at 1024^2, judge, time",
               file=sys.stderr, flush=True)

@@ -92,20 +92,21 @@ __device__ __forceinline__ void row_bases(const Mpm3Args& a, int y, int z, int64
 
 // Tension3.blur. A wave takes one row y of one brick layer of z (8 values of z) and, for each
 // 64 values of x in the span of that brick row in the region (or the whole row when dense),
-// marches up z: the 3x3 taps of a plane of z are loaded once (scaled on the way when `kScaled`)
-// and serve as the dk = +1, 0 and -1 planes of three successive nodes. Each node sums its 27
-// taps in the reference's order, sum += weight * value, no contraction. Held reads make the
-// planes of node z exactly held(z - 1), held(z), held(z + 1), so the march shifts them.
+// loads the 3x3 taps of the layer's ten planes of z at once (scaled on the way when `kScaled`);
+// each plane serves as the dk = +1, 0 and -1 planes of three successive nodes. Each node sums
+// its 27 taps in the reference's order, sum += weight * value, no contraction. Held reads make
+// the planes of node z exactly held(z - 1), held(z), held(z + 1).
 template <bool kScaled>
 __device__ __forceinline__ void load_plane(const Mpm3Args& a, const float* __restrict__ src, int zp,
                                            const int (&ys)[3], const int (&xs)[3], float rho0, float (&p)[9]) {
     const int size[3] = {a.nx, a.ny, a.nz};
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        const int64_t base = ((int64_t)zp * a.ny + ys[j]) * a.nx;
+        // the row's start (the same for the whole wave) and the lane's byte offset in it
+        const char* row = reinterpret_cast<const char*>(src + ((int64_t)zp * a.ny + ys[j]) * a.nx);
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            float value = src[base + xs[i]];
+            float value = *reinterpret_cast<const float*>(row + (uint32_t)xs[i] * 4u);
             if (kScaled) {
                 // full at a rest mass, halved for each axis on which the node read lies on the wall ring
                 const int h[3] = {xs[i], ys[j], zp};
@@ -127,22 +128,27 @@ __global__ __launch_bounds__(kBlock) void blur_march_kernel(Mpm3Args a, const fl
                                                             float* __restrict__ dst, int dense) {
     const int lane = threadIdx.x & (kWave - 1);
     const int layers = (a.nz + kBrick - 1) / kBrick;
-    const int64_t job = (int64_t)blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
-    if (job >= (int64_t)a.ny * layers) return;
-    const int y = (int)(job % a.ny), z0 = (int)(job / a.ny) * kBrick;
+    // the wave's job, row and span held in scalar registers: the rows' addresses are then scalar
+    const int job = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6)));
+    if (job >= a.ny * layers) return;
+    const int y = job % a.ny, z0 = (job / a.ny) * kBrick;
     const int z1 = min(z0 + kBrick, a.nz);
     int x0 = 0, x1 = a.nx;
     if (!dense && !span(a, a.near, nullptr, 2, y, z0, lane, &x0, &x1)) return;
+    x0 = __builtin_amdgcn_readfirstlane(x0);
+    x1 = __builtin_amdgcn_readfirstlane(x1);
     const int ys[3] = {held(y, -1, a.ny), held(y, 0, a.ny), held(y, 1, a.ny)};
     const float rho0 = bits_f(a.params[RHO0]);
     for (int xb = x0; xb < x1; xb += kWave) {
         // lanes past the span's end recompute its last node (held x) and do not store
         const int x = min(xb + lane, x1 - 1);
         const int xs[3] = {held(x, -1, a.nx), held(x, 0, a.nx), held(x, 1, a.nx)};
-        float lo[9], mid[9], hi[9];
-        load_plane<kScaled>(a, src, held(z0, -1, a.nz), ys, xs, rho0, lo);
-        load_plane<kScaled>(a, src, held(z0, 0, a.nz), ys, xs, rho0, mid);
-        load_plane<kScaled>(a, src, held(z0, 1, a.nz), ys, xs, rho0, hi);
+        // every plane the layer reads, in flight at once: node z0 + d reads planes d, d + 1 and
+        // d + 2, plane k being held(z0 + k - 1) (in the grid for every k, so the loads past a
+        // short last layer are harmless)
+        float plane[kBrick + 2][9];
+#pragma unroll
+        for (int k = 0; k < kBrick + 2; ++k) load_plane<kScaled>(a, src, held(z0 + k, -1, a.nz), ys, xs, rho0, plane[k]);
 #pragma unroll
         for (int dz = 0; dz < kBrick; ++dz) {
             const int z = z0 + dz;
@@ -156,17 +162,9 @@ __global__ __launch_bounds__(kBlock) void blur_march_kernel(Mpm3Args a, const fl
                     for (int di = -1; di <= 1; di++) {
                         const float weight = (float)((2 - abs(di)) * (2 - abs(dj)) * (2 - abs(dk)) / 64.0);
                         const int t = 3 * (dj + 1) + (di + 1);
-                        sum = sum + weight * (dk < 0 ? lo[t] : (dk == 0 ? mid[t] : hi[t]));
+                        sum = sum + weight * plane[dz + dk + 1][t];
                     }
             if (xb + lane < x1) dst[((int64_t)z * a.ny + y) * a.nx + x] = sum;
-            if (z + 1 < z1) {
-#pragma unroll
-                for (int t = 0; t < 9; ++t) {
-                    lo[t] = mid[t];
-                    mid[t] = hi[t];
-                }
-                load_plane<kScaled>(a, src, held(z + 1, 1, a.nz), ys, xs, rho0, hi);
-            }
         }
     }
 }
@@ -219,11 +217,14 @@ __device__ __forceinline__ int component(int a, int k) { return a == k ? a : (a 
 __global__ __launch_bounds__(kBlock) void grid_tension_rows_kernel(Mpm3Args a, Stress st,
                                                                    const float* __restrict__ colour) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int64_t row = (int64_t)blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6);
+    // the wave's row and span held in scalar registers: the rows' addresses are then scalar
+    const int64_t row = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBlock / kWave) + (threadIdx.x >> 6)));
     if (row >= (int64_t)a.ny * a.nz) return;
     const int y = (int)(row % a.ny), z = (int)(row / a.ny);
     int x0, x1;
     if (!span(a, a.touched, a.live, 0xff, y, z, lane, &x0, &x1)) return;
+    x0 = __builtin_amdgcn_readfirstlane(x0);
+    x1 = __builtin_amdgcn_readfirstlane(x1);
     int64_t base[3][3];
     int ys[3], zs[3];
     row_bases(a, y, z, base, ys, zs);
@@ -232,18 +233,34 @@ __global__ __launch_bounds__(kBlock) void grid_tension_rows_kernel(Mpm3Args a, S
     for (int x = x0 + lane; x < x1; x += kWave) {
         const int64_t node = row * a.nx + x;
         const int c[3] = {x, y, z};
+        const int xm = held(x, -1, a.nx), xh = held(x, 0, a.nx), xp = held(x, 1, a.nx);
+        // the held neighbour across each face: [axis][sign < 0 ? 0 : 1]
+        const int64_t there[3][2] = {{base[1][1] + xm, base[1][1] + xp},
+                                     {base[1][0] + xh, base[1][2] + xh},
+                                     {base[0][1] + xh, base[2][1] + xh}};
+        // every value the node may use, read at once (all in the grid) rather than behind the
+        // tests that decide whether it is used: the sums, the colours, the node's six stress
+        // components and each face neighbour's three
         const float mass = a.g[0][node];
+        const float sums[3] = {a.g[1][node], a.g[2][node], a.g[3][node]};
+        const float here = colour[node];
+        float across[3][2], mine[6], theirs[3][2][3];
+#pragma unroll
+        for (int k = 0; k < 6; k++) mine[k] = st.t[k][node];
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++)
+#pragma unroll
+            for (int side = 0; side < 2; side++) {
+                across[ax][side] = colour[there[ax][side]];
+#pragma unroll
+                for (int k = 0; k < 3; k++) theirs[ax][side][k] = st.t[component(ax, k)][there[ax][side]];
+            }
         float vel[3] = {0.0f, 0.0f, 0.0f};
         if (mass > kEmpty) {
 #pragma unroll
-            for (int ax = 0; ax < 3; ax++) vel[ax] = a.g[1 + ax][node] / mass + dt * bits_f(a.params[GX + ax]);
-            const int xm = held(x, -1, a.nx), xh = held(x, 0, a.nx), xp = held(x, 1, a.nx);
-            // the held neighbour across each face: [axis][sign < 0 ? 0 : 1]
-            const int64_t there[3][2] = {{base[1][1] + xm, base[1][1] + xp},
-                                         {base[1][0] + xh, base[1][2] + xh},
-                                         {base[0][1] + xh, base[2][1] + xh}};
+            for (int ax = 0; ax < 3; ax++) vel[ax] = sums[ax] / mass + dt * bits_f(a.params[GX + ax]);
             float force[3] = {0.0f, 0.0f, 0.0f};
-            bool ring = colour[node] > kMinColour;
+            bool ring = here > kMinColour;
 #pragma unroll
             for (int ax = 0; ax < 3; ax++) ring = ring && !(c[ax] < kLow) && !(c[ax] > size[ax] - 1 - kLow);
 #pragma unroll
@@ -251,16 +268,14 @@ __global__ __launch_bounds__(kBlock) void grid_tension_rows_kernel(Mpm3Args a, S
 #pragma unroll
                 for (int side = 0; side < 2; side++) {
                     const int sign = side == 0 ? -1 : 1;
-                    const int64_t other = there[ax][side];
-                    const bool open = ring && colour[other] > kMinColour;
+                    const bool open = ring && across[ax][side] > kMinColour;
                     const bool wall = sign < 0 ? !(c[ax] > kLow) : !(c[ax] < size[ax] - 1 - kLow);
                     float shear = 1.0f;
                     if (wall) shear = 0.0f;
                     if (open) {
 #pragma unroll
                         for (int k = 0; k < 3; k++) {
-                            const float* comp = st.t[component(ax, k)];
-                            const float flux = (float)(0.5 * sign) * (comp[node] + comp[other]);
+                            const float flux = (float)(0.5 * sign) * (mine[component(ax, k)] + theirs[ax][side][k]);
                             force[k] = force[k] + (k == ax ? flux : shear * flux);
                         }
                     }
