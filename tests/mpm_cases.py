@@ -115,7 +115,7 @@ def case_scatter_conserves_3d(make3):
         mom = float(np.sum(f["m"].astype(np.float64) * f[v]))
         tot = float(np.sum(rig.read(g).astype(np.float64)))
         assert abs(mom - tot) <= 1e-3 * max(1, abs(mom)) + 1e-3, (a, mom, tot)
-    # The north star's bar: grid mass equal to the particles' to 1e-6 relative.
+    # Mass conservation: grid mass equal to the particles' to 1e-6 relative.
     assert abs(float(np.sum(rig.read("gm").astype(np.float64))) - mass) <= 1e-6 * mass
 
 

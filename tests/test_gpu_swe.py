@@ -96,8 +96,8 @@ def test_bit_exact_budget_runs_dry(gpu):
 
 
 def test_bit_exact_dam_and_bumps_512(gpu):
-    """The north star's shallow-water scene (circular dam plus 32 seeded bumps) at 512^2, 100
-    steps, against the oracle bit for bit."""
+    """The 4096^2 benchmark's shallow-water scene (circular dam plus 32 seeded bumps) at 512^2,
+    100 steps, against the oracle bit for bit."""
     h, hu, hv = scenes.swe_dam_bumps(512, seed=0)
     _parity(512, 512, (0, 0, 0, 0), h, hu, hv, 100, courant=0.45, dx=1.0 / 512)
 
