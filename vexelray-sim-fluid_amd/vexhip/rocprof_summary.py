@@ -32,16 +32,23 @@ _SHORT = {
     "starts_kernel": "starts2", "grid_tension_rows_kernel": "grid3_tension", "blur_march_kernel": "tension3_blur",
     "stress_rows_kernel": "tension3_stress", "region_kernel": "tension3_region", "lod_target_kernel": "lod3_target",
     "lod_kernel": "lod3", "after_sort_kernel": "mpm3_after_sort", "slots_kernel": "mpm3_slots",
-    "fill_zero_kernel": "hll_fill_zero", "f64_to_f32_kernel": "f64_to_f32", "batch2_flags": "batch2_flags",
+    "fill_zero_kernel": "hll_fill_zero", "f64_to_f32_kernel": "f64_to_f32", "batch2_flags_kernel": "batch2_flags",
 }
 
 
 def short_name(kernel: str) -> str:
-    """The library's timer name for a demangled kernel name (the names bench.py reports)."""
+    """The library's timer name for a demangled kernel name (the names bench.py reports): the
+    function's own name first (with, then without, its template arguments), then any key the
+    name contains."""
+    head = kernel.replace("(anonymous namespace)", "").split("(")[0]
+    base = head.split("::")[-1]
+    for candidate in (base, base.split("<")[0]):
+        if candidate in _SHORT:
+            return _SHORT[candidate]
     for key, name in _SHORT.items():
         if key in kernel:
             return name
-    return kernel.split("(")[0][-40:]
+    return head[-40:]
 
 
 def _counters(directory):
