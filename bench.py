@@ -105,7 +105,10 @@ def bench_hll(steps, n=4096):
 
     h, hu, hv = scenes.swe_dam_bumps(n, seed=0)
     sim = patch.PatchSimulation(n, n)
-    sim.load(h, hu, hv, patch.params(9.81, 0.45, 1.0 / n, 1.0 / n))
+    params = patch.params(9.81, 0.45, 1.0 / n, 1.0 / n)
+    sim.load(h, hu, hv, params)
+    rows = sim.sim.autotune(2)["rows"]  # rows a wave marches; the state is loaded again below
+    sim.load(h, hu, hv, params)
     sim.write_budget(2**31 - 1)
     sim.dispatch(10)
     sim.sim.finish()
@@ -116,7 +119,7 @@ def bench_hll(steps, n=4096):
     t1 = time.perf_counter()
     table = kernel_table(sim.sim, ["hll_step"], {"cell": n * n})
     sim.close()
-    return n * n * steps / (t1 - t0), table
+    return n * n * steps / (t1 - t0), table, rows
 
 
 def bench_batch2(steps, scenes=256, sort_every=SORT_EVERY, graph=True, tune=True):
@@ -225,7 +228,10 @@ def bench_surrogate(iters, n=1024):
     hll64_ms = ev[0].elapsed_time(ev[1]) / reps
     h, hu, hv = sc_mod.swe_dam_bumps(n, seed=0)
     sim = patch.PatchSimulation(n, n)
-    sim.load(h, hu, hv, patch.params(9.81, 0.45, 1.0 / n, 1.0 / n))
+    params = patch.params(9.81, 0.45, 1.0 / n, 1.0 / n)
+    sim.load(h, hu, hv, params)
+    hll_rows = sim.sim.autotune(2)["rows"]  # the HLL at its best launch shape for this size
+    sim.load(h, hu, hv, params)
     sim.write_budget(2**31 - 1)
     sim.dispatch(surrogate.STEPS)
     sim.sim.finish()
@@ -244,7 +250,7 @@ def bench_surrogate(iters, n=1024):
             "quality": quality,
             "surrogate_ms_per_application": round(fwd_ms, 4),
             "surrogate_cell_updates_per_s": round(cells / (fwd_ms * 1e-3), 1),
-            "hll_fp32_ms_per_8_steps": round(hll32_ms, 4),
+            "hll_fp32_ms_per_8_steps": round(hll32_ms, 4), "hll_fp32_rows": hll_rows,
             "hll_fp32_cell_updates_per_s": round(cells / (hll32_ms * 1e-3), 1),
             "hll_fp64_ms_per_8_steps": round(hll64_ms, 4),
             "surrogate_speed_vs_hll_fp32": round(hll32_ms / fwd_ms, 4),
@@ -392,9 +398,9 @@ def main():
                 "p2g_us": ltable.get("p2g3_seg", {}).get("avg_us"), "g2p_us": ltable.get("g2p3_fast", {}).get("avg_us")}
     if not args.no_hll:
         print(f"[bench] HLL 4096^2, {args.hll_steps} steps", file=sys.stderr, flush=True)
-        rate, htable = bench_hll(args.hll_steps)
+        rate, htable, hrows = bench_hll(args.hll_steps)
         hk = htable["hll_step"]
-        line["hll"] = {"value": round(rate, 1), "unit": "cell-updates/s", "steps": args.hll_steps,
+        line["hll"] = {"value": round(rate, 1), "unit": "cell-updates/s", "steps": args.hll_steps, "rows": hrows,
                        "workload": "HLL 4096^2 circular dam + 32 seeded bumps, walls, CFL 0.45",
                        "roofline": {"kernel": "hll_step", "bound": "hbm", "achieved": hk["achieved_gbs"],
                                     "peak": PEAK_HBM_GBS, "unit": "GB/s",
