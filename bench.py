@@ -262,33 +262,42 @@ def bench_surrogate(iters, n=1024):
 
 
 def cpu_baseline_mpm3(threads=None):
-    """The oracle on one whole substep of the same 8,388,608-particle workload: the scalar
-    restatement on one thread (about 2 s), and its threaded twin on the host's CPU share
-    (OMP_NUM_THREADS, 16 by default): the same arithmetic, P2G summed per thread."""
+    """The oracle on whole substeps of the same 8,388,608-particle workload, one warm-up substep
+    then the median of five: the scalar restatement on one thread (about 2 s a substep, some
+    13 s in all), and its threaded twin on the host's CPU share (OMP_NUM_THREADS, 16 by
+    default): the same arithmetic, P2G summed per thread."""
+    import statistics
+
     import oracle
     from vexhip import scenes
 
     s = scenes.mpm3_two_cubes()
+    count = len(s["x"])
     g, bulk, rho0 = scenes.mpm3_units(256)
     words = oracle.mpm3_params(5e-5, 0, g, 0, bulk, rho0)
     fields = {"x": s["x"], "y": s["y"], "z": s["z"], "m": s["m"]}
+
+    def median_substep(step):
+        step()  # warm-up: first touch of the grid (and of the threads' private boxes)
+        times = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            step()
+            times.append(time.perf_counter() - t0)
+        return statistics.median(times)
+
     ref = oracle.Mpm3(256, 256, 256, fields, words)
-    t0 = time.perf_counter()
-    ref.step(1)
-    t1 = time.perf_counter()
-    one = {"value": round(len(s["x"]) / (t1 - t0), 1), "unit": "particle-substeps/s", "cores": 1, "kind": "port",
-           "sample": f"one substep (clear, scatter, grid, advect) of the full {len(s['x'])}-particle 256^3 workload "
-                     f"by the scalar oracle, {t1 - t0:.2f} s", "seconds": round(t1 - t0, 3),
+    t = median_substep(lambda: ref.step(1))
+    one = {"value": round(count / t, 1), "unit": "particle-substeps/s", "cores": 1, "kind": "port",
+           "sample": f"substeps (clear, scatter, grid, advect) of the full {count}-particle 256^3 workload by the "
+                     f"scalar oracle: one warm-up, median of 5, {t:.2f} s", "seconds": round(t, 3),
            "host_cores": os.cpu_count()}
     threads = threads or int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
     par = oracle.Mpm3(256, 256, 256, fields, words)
-    par.step_omp(1, threads)  # warm: first touch of the grid and the private boxes
-    t0 = time.perf_counter()
-    par.step_omp(2, threads)
-    t1 = time.perf_counter()
-    many = {"value": round(2 * len(s["x"]) / (t1 - t0), 1), "unit": "particle-substeps/s", "cores": threads,
-            "kind": "port", "sample": f"two substeps of the same workload by the oracle's threaded step on {threads} "
-                                      f"threads, {(t1 - t0) / 2:.3f} s each", "seconds": round((t1 - t0) / 2, 4)}
+    t = median_substep(lambda: par.step_omp(1, threads))
+    many = {"value": round(count / t, 1), "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+            "sample": f"substeps of the same workload by the oracle's threaded step on {threads} threads: one "
+                      f"warm-up, median of 5, {t:.3f} s", "seconds": round(t, 4)}
     return one, many
 
 
@@ -422,8 +431,8 @@ def main():
               file=sys.stderr, flush=True)
         line["surrogate"] = bench_surrogate(args.surrogate_iters)
     if not args.no_cpu:
-        print("[bench] CPU oracle baseline (one substep on one thread, then on the host's CPU share)", file=sys.stderr,
-              flush=True)
+        print("[bench] CPU oracle baseline (median of 5 substeps on one thread, then on the host's CPU share)",
+              file=sys.stderr, flush=True)
         line["cpu_baseline"], line["cpu_baseline_threaded"] = cpu_baseline_mpm3()
     print(json.dumps(line), flush=True)
 
