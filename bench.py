@@ -286,12 +286,18 @@ def cpu_baseline_mpm3(threads=None):
             times.append(time.perf_counter() - t0)
         return statistics.median(times)
 
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
+    except OSError:
+        pass
     ref = oracle.Mpm3(256, 256, 256, fields, words)
     t = median_substep(lambda: ref.step(1))
     one = {"value": round(count / t, 1), "unit": "particle-substeps/s", "cores": 1, "kind": "port",
            "sample": f"substeps (clear, scatter, grid, advect) of the full {count}-particle 256^3 workload by the "
                      f"scalar oracle: one warm-up, median of 5, {t:.2f} s", "seconds": round(t, 3),
-           "host_cores": os.cpu_count()}
+           "host_cores": os.cpu_count(), "host_cpu": model}
     threads = threads or int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
     par = oracle.Mpm3(256, 256, 256, fields, words)
     t = median_substep(lambda: par.step_omp(1, threads))
