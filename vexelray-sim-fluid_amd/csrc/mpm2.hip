@@ -149,15 +149,23 @@ __global__ __launch_bounds__(kBlock) void p2g2_runs_kernel(Mpm2Args a, int run) 
     if (run_base >= 0) flush(run_base);
 }
 
+// dt and the two components of gravity, read from a scene's parameters.
+struct Law2 {
+    float dt, gx, gy;
+};
+
+__device__ __forceinline__ Law2 law2(const int32_t* params) {
+    return {bits_f(params[DT]), bits_f(params[GX]), bits_f(params[GY])};
+}
+
 // Flip.grid, Flip.java:366-412 (weight 1: no buoyancy), for node `node` of an nx x ny grid.
-__device__ __forceinline__ void grid2_node(int64_t node, int nx, const int32_t* params, float m, float mu, float mv,
+__device__ __forceinline__ void grid2_node(int64_t node, int nx, const Law2& law, float m, float mu, float mv,
                                            int ny, float* gu, float* gv) {
     const int ni = (int)(node % nx), nj = (int)(node / nx);
     float u = 0.0f, w = 0.0f;
     if (m > kEmpty) {
-        const float dt = bits_f(params[DT]);
-        u = mu / m + dt * (bits_f(params[GX]) * 1.0f);
-        w = mv / m + dt * (bits_f(params[GY]) * 1.0f);
+        u = mu / m + law.dt * (law.gx * 1.0f);
+        w = mv / m + law.dt * (law.gy * 1.0f);
     }
     if (!(ni > 1)) u = mx(u, 0.0f);
     if (ni > nx - 3) u = mn(u, 0.0f);
@@ -170,7 +178,7 @@ __device__ __forceinline__ void grid2_node(int64_t node, int nx, const int32_t* 
 __global__ __launch_bounds__(kBlock) void grid2_kernel(Mpm2Args a) {
     const int64_t node = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (node >= (int64_t)a.nx * a.ny) return;
-    grid2_node(node, a.nx, a.params, a.g[0][node], a.g[1][node], a.g[2][node], a.ny, a.g[3], a.g[4]);
+    grid2_node(node, a.nx, law2(a.params), a.g[0][node], a.g[1][node], a.g[2][node], a.ny, a.g[3], a.g[4]);
 }
 
 __device__ __forceinline__ void keep_inside(float& at, float& vel, int size) {
@@ -471,13 +479,14 @@ __global__ __launch_bounds__(kBlock) void batch2_grid_kernel(Batch2Args a) {
         const int64_t node = c * kBlock + threadIdx.x - sc.goff;
         if (node >= (int64_t)sc.nx * sc.ny) continue;
         const int64_t at = sc.goff + node;
+        const Law2 law = law2(a.params + 28 * (int64_t)s);  // read before the stores below
         const float m = dequantize(a.q[0][at], kMassBits);
         const float mu = dequantize(a.q[1][at], kMomentumBits);
         const float mv = dequantize(a.q[2][at], kMomentumBits);
         a.g[0][at] = m;
         a.g[1][at] = mu;
         a.g[2][at] = mv;
-        grid2_node(node, sc.nx, a.params + 28 * (int64_t)s, m, mu, mv, sc.ny, a.g[3] + sc.goff, a.g[4] + sc.goff);
+        grid2_node(node, sc.nx, law, m, mu, mv, sc.ny, a.g[3] + sc.goff, a.g[4] + sc.goff);
     }
 }
 

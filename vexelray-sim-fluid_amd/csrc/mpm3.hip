@@ -487,9 +487,9 @@ __device__ __forceinline__ void mark_one(const Mpm3Args& a, const Axis (&ax)[3])
 
 // One particle into this lane's run: its stencil, a flush when its base changes, its amounts.
 // `in_pass`: the particle has mass and lies in the brick this pass takes.
-__device__ __forceinline__ void take(const Mpm3Args& a, float4* tile, int* claim, const Slab& sl, bool in_pass,
-                                     float x, float y, float z, float m, float j, const float (&vel)[3],
-                                     const float (&c)[9], int& run_base, float (&acc)[108]) {
+__device__ __forceinline__ void take(const Mpm3Args& a, const float (&law)[3], float4* tile, int* claim,
+                                     const Slab& sl, bool in_pass, float x, float y, float z, float m, float j,
+                                     const float (&vel)[3], const float (&c)[9], int& run_base, float (&acc)[108]) {
     Axis ax[3] = {axis(x, a.nx), axis(y, a.ny), axis(z, a.nz)};
     const bool live = in_pass;
     const int base = ((ax[2].cell - sl.lo[2]) * sl.dims[1] + (ax[1].cell - sl.lo[1])) * sl.dims[0] +
@@ -498,7 +498,7 @@ __device__ __forceinline__ void take(const Mpm3Args& a, float4* tile, int* claim
     flush_run(tile, claim, sl, run_base, change && run_base >= 0, acc);
     if (change) run_base = base;
     if (live) {
-        const float dt = bits_f(a.params[DT]), bulk = bits_f(a.params[BULK]), rho0 = bits_f(a.params[RHO0]);
+        const float dt = law[0], bulk = law[1], rho0 = law[2];
         const float s = (4.0f * (dt * (m / rho0))) * (bulk * (1.0f - j));
         p2g_fold(ax, m, vel, c, s, acc);
     }
@@ -547,6 +547,8 @@ __global__ __launch_bounds__(kWave) void p2g3_wave_kernel(Mpm3Args a) {
     __shared__ int claim[kWaveTile];
     const int lane = threadIdx.x;
     if ((int64_t)blockIdx.x * kWave * R >= *a.active) return;  // a wave of inactive slots
+    // dt, bulk and rho0, read before the kernel stores anything, so once and into scalar registers
+    const float law[3] = {bits_f(a.params[DT]), bits_f(a.params[BULK]), bits_f(a.params[RHO0])};
     const int64_t first = ((int64_t)blockIdx.x * kWave + lane) * R;  // first + R <= capacity
     float f[17][R];
 #pragma unroll
@@ -587,8 +589,8 @@ __global__ __launch_bounds__(kWave) void p2g3_wave_kernel(Mpm3Args a) {
             float c[9];
 #pragma unroll
             for (int i = 0; i < 9; ++i) c[i] = f[8 + i][r];
-            take(a, tile, claim, sl, bk[r] != INT_MAX, f[0][r], f[1][r], f[2][r], f[6][r], f[7][r], vel, c, run_base,
-                 acc);
+            take(a, law, tile, claim, sl, bk[r] != INT_MAX, f[0][r], f[1][r], f[2][r], f[6][r], f[7][r], vel, c,
+                 run_base, acc);
         }
         flush_run(tile, claim, sl, run_base, run_base >= 0, acc);
         leave_tile(a, tile, sl);
@@ -620,7 +622,7 @@ __global__ __launch_bounds__(kWave) void p2g3_wave_kernel(Mpm3Args a) {
             float c[9];
 #pragma unroll
             for (int i = 0; i < 9; ++i) c[i] = a.p[8 + i][q];
-            take(a, tile, claim, sl, bk[r] == cur, a.p[0][q], a.p[1][q], a.p[2][q], a.p[6][q], a.p[7][q], vel, c,
+            take(a, law, tile, claim, sl, bk[r] == cur, a.p[0][q], a.p[1][q], a.p[2][q], a.p[6][q], a.p[7][q], vel, c,
                  run_base, acc);
         }
         flush_run(tile, claim, sl, run_base, run_base >= 0, acc);
@@ -628,7 +630,7 @@ __global__ __launch_bounds__(kWave) void p2g3_wave_kernel(Mpm3Args a) {
         cur = wave_min(min_of(bk, cur));
     }
     if (cur == INT_MAX) return;
-    const float dt = bits_f(a.params[DT]), bulk = bits_f(a.params[BULK]), rho0 = bits_f(a.params[RHO0]);
+    const float dt = law[0], bulk = law[1], rho0 = law[2];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int64_t q = first + r;
@@ -652,13 +654,13 @@ __global__ __launch_bounds__(kWave) void p2g3_wave_kernel(Mpm3Args a) {
 
 // Flip3.grid for node `node` at `at` with its sums.
 __device__ __forceinline__ void grid_node(const Mpm3Args& a, const int (&at)[3], float mass, const float (&mom)[3],
-                                          float dt, float (&vel)[3]) {
+                                          float dt, const float (&grav)[3], float (&vel)[3]) {
     const int size[3] = {a.nx, a.ny, a.nz};
 #pragma unroll
     for (int i = 0; i < 3; ++i) vel[i] = 0.0f;
     if (mass > kEmpty) {
 #pragma unroll
-        for (int i = 0; i < 3; ++i) vel[i] = mom[i] / mass + dt * bits_f(a.params[GX + i]);
+        for (int i = 0; i < 3; ++i) vel[i] = mom[i] / mass + dt * grav[i];
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -679,7 +681,9 @@ __global__ __launch_bounds__(256) void grid3_kernel(Mpm3Args a) {
     int x0, x1;
     if (!row_range(a, y, z, lane, &x0, &x1)) return;
     const int64_t first = row * a.nx;
+    // read before the pass stores anything (a store to the grid could alias them for the compiler)
     const float dt = bits_f(a.params[DT]);
+    const float grav[3] = {bits_f(a.params[GX]), bits_f(a.params[GX + 1]), bits_f(a.params[GX + 2])};
     if ((a.nx & 3) == 0) {
         for (int x = x0 + 4 * lane; x < x1; x += 4 * kWave) {
             float4 v[4];
@@ -695,7 +699,7 @@ __global__ __launch_bounds__(256) void grid3_kernel(Mpm3Args a) {
                 const int at[3] = {x + k, y, z};
                 const float mom[3] = {u4[k], w4[k], q4[k]};
                 float vel[3];
-                grid_node(a, at, m4[k], mom, dt, vel);
+                grid_node(a, at, m4[k], mom, dt, grav, vel);
 #pragma unroll
                 for (int i = 0; i < 3; ++i) out[i][k] = vel[i];
             }
@@ -709,7 +713,7 @@ __global__ __launch_bounds__(256) void grid3_kernel(Mpm3Args a) {
             const int at[3] = {x, y, z};
             const float mom[3] = {a.g[1][node], a.g[2][node], a.g[3][node]};
             float vel[3];
-            grid_node(a, at, a.g[0][node], mom, dt, vel);
+            grid_node(a, at, a.g[0][node], mom, dt, grav, vel);
 #pragma unroll
             for (int i = 0; i < 3; ++i) a.g[4 + i][node] = vel[i];
         }

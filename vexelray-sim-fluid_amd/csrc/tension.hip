@@ -229,7 +229,9 @@ __global__ __launch_bounds__(kBlock) void grid_tension_rows_kernel(Mpm3Args a, S
     int ys[3], zs[3];
     row_bases(a, y, z, base, ys, zs);
     const int size[3] = {a.nx, a.ny, a.nz};
-    const float dt = bits_f(a.params[DT]);
+    // the scene's constants, read before the loop's stores (which could alias them for the compiler)
+    const float dt = bits_f(a.params[DT]), sigma = bits_f(a.params[SIGMA]), rho0 = bits_f(a.params[RHO0]);
+    const float grav[3] = {bits_f(a.params[GX]), bits_f(a.params[GX + 1]), bits_f(a.params[GX + 2])};
     for (int x = x0 + lane; x < x1; x += kWave) {
         const int64_t node = row * a.nx + x;
         const int c[3] = {x, y, z};
@@ -258,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void grid_tension_rows_kernel(Mpm3Args a, S
         float vel[3] = {0.0f, 0.0f, 0.0f};
         if (mass > kEmpty) {
 #pragma unroll
-            for (int ax = 0; ax < 3; ax++) vel[ax] = sums[ax] / mass + dt * bits_f(a.params[GX + ax]);
+            for (int ax = 0; ax < 3; ax++) vel[ax] = sums[ax] / mass + dt * grav[ax];
             float force[3] = {0.0f, 0.0f, 0.0f};
             bool ring = here > kMinColour;
 #pragma unroll
@@ -281,7 +283,7 @@ __global__ __launch_bounds__(kBlock) void grid_tension_rows_kernel(Mpm3Args a, S
                     }
                 }
             }
-            const float push = (dt * bits_f(a.params[SIGMA])) / mx(mass, kMassFloor * bits_f(a.params[RHO0]));
+            const float push = (dt * sigma) / mx(mass, kMassFloor * rho0);
 #pragma unroll
             for (int ax = 0; ax < 3; ax++) vel[ax] = vel[ax] + push * force[ax];
         }
