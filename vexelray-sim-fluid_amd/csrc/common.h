@@ -22,6 +22,27 @@ __device__ __forceinline__ void atomic_add(float* p, float v) { unsafeAtomicAdd(
 
 constexpr int kWave = 64;
 
+// The least or greatest value over a wave's 64 lanes, all of them active, in every lane, without
+// going through LDS: DPP folds each row of 16 lanes (swap neighbours, swap pairs, mirror within 8,
+// mirror within 16), then the four rows' results are read into scalar registers.
+template <typename Op>
+__device__ __forceinline__ int wave_fold(int v, Op op) {
+    v = op(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    v = op(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    v = op(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = op(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false));  // row_mirror
+    return op(op(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+              op(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+    return wave_fold(v, [](int a, int b) { return a < b ? a : b; });
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+    return wave_fold(v, [](int a, int b) { return a > b ? a : b; });
+}
+
 // ------------------------------------------------------------------ launch plumbing
 
 struct Timer;  // event pairs per kernel name, owned by the runtime

@@ -323,16 +323,6 @@ __device__ __forceinline__ void add_fixed(unsigned long long* p, long long v) {
 // atomic each; otherwise straight to the grid. Integer sums: the same bits either way.
 constexpr int kTile2 = 2048;  // 3 x 8 B a node: 48 KB
 
-__device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
-    return v;
-}
-__device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
-    return v;
-}
 
 __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a, int run) {
     __shared__ unsigned long long tile[3][kTile2];

@@ -199,6 +199,21 @@ __global__ __launch_bounds__(kBlock) void p2g3_atomic_kernel(Mpm3Args a) {
 
 // ------------------------------------------------------------------ block bounding box
 
+// Min and max over a wave by ds_bpermute exchanges. The DPP folds (wave_min, wave_max) issue
+// fewer instructions but, where the wave holds its particles' fields, need more registers:
+// G2P went from 122 to 130 VGPRs (four waves a SIMD to three) and P2G spilled into AGPRs.
+__device__ __forceinline__ int lanes_min(int v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+    return v;
+}
+
+__device__ __forceinline__ int lanes_max(int v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+    return v;
+}
+
 struct Box {
     int lo[3], hi[3];
     int ex, ey, ez, nodes;
@@ -209,12 +224,9 @@ struct Box {
 // thread returns the same box; nodes = INT_MAX when the box's nodes exceed `cap`. Uses red[24].
 __device__ Box reduce_box(int (&lo)[3], int (&hi)[3], int* red, int cap) {
 #pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            lo[i] = min(lo[i], __shfl_xor(lo[i], off));
-            hi[i] = max(hi[i], __shfl_xor(hi[i], off));
-        }
+    for (int i = 0; i < 3; ++i) {
+        lo[i] = lanes_min(lo[i]);
+        hi[i] = lanes_max(hi[i]);
     }
     const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & (kWave - 1)) == 0) {
@@ -317,12 +329,9 @@ __device__ __forceinline__ void lds_order() { asm volatile("" ::: "memory"); }
 // The wave's box of base cells from each lane's lo/hi (empty: lo > hi); the same in every lane.
 __device__ __forceinline__ Box wave_box(int (&lo)[3], int (&hi)[3], int cap) {
 #pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            lo[i] = min(lo[i], __shfl_xor(lo[i], off));
-            hi[i] = max(hi[i], __shfl_xor(hi[i], off));
-        }
+    for (int i = 0; i < 3; ++i) {
+        lo[i] = lanes_min(lo[i]);
+        hi[i] = lanes_max(hi[i]);
     }
     Box b;
 #pragma unroll
@@ -365,7 +374,7 @@ __device__ __forceinline__ int brick_of_cell(const int (&c)[3], int nx, int ny) 
     return ((c[2] >> 3) * by + (c[1] >> 3)) * bx + (c[0] >> 3);
 }
 
-// The least of v[] above `after` (INT_MAX if none), in this lane; wave_min takes it over the wave.
+// The least of v[] above `after` (INT_MAX if none), in this lane; lanes_min takes it over the wave.
 template <int R>
 __device__ __forceinline__ int min_of(const int (&v)[R], int after) {
     int m = INT_MAX;
@@ -394,18 +403,6 @@ __device__ __forceinline__ int pick_run(const int (&v)[R], int r) {
     return out;
 }
 
-__device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
-    return v;
-}
-
-__device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
-    return v;
-}
-
 __device__ __forceinline__ void clear_tile(float4* tile, const Slab& sl) {
     lds_order();  // whatever the wave read of the tile before is done with
     for (int t = threadIdx.x; t < sl.nodes; t += kWave) tile[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -428,13 +425,10 @@ __device__ __forceinline__ void flush_run(float4* tile, int* claim, const Slab& 
         if (pending) claim[b] = lane;
         lds_order();
         const bool won = pending && claim[b] == lane;
-        int zlo = won ? layer : INT_MAX, zhi = won ? layer : INT_MIN;
-#pragma unroll
-        for (int off = kWave / 2; off > 0; off >>= 1) {
-            zlo = min(zlo, __shfl_xor(zlo, off));
-            zhi = max(zhi, __shfl_xor(zhi, off));
-        }
-        if (zlo == zhi) {
+        // a round has a winner (the last lane to write each claimed word); do all share its layer?
+        const uint64_t winners = __ballot(won);
+        const int layer0 = __builtin_amdgcn_readlane(layer, (int)__builtin_ctzll(winners));
+        if (__ballot(won && layer != layer0) == 0) {
             if (won) {
 #pragma unroll
                 for (int j = 0; j < 9; ++j) {
@@ -563,12 +557,12 @@ __global__ __launch_bounds__(kWave) void p2g3_wave_kernel(Mpm3Args a) {
         for (int i = 0; i < 3; ++i) c[i] = base_cell(f[i][r], size[i]);
         bk[r] = f[6][r] > 0.0f ? brick_of_cell(c, a.nx, a.ny) : INT_MAX;
     }
-    const int first_brick = wave_min(min_of(bk, INT_MIN));
+    const int first_brick = lanes_min(min_of(bk, INT_MIN));
     if (first_brick == INT_MAX) return;
     float acc[108];
 #pragma unroll
     for (int i = 0; i < 108; ++i) acc[i] = 0.0f;
-    if (wave_max(max_of(bk)) == first_brick) {
+    if (lanes_max(max_of(bk)) == first_brick) {
         // the common case: one brick, the particles from the registers they were loaded into
         int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
 #pragma unroll
@@ -627,7 +621,7 @@ __global__ __launch_bounds__(kWave) void p2g3_wave_kernel(Mpm3Args a) {
         }
         flush_run(tile, claim, sl, run_base, run_base >= 0, acc);
         leave_tile(a, tile, sl);
-        cur = wave_min(min_of(bk, cur));
+        cur = lanes_min(min_of(bk, cur));
     }
     if (cur == INT_MAX) return;
     const float dt = law[0], bulk = law[1], rho0 = law[2];
