@@ -103,6 +103,18 @@ __device__ __forceinline__ void mark_bricks(const Mpm3Args& a, const int (&lo)[3
     const int b0[3] = {lo[0] / kBrick, lo[1] / kBrick, lo[2] / kBrick};
     const int e[3] = {hi[0] / kBrick - b0[0] + 1, hi[1] / kBrick - b0[1] + 1, hi[2] / kBrick - b0[2] + 1};
     const int total = e[0] * e[1] * e[2];
+    if (e[0] <= 2 && e[1] <= 2 && e[2] <= 2) {
+        // every caller's box: a stencil or one brick's stencils, at most 10 nodes an axis, so at
+        // most two bricks an axis and eight in all; brick t's x and y offsets are bits of t
+        const int sx = e[0] - 1, sy = e[1] - 1;
+#pragma unroll 1
+        for (int t = lane; t < total; t += lanes) {
+            const int bx = b0[0] + (t & sx), by = b0[1] + ((t >> sx) & sy), bz = b0[2] + (t >> (sx + sy));
+            a.touched[((int64_t)bz * byn + by) * bxn + bx] = 1;
+        }
+        return;
+    }
+#pragma unroll 1
     for (int t = lane; t < total; t += lanes) {
         const int bx = b0[0] + t % e[0], by = b0[1] + (t / e[0]) % e[1], bz = b0[2] + t / (e[0] * e[1]);
         a.touched[((int64_t)bz * byn + by) * bxn + bx] = 1;
@@ -514,10 +526,16 @@ __device__ __forceinline__ Slab enter_tile(const Mpm3Args& a, float4* tile, int 
 __device__ __forceinline__ void leave_tile(const Mpm3Args& a, const float4* tile, const Slab& sl) {
     lds_order();
     const int ex = sl.dims[0], ey = sl.dims[1];
+    // a node's (x, y, z) in the tile from its number t by float reciprocals: t < 2^12, so t + 1/2
+    // lies at least 1/(2 ex) from a multiple of ex and the rounded product floors exactly
+    const float inv_ex = 1.0f / (float)ex, inv_ey = 1.0f / (float)ey;
     for (int t = threadIdx.x; t < sl.nodes; t += kWave) {
         const float4 v = tile[t];
         if (v.x == 0.0f) continue;  // a node no live particle weighs on holds nothing
-        const int lx = t % ex, ly = (t / ex) % ey, lz = t / (ex * ey);
+        const int row = (int)(((float)t + 0.5f) * inv_ex);
+        const int lx = t - row * ex;
+        const int lz = (int)(((float)row + 0.5f) * inv_ey);
+        const int ly = row - lz * ey;
         const int64_t node = ((int64_t)(sl.lo[2] + lz) * a.ny + (sl.lo[1] + ly)) * a.nx + (sl.lo[0] + lx);
         atomic_add(a.g[0] + node, v.x);
         atomic_add(a.g[1] + node, v.y);
