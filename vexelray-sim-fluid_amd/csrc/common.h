@@ -43,6 +43,24 @@ __device__ __forceinline__ int wave_max(int v) {
     return wave_fold(v, [](int a, int b) { return a > b ? a : b; });
 }
 
+// R consecutive floats from i (i a multiple of R, the arrays 256-byte aligned): one load.
+template <int R>
+__device__ __forceinline__ void load_run(const float* __restrict__ p, int64_t i, float (&v)[R]) {
+    if constexpr (R == 4) {
+        const float4 t = *reinterpret_cast<const float4*>(p + i);
+        v[0] = t.x;
+        v[1] = t.y;
+        v[2] = t.z;
+        v[3] = t.w;
+    } else if constexpr (R == 2) {
+        const float2 t = *reinterpret_cast<const float2*>(p + i);
+        v[0] = t.x;
+        v[1] = t.y;
+    } else {
+        v[0] = p[i];
+    }
+}
+
 // ------------------------------------------------------------------ launch plumbing
 
 struct Timer;  // event pairs per kernel name, owned by the runtime

@@ -51,12 +51,13 @@ __device__ __forceinline__ Axis axis(float pos, int n) {
 
 // The 27 amounts of one particle (Flip.java:283-300), node k's three at 3k .. 3k+2, handed to
 // `take(k, mass, momentum_u, momentum_v)`.
+// `v` holds the particle's ten fields (x, y, u, v, m, J, C00, C01, C10, C11).
 template <class Take>
-__device__ __forceinline__ void amounts(float* const* p, int64_t q, const Axis& ax, const Axis& ay, float dt,
-                                        float bulk, float rho0, Take take) {
-    const float m = p[4][q], u = p[2][q], w = p[3][q];
-    const float c00 = p[6][q], c01 = p[7][q], c10 = p[8][q], c11 = p[9][q];
-    const float s = (4.0f * (dt * (m / rho0))) * (bulk * (1.0f - p[5][q]));
+__device__ __forceinline__ void amounts(const float (&v)[10], const Axis& ax, const Axis& ay, float dt, float bulk,
+                                        float rho0, Take take) {
+    const float m = v[4], u = v[2], w = v[3];
+    const float c00 = v[6], c01 = v[7], c10 = v[8], c11 = v[9];
+    const float s = (4.0f * (dt * (m / rho0))) * (bulk * (1.0f - v[5]));
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const float wgt = ax.w[k % 3] * ay.w[k / 3];
@@ -71,7 +72,10 @@ __device__ __forceinline__ void amounts(float* const* p, int64_t q, const Axis& 
 
 __device__ __forceinline__ void amounts(float* const* p, int64_t q, const Axis& ax, const Axis& ay, float dt,
                                         float bulk, float rho0, float (&acc)[27]) {
-    amounts(p, q, ax, ay, dt, bulk, rho0, [&](int k, float a0, float a1, float a2) {
+    float v[10];
+#pragma unroll
+    for (int f = 0; f < 10; ++f) v[f] = p[f][q];
+    amounts(v, ax, ay, dt, bulk, rho0, [&](int k, float a0, float a1, float a2) {
         acc[3 * k] += a0;
         acc[3 * k + 1] += a1;
         acc[3 * k + 2] += a2;
@@ -324,11 +328,12 @@ __device__ __forceinline__ void add_fixed(unsigned long long* p, long long v) {
 constexpr int kTile2 = 2048;  // 3 x 8 B a node: 48 KB
 
 
-__global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a, int run) {
+template <int kRun>
+__global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a) {
     __shared__ unsigned long long tile[3][kTile2];
     __shared__ int box[4];
     const float mscale = 0x1p40f, pscale = 0x1p32f;
-    const int passes = kBatchAlign / (kBlock * run);
+    constexpr int passes = kBatchAlign / (kBlock * kRun);
     for (int64_t c = blockIdx.x; c < a.pchunks; c += gridDim.x) {
         const int s = a.pscene[c];
         const Scene2 sc = a.scenes[s];
@@ -382,7 +387,12 @@ __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a, int ru
         unsigned long long* q1 = a.q[1] + sc.goff;
         unsigned long long* q2 = a.q[2] + sc.goff;
         for (int pass = 0; pass < passes; ++pass) {
-            const int64_t first = c * kBatchAlign + ((int64_t)pass * kBlock + threadIdx.x) * run;
+            const int64_t first = c * kBatchAlign + ((int64_t)pass * kBlock + threadIdx.x) * kRun;
+            // the run's fields, every load in flight at once: kRun-wide loads, inside the chunk
+            // (the scene's range is padded to whole chunks, its slots past `count` massless)
+            float f[10][kRun];
+#pragma unroll
+            for (int k = 0; k < 10; ++k) load_run<kRun>(a.p[k], first, f[k]);
             double acc[27];  // the run's fixed-point sums, exact integers in double
 #pragma unroll
             for (int i = 0; i < 27; ++i) acc[i] = 0.0;
@@ -407,16 +417,19 @@ __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a, int ru
 #pragma unroll
                 for (int i = 0; i < 27; ++i) acc[i] = 0.0;
             };
-            for (int r = 0; r < run; ++r) {
-                const int64_t q = first + r;
-                if (q >= live) break;
-                const Axis ax = axis(a.p[0][q], sc.nx), ay = axis(a.p[1][q], sc.ny);
+#pragma unroll
+            for (int r = 0; r < kRun; ++r) {
+                if (first + r >= live) break;
+                float v[10];
+#pragma unroll
+                for (int k = 0; k < 10; ++k) v[k] = f[k][r];
+                const Axis ax = axis(v[0], sc.nx), ay = axis(v[1], sc.ny);
                 if (ax.cell != rx || ay.cell != ry) {
                     if (rx >= 0) flush();
                     rx = ax.cell;
                     ry = ay.cell;
                 }
-                amounts(a.p, q, ax, ay, dt, bulk, rho0, [&](int k, float m, float mu, float mv) {
+                amounts(v, ax, ay, dt, bulk, rho0, [&](int k, float m, float mu, float mv) {
                     acc[3 * k] += quantize_d(m, mscale);
                     acc[3 * k + 1] += quantize_d(mu, pscale);
                     acc[3 * k + 2] += quantize_d(mv, pscale);
@@ -723,7 +736,12 @@ int batch2_scatter(const Launch& l, const Batch2Args& a, int run, int blocks) {
     const unsigned g = grid_of(a.pchunks, blocks);
     if (!g) return 0;
     timed_begin(l, "batch2_p2g");
-    hipLaunchKernelGGL(batch2_p2g_kernel, dim3(g), dim3(kBlock), 0, l.stream, a, run);
+    if (run == 4)
+        hipLaunchKernelGGL(batch2_p2g_kernel<4>, dim3(g), dim3(kBlock), 0, l.stream, a);
+    else if (run == 2)
+        hipLaunchKernelGGL(batch2_p2g_kernel<2>, dim3(g), dim3(kBlock), 0, l.stream, a);
+    else
+        hipLaunchKernelGGL(batch2_p2g_kernel<1>, dim3(g), dim3(kBlock), 0, l.stream, a);
     timed_end(l, "batch2_p2g");
     return check_launch("batch2_p2g");
 }

@@ -271,24 +271,6 @@ __device__ Box reduce_box(int (&lo)[3], int (&hi)[3], int* red, int cap) {
     return b;
 }
 
-// R consecutive floats from i (i a multiple of R, the arrays 256-byte aligned): one load.
-template <int R>
-__device__ __forceinline__ void load_run(const float* __restrict__ p, int64_t i, float (&v)[R]) {
-    if constexpr (R == 4) {
-        const float4 t = *reinterpret_cast<const float4*>(p + i);
-        v[0] = t.x;
-        v[1] = t.y;
-        v[2] = t.z;
-        v[3] = t.w;
-    } else if constexpr (R == 2) {
-        const float2 t = *reinterpret_cast<const float2*>(p + i);
-        v[0] = t.x;
-        v[1] = t.y;
-    } else {
-        v[0] = p[i];
-    }
-}
-
 // ------------------------------------------------------------------ P2G, sorted-segmented
 
 // Node (i, j, l) of a particle's stencil, k = i + 3j + 9l, receives mass w m and momentum
