@@ -321,7 +321,8 @@ __device__ __forceinline__ void add_fixed(unsigned long long* p, long long v) {
 }
 
 // P2G in fixed point. A block takes a chunk of kBatchAlign particles of one scene; a thread
-// folds `run` consecutive particles sharing a stencil base into 27 integer sums. When the
+// loads four consecutive ones, which give the chunk's box and then fold, kRun at a time, while
+// they share a stencil base, into 27 integer sums. When the
 // chunk's stencils fit kTile2 nodes (sorted particles: a few rows of cells) the sums go to an
 // LDS tile by 64-bit LDS atomics, and the tile's touched nodes to the grid by one 64-bit global
 // atomic each; otherwise straight to the grid. Integer sums: the same bits either way.
@@ -333,13 +334,20 @@ __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a) {
     __shared__ unsigned long long tile[3][kTile2];
     __shared__ int box[4];
     const float mscale = 0x1p40f, pscale = 0x1p32f;
-    constexpr int passes = kBatchAlign / (kBlock * kRun);
+    constexpr int kOwn = kBatchAlign / kBlock;  // a thread's particles: 4, consecutive
+    static_assert(kOwn == 4 && kOwn % kRun == 0, "a thread's particles split into whole runs");
     for (int64_t c = blockIdx.x; c < a.pchunks; c += gridDim.x) {
         const int s = a.pscene[c];
         const Scene2 sc = a.scenes[s];
         const int32_t* params = a.params + 28 * (int64_t)s;
         const float dt = bits_f(params[DT]), bulk = bits_f(params[BULK]), rho0 = bits_f(params[RHO0]);
         const int64_t live = sc.poff + sc.count;  // the chunk's live particles are a prefix of it
+        // the thread's particles, every field with one float4 load, all in flight at once (the
+        // chunk lies inside its scene's range, padded to whole chunks with massless slots)
+        const int64_t first = c * kBatchAlign + (int64_t)threadIdx.x * kOwn;
+        float f[10][kOwn];
+#pragma unroll
+        for (int k = 0; k < 10; ++k) load_run<kOwn>(a.p[k], first, f[k]);
 
         // the chunk's box of stencil bases
         if (threadIdx.x == 0) {
@@ -348,10 +356,10 @@ __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a) {
         }
         __syncthreads();
         int x0 = 0x7fffffff, y0 = 0x7fffffff, x1 = -1, y1 = -1;
-        for (int i = threadIdx.x; i < kBatchAlign; i += kBlock) {
-            const int64_t q = c * kBatchAlign + i;
-            if (q >= live) break;
-            const int bx = axis(a.p[0][q], sc.nx).cell, by = axis(a.p[1][q], sc.ny).cell;
+#pragma unroll
+        for (int r = 0; r < kOwn; ++r) {
+            if (first + r >= live) break;
+            const int bx = axis(f[0][r], sc.nx).cell, by = axis(f[1][r], sc.ny).cell;
             x0 = min(x0, bx);
             y0 = min(y0, by);
             x1 = max(x1, bx);
@@ -386,13 +394,8 @@ __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a) {
         unsigned long long* q0 = a.q[0] + sc.goff;
         unsigned long long* q1 = a.q[1] + sc.goff;
         unsigned long long* q2 = a.q[2] + sc.goff;
-        for (int pass = 0; pass < passes; ++pass) {
-            const int64_t first = c * kBatchAlign + ((int64_t)pass * kBlock + threadIdx.x) * kRun;
-            // the run's fields, every load in flight at once: kRun-wide loads, inside the chunk
-            // (the scene's range is padded to whole chunks, its slots past `count` massless)
-            float f[10][kRun];
 #pragma unroll
-            for (int k = 0; k < 10; ++k) load_run<kRun>(a.p[k], first, f[k]);
+        for (int pass = 0; pass < kOwn / kRun; ++pass) {
             double acc[27];  // the run's fixed-point sums, exact integers in double
 #pragma unroll
             for (int i = 0; i < 27; ++i) acc[i] = 0.0;
@@ -418,7 +421,7 @@ __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a) {
                 for (int i = 0; i < 27; ++i) acc[i] = 0.0;
             };
 #pragma unroll
-            for (int r = 0; r < kRun; ++r) {
+            for (int r = pass * kRun; r < (pass + 1) * kRun; ++r) {
                 if (first + r >= live) break;
                 float v[10];
 #pragma unroll
