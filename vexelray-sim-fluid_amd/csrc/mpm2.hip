@@ -205,11 +205,8 @@ struct G2p2In {
     float gu[9], gv[9];
 };
 
-__device__ __forceinline__ void g2p2_gather(float* const* p, int64_t q, int nx, int ny, const float* gu,
-                                            const float* gv, G2p2In& in) {
-    in.x = p[0][q];
-    in.y = p[1][q];
-    in.j = p[5][q];
+// The stencil's node velocities for a particle whose position and J are in `in` already.
+__device__ __forceinline__ void g2p2_nodes(int nx, int ny, const float* gu, const float* gv, G2p2In& in) {
     in.ax = axis(in.x, nx);
     in.ay = axis(in.y, ny);
     const int64_t base = (int64_t)in.ay.cell * nx + in.ax.cell;
@@ -221,11 +218,18 @@ __device__ __forceinline__ void g2p2_gather(float* const* p, int64_t q, int nx, 
     }
 }
 
+__device__ __forceinline__ void g2p2_gather(float* const* p, int64_t q, int nx, int ny, const float* gu,
+                                            const float* gv, G2p2In& in) {
+    in.x = p[0][q];
+    in.y = p[1][q];
+    in.j = p[5][q];
+    g2p2_nodes(nx, ny, gu, gv, in);
+}
+
 // Flip.advectBody, Flip.java:494-545 (not relaxing), for particle q of one scene, from what
 // g2p2_gather read. Reads and stores are split so a thread can have several particles' reads
 // in flight before its first store (a read cannot be moved past a store that may alias it).
-__device__ __forceinline__ void g2p2_finish(float* const* p, int64_t q, int nx, int ny, const int32_t* params,
-                                            const G2p2In& in) {
+__device__ __forceinline__ void g2p2_finish(float* const* p, int64_t q, int nx, int ny, float dt, const G2p2In& in) {
     const Axis& ax = in.ax;
     const Axis& ay = in.ay;
     float u = 0, w = 0, b00 = 0, b01 = 0, b10 = 0, b11 = 0;
@@ -244,7 +248,6 @@ __device__ __forceinline__ void g2p2_finish(float* const* p, int64_t q, int nx, 
         b11 = b11 + vi * dy;
     }
     const float c00 = 4.0f * b00, c01 = 4.0f * b01, c10 = 4.0f * b10, c11 = 4.0f * b11;
-    const float dt = bits_f(params[DT]);
     float x = in.x + dt * u;
     float y = in.y + dt * w;
     keep_inside(x, u, nx);
@@ -264,8 +267,9 @@ __device__ __forceinline__ void g2p2_finish(float* const* p, int64_t q, int nx, 
 __device__ __forceinline__ void g2p2_particle(float* const* p, int64_t q, int nx, int ny, const float* gu,
                                               const float* gv, const int32_t* params) {
     G2p2In in;
+    const float dt = bits_f(params[DT]);
     g2p2_gather(p, q, nx, ny, gu, gv, in);
-    g2p2_finish(p, q, nx, ny, params, in);
+    g2p2_finish(p, q, nx, ny, dt, in);
 }
 
 __global__ __launch_bounds__(kBlock) void g2p2_kernel(Mpm2Args a) {
@@ -501,20 +505,30 @@ __global__ __launch_bounds__(kBlock) void batch2_grid_kernel(Batch2Args a) {
 __global__ __launch_bounds__(kBlock) void batch2_g2p_kernel(Batch2Args a) {
     constexpr int kPer = kBatchAlign / kBlock;
     for (int64_t pc = blockIdx.x; pc < a.pchunks; pc += gridDim.x) {
-        const int s = a.pscene[pc];
-        const Scene2 sc = a.scenes[s];
-        const float* gu = a.g[3] + sc.goff;
-        const float* gv = a.g[4] + sc.goff;
+        // positions and J first, beside the scene's number: the chunk lies inside its scene's
+        // range (padded to whole chunks), so these reads need nothing from the scene
         G2p2In in[kPer];
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int64_t q = pc * kBatchAlign + k * kBlock + threadIdx.x;
-            if (q - sc.poff < sc.count) g2p2_gather(a.p, q, sc.nx, sc.ny, gu, gv, in[k]);
+            in[k].x = a.p[0][q];
+            in[k].y = a.p[1][q];
+            in[k].j = a.p[5][q];
+        }
+        const int s = a.pscene[pc];
+        const Scene2 sc = a.scenes[s];
+        const float* gu = a.g[3] + sc.goff;
+        const float* gv = a.g[4] + sc.goff;
+        const float dt = bits_f(a.params[28 * (int64_t)s + DT]);  // read before the stores below
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int64_t q = pc * kBatchAlign + k * kBlock + threadIdx.x;
+            if (q - sc.poff < sc.count) g2p2_nodes(sc.nx, sc.ny, gu, gv, in[k]);
         }
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int64_t q = pc * kBatchAlign + k * kBlock + threadIdx.x;
-            if (q - sc.poff < sc.count) g2p2_finish(a.p, q, sc.nx, sc.ny, a.params + 28 * (int64_t)s, in[k]);
+            if (q - sc.poff < sc.count) g2p2_finish(a.p, q, sc.nx, sc.ny, dt, in[k]);
         }
     }
 }
@@ -527,16 +541,16 @@ __global__ __launch_bounds__(kBlock) void batch2_keys_kernel(Batch2Args a, int32
                                                              int32_t* __restrict__ index) {
     constexpr int kPer = kBatchAlign / kBlock;
     for (int64_t pc = blockIdx.x; pc < a.pchunks; pc += gridDim.x) {
-        const int s = a.pscene[pc];
-        const Scene2 sc = a.scenes[s];
+        // positions beside the scene's number: the chunk lies inside its scene's padded range
         float x[kPer], y[kPer];
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int64_t q = pc * kBatchAlign + k * kBlock + threadIdx.x;
-            const bool real = q - sc.poff < sc.count;
-            x[k] = real ? a.p[0][q] : 0.0f;
-            y[k] = real ? a.p[1][q] : 0.0f;
+            x[k] = a.p[0][q];
+            y[k] = a.p[1][q];
         }
+        const int s = a.pscene[pc];
+        const Scene2 sc = a.scenes[s];
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int64_t q = pc * kBatchAlign + k * kBlock + threadIdx.x;
