@@ -341,17 +341,18 @@ __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a) {
     constexpr int kOwn = kBatchAlign / kBlock;  // a thread's particles: 4, consecutive
     static_assert(kOwn == 4 && kOwn % kRun == 0, "a thread's particles split into whole runs");
     for (int64_t c = blockIdx.x; c < a.pchunks; c += gridDim.x) {
+        // the thread's particles, every field with one float4 load, all in flight at once and
+        // beside the scene's number (the chunk lies inside its scene's range, padded to whole
+        // chunks with massless slots, so these reads need nothing from the scene)
+        const int64_t first = c * kBatchAlign + (int64_t)threadIdx.x * kOwn;
+        float f[10][kOwn];
+#pragma unroll
+        for (int k = 0; k < 10; ++k) load_run<kOwn>(a.p[k], first, f[k]);
         const int s = a.pscene[c];
         const Scene2 sc = a.scenes[s];
         const int32_t* params = a.params + 28 * (int64_t)s;
         const float dt = bits_f(params[DT]), bulk = bits_f(params[BULK]), rho0 = bits_f(params[RHO0]);
         const int64_t live = sc.poff + sc.count;  // the chunk's live particles are a prefix of it
-        // the thread's particles, every field with one float4 load, all in flight at once (the
-        // chunk lies inside its scene's range, padded to whole chunks with massless slots)
-        const int64_t first = c * kBatchAlign + (int64_t)threadIdx.x * kOwn;
-        float f[10][kOwn];
-#pragma unroll
-        for (int k = 0; k < 10; ++k) load_run<kOwn>(a.p[k], first, f[k]);
 
         // the chunk's box of stencil bases
         if (threadIdx.x == 0) {
@@ -578,17 +579,17 @@ __global__ __launch_bounds__(kBlock) void batch2_bounds_kernel(Batch2Args a, con
     constexpr int kPer = kBatchAlign / kBlock;
     const uint32_t mask = (1u << a.cell_bits) - 1u;
     for (int64_t pc = blockIdx.x; pc < a.pchunks; pc += gridDim.x) {
-        const Scene2 sc = a.scenes[a.pscene[pc]];  // a particle chunk lies in one scene
-        const int64_t c0 = sc.soff / kBlock;
+        // the keys beside the scene lookup: the chunk lies inside its scene's padded range
         const int64_t base = pc * kBatchAlign + threadIdx.x;
         int32_t key[kPer], before[kPer];
 #pragma unroll
         for (int j = 0; j < kPer; ++j) {
             const int64_t q = base + j * kBlock;
-            const bool real = q - sc.poff < sc.count;
-            key[j] = real ? keys[q] : 0;
-            before[j] = real && q > sc.poff ? keys[q - 1] : 0;
+            key[j] = keys[q];
+            before[j] = q > 0 ? keys[q - 1] : 0;  // used for a scene's particles after its first
         }
+        const Scene2 sc = a.scenes[a.pscene[pc]];  // a particle chunk lies in one scene
+        const int64_t c0 = sc.soff / kBlock;
 #pragma unroll
         for (int j = 0; j < kPer; ++j) {
             const int64_t q = base + j * kBlock;
