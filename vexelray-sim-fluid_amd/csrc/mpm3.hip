@@ -387,16 +387,6 @@ __device__ __forceinline__ int max_of(const int (&v)[R]) {
     return m;
 }
 
-// v[r] for a run-time r without indexing a local array (which would put it in scratch memory).
-template <int R>
-__device__ __forceinline__ int pick_run(const int (&v)[R], int r) {
-    int out = v[0];
-#pragma unroll
-    for (int k = 1; k < R; ++k)
-        if (r == k) out = v[k];
-    return out;
-}
-
 __device__ __forceinline__ void clear_tile(float4* tile, const Slab& sl) {
     lds_order();  // whatever the wave read of the tile before is done with
     for (int t = threadIdx.x; t < sl.nodes; t += kWave) tile[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
