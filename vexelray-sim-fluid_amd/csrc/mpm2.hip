@@ -320,6 +320,15 @@ __global__ __launch_bounds__(kBlock) void starts_kernel(const int32_t* __restric
 // each scene gets exactly the arithmetic a single-scene run gives it: the P2G is the
 // fixed-point one, whose integer sums do not depend on the order or grouping of the atomics.
 
+// An integer held exactly in a double (|v| < 2^53) as a 64-bit integer: the high word
+// floor(v / 2^32) and the low word v - 2^32 high, both exact. No rounding step is needed, since
+// v is an integer already (a general conversion rounds first).
+__device__ __forceinline__ long long exact_ll(double v) {
+    const double hi = floor(v * 0x1p-32);
+    const double lo = fma(hi, -0x1p32, v);  // in [0, 2^32)
+    return (long long)(((unsigned long long)(unsigned int)(int)hi << 32) | (unsigned int)lo);
+}
+
 __device__ __forceinline__ void add_fixed(unsigned long long* p, long long v) {
     atomicAdd(p, (unsigned long long)v);
 }
@@ -408,8 +417,8 @@ __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a) {
             auto flush = [&]() {
 #pragma unroll
                 for (int k = 0; k < 9; ++k) {
-                    const long long s0 = __double2ll_rn(acc[3 * k]), s1 = __double2ll_rn(acc[3 * k + 1]),
-                                    s2 = __double2ll_rn(acc[3 * k + 2]);
+                    const long long s0 = exact_ll(acc[3 * k]), s1 = exact_ll(acc[3 * k + 1]),
+                                    s2 = exact_ll(acc[3 * k + 2]);
                     if (tiled) {
                         const int at = (ry - by0 + k / 3) * w + (rx - bx0 + k % 3);
                         atomicAdd(&tile[0][at], (unsigned long long)s0);
