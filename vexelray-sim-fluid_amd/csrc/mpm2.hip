@@ -446,10 +446,16 @@ __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a) {
                     rx = ax.cell;
                     ry = ay.cell;
                 }
+                // The amounts with the mass scaled by pscale = 2^32: every amount is a product,
+                // quotient or sum of binary32 values with the mass as a factor, so each comes out
+                // scaled by 2^32 exactly, and the momentum's quantisation needs no multiply (the
+                // mass's, 2^40, is 2^8 more). Exact while nothing leaves the normal range; a
+                // product small enough to lose bits there quantises to zero either way.
+                v[4] = v[4] * pscale;
                 amounts(v, ax, ay, dt, bulk, rho0, [&](int k, float m, float mu, float mv) {
-                    acc[3 * k] += quantize_d(m, mscale);
-                    acc[3 * k + 1] += quantize_d(mu, pscale);
-                    acc[3 * k + 2] += quantize_d(mv, pscale);
+                    acc[3 * k] += quantize_d(m, mscale / pscale);
+                    acc[3 * k + 1] += (double)rintf(mu);
+                    acc[3 * k + 2] += (double)rintf(mv);
                 });
             }
             if (rx >= 0) flush();
