@@ -183,7 +183,7 @@ def test_autotune_keeps_results(gpu):
         b.load(sc, params)
     chosen = tuned.sim.autotune(2)
     assert chosen["run"] in (1, 2, 4) and 4 <= chosen["radix_bits"] <= 8
-    assert chosen["blocks"] in (0, 1024, 2048, 4096, 8192)
+    assert chosen["blocks"] in (0, 1024, 2048, 4096, 8192, 16384)
     for b in (plain, tuned):
         b.advance(22, 10)
     for name in FIELDS:

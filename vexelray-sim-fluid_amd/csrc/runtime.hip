@@ -659,7 +659,7 @@ int autotune(vxr_sim* s, int reps) {
     int rc = 0;
     if (opt(s, "p2g") >= 1) rc = tune_option(s, "run", {1, 2, 4}, reps, scatter);
     if (!rc && fixed_p2g(s))
-        rc = tune_option(s, "blocks", {0, 1024, 2048, 4096, 8192}, reps, [&]() {
+        rc = tune_option(s, "blocks", {0, 1024, 2048, 4096, 8192, 16384}, reps, [&]() {
             for (const char* p : kStep)
                 if (p != std::string("advect"))
                     if (int r = run_pass(s, p)) return r;
