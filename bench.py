@@ -25,7 +25,7 @@ SORT_EVERY = 10
 # Algorithmic bytes per unit (particle, node, element) of each kernel: what the pass must move
 # through HBM at least, as derived in the design notes.
 BYTES = {
-    "p2g3_seg": ("particle", 68), "p2g3_atomic": ("particle", 68),
+    "p2g3_seg": ("particle", 68), "p2g3_atomic": ("particle", 68), "p2g3_layer": ("particle", 68),
     "g2p3_fast": ("particle", 84), "g2p3_tiled": ("particle", 84), "g2p3": ("particle", 84),
     "grid3": ("node", 28), "mpm3_clear": ("node", 16),
     "keys3": ("particle", 20), "radix_hist": ("element", 4), "radix_scatter": ("element", 16),
@@ -85,9 +85,9 @@ def bench_mpm3(steps, warmup, p2g, g2p, run, tension=False, lod_level=None):
     sim.advance(steps, sort_every=SORT_EVERY)
     sim.finish()
     t1 = time.perf_counter()
-    names = ["mpm3_clear", "p2g3_seg", "p2g3_atomic", "grid3", "g2p3_fast", "g2p3_tiled", "g2p3", "keys3",
-             "radix_hist", "radix_scan", "radix_scatter", "permute", "tension3_blur", "tension3_stress",
-             "grid3_tension"]
+    names = ["mpm3_clear", "p2g3_seg", "p2g3_atomic", "p2g3_layer", "grid3", "g2p3_fast", "g2p3_tiled", "g2p3",
+             "keys3", "radix_hist", "radix_scan", "radix_scatter", "layers3", "permute", "tension3_blur",
+             "tension3_stress", "grid3_tension"]
     units = {"particle": count, "node": 256 ** 3, "element": count}
     table = kernel_table(sim.sim, names, units)
     sim.sim.timing(False)
@@ -389,7 +389,7 @@ def main():
         "config": {"workload": "3D MLS-MPM/APIC substep, 8,388,608 particles (two seeded cubes, 8 per cell), "
                                "256^3 nodes, dt=5e-5, sorted-segmented P2G, sort every 10 substeps",
                    "particles": count, "grid": [256, 256, 256], "sort_every": SORT_EVERY,
-                   "p2g": ["atomic", "sorted-segmented"][args.p2g], "g2p": ["gather", "lds-tile", "lds-tile-contracted"][args.g2p],
+                   "p2g": ["atomic", "sorted-segmented", "fixed-point", "layers"][args.p2g], "g2p": ["gather", "lds-tile", "lds-tile-contracted"][args.g2p],
                    "run": args.run},
         "roofline": roofline, "kernels": table, "grid_mass_drift": drift,
     }

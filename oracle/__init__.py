@@ -365,11 +365,45 @@ class Mpm3:
         lib.orc_mpm3_lod_target(self.count // 8, self.p["x"], self.p["y"], self.p["z"], level, target,
                                 _c(np.asarray(view, np.float32)))
 
-    def sort(self):
-        order = stable_order(self.keys())
+    def sort(self, layers=False):
+        """The sort by brick key; with ``layers`` the layer-by-layer order of P2G schedule 3.
+        Returns the permutation applied."""
+        keys = self.keys()
+        order = stable_order(keys)
+        if layers:
+            order = order[mpm3_layer_order(keys[order], mpm3_inactive_key(*self.n))]
         for f in FIELDS3:
             self.p[f] = self.p[f][order].copy()
         return order
+
+
+MPM3_GROUP_SHIFT = 8  # a group of P2G schedule 3: the 256 cells whose keys share key >> 8
+
+
+def mpm3_layer_order(sorted_keys, inactive_key) -> np.ndarray:
+    """The group-by-group order of the 3D sort for P2G schedule 3, as a permutation of the
+    key-sorted particles (no reference counterpart: the reference's 3D step has no sort; this is
+    the library's layout). A group is the keys sharing key >> 8 (256 cells, half a sort brick).
+    Its cells are placed by particle count, most first, counts compared as min(count, 255), ties by
+    cell number; place k is generation k // 64, lane k % 64. The group's particles go generation
+    by generation, and within one rank-major: the first particle of each cell in lane order, then
+    every second, and so on (each cell's particles keep their sorted order). The inactive keys
+    (from ``inactive_key`` on) keep their places at the end."""
+    keys = np.asarray(sorted_keys, np.int64)
+    n = len(keys)
+    live = int(np.searchsorted(keys, inactive_key, side="left"))
+    order = np.arange(n, dtype=np.int64)
+    if live:
+        k = keys[:live]
+        rank = np.arange(live) - np.searchsorted(k, k, side="left")  # place within the cell
+        cells, inverse, counts = np.unique(k, return_inverse=True, return_counts=True)
+        group, cell = cells >> MPM3_GROUP_SHIFT, cells & ((1 << MPM3_GROUP_SHIFT) - 1)
+        by_count = np.lexsort((cell, -np.minimum(counts, 255), group))
+        place = np.empty(len(cells), np.int64)
+        place[by_count] = np.arange(len(cells)) - np.searchsorted(group[by_count], group[by_count], side="left")
+        p = place[inverse]
+        order[:live] = np.lexsort((p % 64, rank, p // 64, k >> MPM3_GROUP_SHIFT))
+    return order
 
 
 def mpm3_key_bits(nx, ny, nz) -> int:

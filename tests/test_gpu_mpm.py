@@ -132,8 +132,11 @@ def test_grid_and_advect_bit_exact_3d(gpu):
                                           err_msg=f"g2p={g2p} field {name}")
 
 
+LAYERS = dict(p2g=3, g2p=2, run=4)
+
+
 @pytest.mark.parametrize("sort", [False, True], ids=["unsorted", "sorted"])
-@pytest.mark.parametrize("opts", SCHEDULES[:3], ids=IDS[:3])
+@pytest.mark.parametrize("opts", SCHEDULES[:3] + [LAYERS], ids=IDS[:3] + ["layers"])
 def test_steps_track_the_oracle_3d(gpu, opts, sort):
     """Twenty whole steps of a falling, compressing block: positions within 1e-5 relative of the
     oracle's, grid mass equal to particle mass to 1e-6. Sorted, the segmented P2G and the tiled
@@ -150,7 +153,7 @@ def test_steps_track_the_oracle_3d(gpu, opts, sort):
     rig.params(params)
     for k in range(20):
         if sort and k % 10 == 0:
-            ref.sort()
+            ref.sort(layers=opts.get("p2g") == 3)
             rig.run(["sort"])
         ref.step()
         rig.run("step")
@@ -218,7 +221,8 @@ def test_sort_3d_bit_exact(gpu):
     np.testing.assert_array_equal(perm[active:], np.flatnonzero(dead))
 
 
-def test_inactive_slots_skipped_after_sort_3d(gpu):
+@pytest.mark.parametrize("p2g", [1, 3], ids=["seg", "layers"])
+def test_inactive_slots_skipped_after_sort_3d(gpu, p2g):
     """A block with 40% of its slots emptied, sorted every 5 steps: the sort moves the empty slots
     behind the rest and P2G and G2P stop at the last active one. Twenty steps track the oracle
     (which visits every slot and skips the massless ones) slot for slot within 1e-5, and the
@@ -234,7 +238,7 @@ def test_inactive_slots_skipped_after_sort_3d(gpu):
     params = oracle.mpm3_params(5e-4, 0, -200.0, 0, 2000.0, 1)
     f = {"x": x, "y": y, "z": z, "m": m, "u": u}
     ref = oracle.Mpm3(n, n, n, f, params)
-    rig = maker3()(n, n, n, count)
+    rig = maker3(p2g=p2g)(n, n, n, count)
     for name, v in (("x", x), ("y", y), ("z", z), ("m", m), ("u", u), ("j", np.ones(count, np.float32))):
         rig.write(name, v)
     rig.params(params)
@@ -261,6 +265,104 @@ def test_inactive_slots_skipped_after_sort_3d(gpu):
                                       err_msg=name)
     mass = float(np.sum(m, dtype=np.float64))
     assert abs(float(np.sum(rig.read("gm"), dtype=np.float64)) - mass) <= 1e-6 * mass
+
+
+def _device_fields(rig):
+    return {name: rig.read(name) for name in oracle.FIELDS3}
+
+
+def _launches(rig, kernel):
+    return rig.sim.kernel_time(kernel)[1]
+
+
+def test_sort_3d_layers_bit_exact(gpu):
+    """With P2G by layers (p2g=3) the sort lays each layer of 64 cells out rank-major: the keys,
+    every field, the slot map and the active count are those of the oracle's restatement of that
+    order, bit for bit, the massless slots last in slot order."""
+    n = 48
+    x, y, z = scenes.jittered_block3((2, 2, 2), (40, 30, 20), 2, scenes.JavaRandomFloats(21))
+    count = len(x)
+    order = oracle.JavaRandom(22).shuffle_order(count)
+    f = {"x": x[order], "y": y[order], "z": z[order]}
+    rng = np.random.default_rng(1)
+    for name in oracle.FIELDS3[3:]:
+        f[name] = rng.standard_normal(count).astype(np.float32)
+    f["m"] = np.abs(f["m"]) + np.float32(0.1)
+    dead = rng.random(count) < 0.2
+    f["m"][dead] = 0.0
+    ref = oracle.Mpm3(n, n, n, f, oracle.mpm3_params(1e-3, 0, 0, 0, 0, 1))
+    rig = maker3(**LAYERS)(n, n, n, count)
+    cases.load3(rig, f)
+    rig.run(["sort"])
+    perm = ref.sort(layers=True)
+    np.testing.assert_array_equal(rig.sim.read_words("keys"), ref.keys())
+    for name in oracle.FIELDS3:
+        np.testing.assert_array_equal(rig.read(name).view(np.int32), ref.p[name].view(np.int32), err_msg=name)
+    np.testing.assert_array_equal(rig.sim.read_words("slot"), perm)
+    active = int(np.count_nonzero(~dead))
+    assert int(rig.sim.read_words("active")[0]) == active
+    np.testing.assert_array_equal(perm[active:], np.flatnonzero(dead))
+
+
+def test_scatter_parity_3d_layers(gpu):
+    """P2G by layers after a sort: each grid sum within FP32 reassociation (1e-5 max(1, |e|)) of
+    the oracle's from the same particles, the mass to 1e-6. Then an advect through a strong grid
+    velocity carries a fifth of the particles out of the cells the sort filed them under, and P2G
+    (the moved ones by direct atomics) again matches the oracle from the device's particles."""
+    n, count = 40, 60_000
+    f = _state3(n, count, 7)
+    params = oracle.mpm3_params(1e-3, 0, -50, 0, 400.0, 1)
+    rig = maker3(**LAYERS)(n, n, n, count)
+    cases.load3(rig, f)
+    rig.params(params)
+    rig.sim.timing(True)
+    rig.run(["sort", "clear", "scatter"])
+    assert _launches(rig, "p2g3_layer") == 1
+    mass = float(np.sum(f["m"][f["m"] > 0], dtype=np.float64))
+    for stage in ("sorted", "moved"):
+        ref = oracle.Mpm3(n, n, n, _device_fields(rig), params)
+        ref.clear()
+        ref.scatter()
+        for g in ("gm", "gmu", "gmv", "gmw"):
+            _close(rig.read(g), ref.g[g], f"{stage} {g}")
+        assert abs(float(np.sum(rig.read("gm"), dtype=np.float64)) - mass) <= 1e-6 * mass
+        if stage == "sorted":
+            keys_before = ref.keys()
+            rng = np.random.default_rng(3)
+            for g in ("gu", "gv", "gw"):
+                rig.write(g, rng.uniform(-400, 400, n ** 3).astype(np.float32))
+            rig.run(["advect", "clear", "scatter"])
+            moved = float(np.mean(oracle.Mpm3(n, n, n, _device_fields(rig), params).keys() != keys_before))
+            assert moved > 0.1, moved
+    assert _launches(rig, "p2g3_layer") == 2
+    rig.sim.timing(False)
+
+
+def test_layers_fall_back_after_a_write_3d(gpu):
+    """A host write of positions or masses (or a lod pass) leaves P2G by layers without a layout
+    that covers every particle: until the next sort it takes the sorted-segmented schedule, and
+    the result still matches the oracle."""
+    n, count = 32, 20_000
+    f = _state3(n, count, 5)
+    params = oracle.mpm3_params(1e-3, 0, -50, 0, 400.0, 1)
+    rig = maker3(**LAYERS)(n, n, n, count)
+    cases.load3(rig, f)
+    rig.params(params)
+    rig.run(["sort"])
+    m = rig.read("m")
+    m[::3] = 0.25
+    rig.write("m", m)
+    rig.sim.timing(True)
+    rig.run(["clear", "scatter"])
+    assert _launches(rig, "p2g3_layer") == 0 and _launches(rig, "p2g3_seg") == 1
+    ref = oracle.Mpm3(n, n, n, _device_fields(rig), params)
+    ref.clear()
+    ref.scatter()
+    for g in ("gm", "gmu", "gmv", "gmw"):
+        _close(rig.read(g), ref.g[g], g)
+    rig.run(["sort", "clear", "scatter"])
+    assert _launches(rig, "p2g3_layer") == 1
+    rig.sim.timing(False)
 
 
 def test_slot_writes_are_checked(gpu):
@@ -356,6 +458,19 @@ def test_config_two_cubes_full_size(gpu):
         grids.append(gm)
         assert abs(float(np.sum(gm[0], dtype=np.float64)) - mass) <= 1e-6 * mass
     for a, b, name in zip(grids[0], grids[1], ("gm", "gmu", "gmv", "gmw")):
+        _close(b, a, name)
+    # P2G by layers: the sort's layer-by-layer order bit for bit, and the same grid
+    sim.sim.set_option("p2g", 3)
+    sim.sort()
+    sorted_keys = keys_host[perm]
+    lperm = perm[oracle.mpm3_layer_order(sorted_keys, oracle.mpm3_inactive_key(256, 256, 256))]
+    np.testing.assert_array_equal(sim.sim.read_words("keys"), keys_host[lperm])
+    np.testing.assert_array_equal(sim.positions()[1].view(np.int32), s["y"][lperm].view(np.int32))
+    sim.sim.run_pass("clear")
+    sim.sim.run_pass("scatter")
+    gm = sim.grid()
+    assert abs(float(np.sum(gm[0], dtype=np.float64)) - mass) <= 1e-6 * mass
+    for a, b, name in zip(grids[0], gm, ("gm", "gmu", "gmv", "gmw")):
         _close(b, a, name)
     sim.close()
 

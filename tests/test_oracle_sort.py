@@ -110,3 +110,35 @@ def test_inactive_slots_sort_last_in_slot_order(n):
     full = (n - 2) % 8 == 0
     cells = ((n - 2 + 7) // 8) ** 3 * 512
     assert oracle.mpm3_key_bits(n, n, n) == int(np.ceil(np.log2(cells + (1 if full else 0))))
+
+
+def test_layer_order_by_brute_force():
+    """The group-by-group order (P2G schedule 3's layout) against a direct construction: per
+    group (key >> 8) in key order, its cells ranked by count (most first, counts above 255 equal,
+    ties by cell), 64 to a generation; generation by generation, round r takes the r-th particle of
+    each cell in rank order; the inactive keys keep their places at the end."""
+    rng = np.random.default_rng(11)
+    inactive = 256 * 5 + 17
+    heavy = np.full(300, 256 + 5)  # a cell past the count clamp, and one just at it
+    edge = np.full(255, 256 + 9)
+    keys = np.sort(np.concatenate([rng.integers(0, inactive, 6000), heavy, edge, np.full(40, inactive),
+                                   rng.integers(inactive + 1, inactive + 50, 10)])).astype(np.int32)
+    got = oracle.mpm3_layer_order(keys, inactive)
+    expect = []
+    pos = {}
+    for p, k in enumerate(keys):
+        if k < inactive:
+            pos.setdefault(int(k) >> 8, {}).setdefault(int(k) & 255, []).append(p)
+    for group in sorted(pos):
+        cells = pos[group]
+        ranked = sorted(cells, key=lambda c: (-min(len(cells[c]), 255), c))
+        for g in range(0, len(ranked), 64):
+            gen = ranked[g:g + 64]
+            for r in range(max(len(cells[c]) for c in gen)):
+                for c in gen:
+                    if len(cells[c]) > r:
+                        expect.append(cells[c][r])
+    live = len(expect)
+    np.testing.assert_array_equal(got[:live], expect)
+    np.testing.assert_array_equal(got[live:], np.arange(live, len(keys)))
+    assert sorted(got) == list(range(len(keys)))

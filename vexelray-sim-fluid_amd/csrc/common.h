@@ -159,6 +159,28 @@ int mpm3_lod(const Launch& l, const Mpm3Args& a, float* level, const float* targ
 int mpm3_lod_target(const Launch& l, const Mpm3Args& a, const float* level, float* target, const float* view);
 int mpm3_key_bits(int nx, int ny, int nz);
 
+// The non-empty groups (256 cells, half a sort brick: keys key >> 8) of the last sort that laid
+// the particles out group by group for P2G schedule 3: per group, its cells ranked by particle
+// count in generations of 64, and the particles generation by generation, rank-major.
+struct Layers {
+    int32_t* layer;   // each non-empty group's number, in key order
+    int32_t* start;   // its first particle
+    int32_t* count;   // mpm3_layer_cells() words a group: the particles of each of its cells
+    uint16_t* cell;   // mpm3_layer_cells() a group: the cell at each place of the count order
+    int32_t* meta;    // [0] non-empty groups, [1] end of the active particles, [2..5] work tickets (zero between launches)
+    int64_t cap;      // groups the key space holds
+};
+int mpm3_layer_cells();
+int64_t mpm3_layer_capacity(int nx, int ny, int nz);
+// Words of scratch mpm3_layers takes: the new index, the new keys (capacity each), block counts.
+int64_t mpm3_layer_scratch_words(int64_t capacity);
+// After the radix sort (keys sorted, index its permutation): the layer tables, the source index
+// of the layer-by-layer order (scratch[0, count)), and the keys in that order (copied into keys).
+int mpm3_layers(const Launch& l, const Mpm3Args& a, int32_t* keys, const int32_t* index, int32_t* scratch,
+                const Layers& L);
+// P2G over the layers (particles in the order mpm3_layers made, possibly moved since).
+int mpm3_scatter_layers(const Launch& l, const Mpm3Args& a, const Layers& L);
+
 struct Mpm2Args {
     int nx, ny;
     int64_t count, capacity;

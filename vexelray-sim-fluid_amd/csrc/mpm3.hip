@@ -634,6 +634,458 @@ __global__ __launch_bounds__(kWave) void p2g3_wave_kernel(Mpm3Args a) {
     }
 }
 
+// ------------------------------------------------------------------ P2G by bricks
+//
+// P2G schedule 3. The unit is a group: the 256 cells (8 x 8 by 4 layers, half a sort brick) whose
+// keys share key >> 8. After the radix sort, each non-empty group's cells are ranked by their
+// particle count, most first (ties by cell number), and dealt to lanes in generations of 64:
+// generation g holds the cells ranked 64 g .. 64 g + 63, lane l the one ranked 64 g + l. The
+// group's particles are then laid out generation by generation, and within a generation
+// rank-major: first the first particle of each of its cells that has one, in lane order, then
+// every second one, and so on (a cell's particles keep their sorted order). A wave takes a group,
+// generation by generation: lane l owns one cell, and in round r every lane whose cell has an
+// r-th particle reads it -- consecutive words. Cells of one generation hold similar counts, so
+// few lanes idle (76% busy on the two-cubes scene, against 47% for a wave per 64-cell layer). A
+// lane folds its cell's particles into 108 register sums; at the end of a generation the 64
+// lanes add them into the wave's LDS node tile of the group (10 x 10 x 6 nodes) in 27 steps, one
+// stencil offset a step: the lanes' cells differ, so no two lanes touch one node in a step, and
+// the steps run in program order. The tile leaves with one global float atomic per touched node
+// and field, about 1.5 a particle. A particle whose base cell has changed since the sort (it
+// moved) goes by the reference's direct atomics. Per-particle LDS float atomics (ds_add_f32) are
+// no option: on gfx950 they run at a few percent of the rate of integer LDS atomics.
+
+constexpr int kGroupShift = 8;                 // half a sort brick: keys key >> 8
+constexpr int kGroupCells = 1 << kGroupShift;  // 256: 8 x 8 cells by 4 layers
+constexpr int kGroupGens = kGroupCells / kWave;
+constexpr int kGroupLayers = kGroupCells / 64;
+constexpr int kTileSide = 10;                  // the group's stencil nodes in x and y
+constexpr int kTileZ = kGroupLayers + 2;       // and in z
+constexpr int kTileNodes = kTileSide * kTileSide * kTileZ;
+constexpr int kMovedQueue = 256;               // moved particles a wave queues per group
+constexpr int kGroupPer = 4096;                // sorted keys a block of the boundary passes takes
+constexpr int kGroupWaves = 4096;              // waves of a brick launch, each taking bricks until none is left
+constexpr int kOrderRounds = 256;              // rounds a brick's order tables hold
+
+// The next work item of this wave from a ticket counter t[0] (items 0 .. items - 1), or -1 once
+// they are gone. t[1] counts the waves that have run out; the last of the launch's `waves` to do
+// so puts both words back to zero, so the next launch on the stream starts from zero again.
+__device__ __forceinline__ int take_ticket(int32_t* t, int waves, int items) {
+    int got = 0;
+    if ((threadIdx.x & (kWave - 1)) == 0) got = atomicAdd(t, 1);
+    got = __shfl(got, 0);
+    if (got < items) return got;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        if (atomicAdd(t + 1, 1) == waves - 1) {
+            atomicExch(t, 0);
+            atomicExch(t + 1, 0);
+        }
+    }
+    return -1;
+}
+
+// Counts the brick heads (the first sorted particle of each non-empty brick) of a block's 4096
+// sorted keys; an inactive particle keeps its sorted place (and key) in the new order.
+__global__ __launch_bounds__(256) void group_count_kernel(const int32_t* __restrict__ keys, int64_t n,
+                                                          int32_t inactive, const int32_t* __restrict__ index,
+                                                          int32_t* __restrict__ new_index, int32_t* __restrict__ new_keys,
+                                                          int32_t* __restrict__ heads) {
+    __shared__ int part[256 / kWave];
+    const int64_t begin = (int64_t)blockIdx.x * kGroupPer;
+    const int64_t end = min(begin + (int64_t)kGroupPer, n);
+    int c = 0;
+    for (int64_t p = begin + threadIdx.x; p < end; p += 256) {
+        const int32_t k = keys[p];
+        if (k >= inactive) {
+            new_index[p] = index[p];
+            new_keys[p] = k;
+        } else if (p == 0 || (keys[p - 1] >> kGroupShift) != (k >> kGroupShift)) {
+            ++c;
+        }
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) heads[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// Exclusive scan of the blocks' head counts in place (one block); meta[0] = the non-empty bricks,
+// and meta[1] = 0 until the list pass finds the last active particle.
+__global__ __launch_bounds__(256) void group_scan_kernel(int32_t* __restrict__ heads, int blocks,
+                                                         int32_t* __restrict__ meta, int64_t cap) {
+    __shared__ int part[256];
+    const int per = (blocks + 255) / 256;
+    const int lo = threadIdx.x * per, hi = min(lo + per, blocks);
+    int sum = 0;
+    for (int i = lo; i < hi; ++i) sum += heads[i];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = part[threadIdx.x] - sum;
+    for (int i = lo; i < hi; ++i) {
+        const int v = heads[i];
+        heads[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0) {
+        meta[0] = (int32_t)min((int64_t)part[255], cap);  // never more than the tables hold
+        meta[1] = 0;
+    }
+}
+
+// The list of non-empty bricks in key order: each head writes its brick and first particle at its
+// block's offset plus its rank among the block's heads; the last active particle writes the end
+// of the active ones (meta[1]).
+__global__ __launch_bounds__(256) void group_list_kernel(const int32_t* __restrict__ keys, int64_t n,
+                                                         int32_t inactive, const int32_t* __restrict__ offsets,
+                                                         int32_t* __restrict__ group, int32_t* __restrict__ start,
+                                                         int32_t* __restrict__ meta, int64_t cap) {
+    __shared__ int wsum[256 / kWave];
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
+    const int64_t begin = (int64_t)blockIdx.x * kGroupPer;
+    int base = offsets[blockIdx.x];
+    for (int r = 0; r < kGroupPer / 256; ++r) {
+        const int64_t p = begin + r * 256 + threadIdx.x;
+        bool head = false;
+        int32_t k = 0;
+        if (p < n) {
+            k = keys[p];
+            head = k < inactive && (p == 0 || (keys[p - 1] >> kGroupShift) != (k >> kGroupShift));
+            if (k < inactive && (p + 1 == n || keys[p + 1] >= inactive)) meta[1] = (int32_t)(p + 1);
+        }
+        const uint64_t b = __ballot(head);
+        if (lane == 0) wsum[wave] = __popcll(b);
+        __syncthreads();
+        int before = 0;
+        for (int w = 0; w < wave; ++w) before += wsum[w];
+        const int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (head) {
+            const int i = base + before + __popcll(b & ((1ull << lane) - 1ull));
+            if (i < cap) {
+                group[i] = k >> kGroupShift;
+                start[i] = (int32_t)p;
+            }
+        }
+        base += total;
+        __syncthreads();
+    }
+}
+
+// A wave a brick (see the section head): the cells' counts (LDS integer atomics over the brick's
+// sorted keys), their ranks by count (a stable counting sort on min(count, 255), most first, in
+// eight passes of 64 cells matched by ballots), then the new order -- as the permutation's source
+// index, and the keys in that order. Each round's offset and mask of lanes go to LDS tables, so
+// every particle then finds its place on its own, with coalesced reads: particle p of cell c, of
+// rank r in it, goes to S + off[t] + (the lanes below c's in mask[t]), t the round (generation of
+// c, r). A brick of more rounds than the tables hold goes round by round.
+__global__ __launch_bounds__(256) void group_order_kernel(const int32_t* __restrict__ keys,
+                                                          const int32_t* __restrict__ index,
+                                                          int32_t* __restrict__ new_index,
+                                                          int32_t* __restrict__ new_keys, Layers L, int waves) {
+    struct Lds {
+        int cnt[kGroupCells];    // particles of each cell
+        int first[kGroupCells];  // each cell's first particle
+        int rank[kGroupCells];   // each cell's place in count order
+        int at[kGroupCells];     // the cell at each place
+        int vbase[256];          // per clamped count: cells with more, then the running count of equal ones
+        int toff[kOrderRounds];
+        uint64_t tmask[kOrderRounds];
+        int gbase[kGroupGens + 1];
+    };
+    __shared__ Lds lds_all[256 / kWave];
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
+    Lds& s = lds_all[wave];
+    const uint64_t below = (1ull << lane) - 1ull;
+    const int nl = L.meta[0];
+    for (;;) {
+        const int i = take_ticket(L.meta + 4, waves, nl);
+        if (i < 0) return;
+        const int S = L.start[i];
+        const int E = i + 1 < nl ? L.start[i + 1] : L.meta[1];
+        const int group = L.layer[i];
+        lds_order();  // the previous brick's reads of the tables are behind us
+        for (int c = lane; c < kGroupCells; c += kWave) s.cnt[c] = 0;
+        for (int v = lane; v < 256; v += kWave) s.vbase[v] = 0;
+        lds_order();
+        for (int p = S + lane; p < E; p += kWave) atomicAdd(&s.cnt[keys[p] & (kGroupCells - 1)], 1);
+        lds_order();
+        // first particle of each cell: lane takes cells 8 lane .. 8 lane + 7
+        {
+            int n8[kGroupGens], sum = 0;
+#pragma unroll
+            for (int j = 0; j < kGroupGens; ++j) {
+                n8[j] = s.cnt[kGroupGens * lane + j];
+                sum += n8[j];
+            }
+            int incl = sum;
+#pragma unroll
+            for (int off = 1; off < kWave; off <<= 1) {
+                const int v = __shfl_up(incl, off);
+                if (lane >= off) incl += v;
+            }
+            int run = S + incl - sum;
+#pragma unroll
+            for (int j = 0; j < kGroupGens; ++j) {
+                s.first[kGroupGens * lane + j] = run;
+                L.count[(int64_t)i * kGroupCells + kGroupGens * lane + j] = n8[j];
+                run += n8[j];
+                atomicAdd(&s.vbase[min(n8[j], 255)], 1);
+            }
+        }
+        lds_order();
+        // vbase[v] := cells whose clamped count is above v (lane takes v = 4 lane .. 4 lane + 3)
+        {
+            int h4[4], sum = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                h4[j] = s.vbase[4 * lane + j];
+                sum += h4[j];
+            }
+            int incl = sum;  // suffix sums: lanes above this one
+#pragma unroll
+            for (int off = 1; off < kWave; off <<= 1) {
+                const int v = __shfl_down(incl, off);
+                if (lane + off < kWave) incl += v;
+            }
+            int above = incl - sum;
+            lds_order();
+#pragma unroll
+            for (int j = 3; j >= 0; --j) {
+                s.vbase[4 * lane + j] = above;
+                above += h4[j];
+            }
+        }
+        lds_order();
+        // ranks: cells in number order, 64 at a time; equal clamped counts keep that order
+        for (int q = 0; q < kGroupGens; ++q) {
+            const int c = kWave * q + lane;
+            const int v = min(s.cnt[c], 255);
+            uint64_t same = ~0ull;
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const uint64_t ones = __ballot((v >> b) & 1);
+                same &= ((v >> b) & 1) ? ones : ~ones;
+            }
+            const int within = __popcll(same & below);
+            const int rank = s.vbase[v] + within;
+            lds_order();  // every lane has read its count's base before the leaders move it on
+            if (within == 0) s.vbase[v] += __popcll(same);
+            s.rank[c] = rank;
+            s.at[rank] = c;
+            lds_order();
+        }
+        // the rounds: generation by generation, rank by rank
+        int t = 0, off = 0;
+        int gens = 0;
+        for (int g = 0; g < kGroupGens; ++g) {
+            const int n = s.cnt[s.at[kWave * g + lane]];
+            const int maxn = wave_max(n);
+            if (maxn == 0) break;
+            ++gens;
+            if (lane == 0) s.gbase[g] = t;
+            for (int r = 0; r < maxn; ++r, ++t) {
+                const uint64_t b = __ballot(n > r);
+                if (lane == 0 && t < kOrderRounds) {
+                    s.toff[t] = off;
+                    s.tmask[t] = b;
+                }
+                off += __popcll(b);
+            }
+        }
+        lds_order();
+        if (t <= kOrderRounds) {
+            for (int p = S + lane; p < E; p += kWave) {
+                const int k = keys[p];
+                const int c = k & (kGroupCells - 1);
+                const int rank = s.rank[c];
+                const int tt = s.gbase[rank >> 6] + (p - s.first[c]);
+                const int dest = S + s.toff[tt] + __popcll(s.tmask[tt] & ((1ull << (rank & 63)) - 1ull));
+                new_index[dest] = index[p];
+                new_keys[dest] = k;
+            }
+        } else {
+            int pos = S;
+            for (int g = 0; g < gens; ++g) {
+                const int c = s.at[kWave * g + lane];
+                const int n = s.cnt[c];
+                const int maxn = wave_max(n);
+                for (int r = 0; r < maxn; ++r) {
+                    const uint64_t b = __ballot(n > r);
+                    if (n > r) {
+                        const int dest = pos + __popcll(b & below);
+                        new_index[dest] = index[s.first[c] + r];
+                        new_keys[dest] = (group << kGroupShift) | c;
+                    }
+                    pos += __popcll(b);
+                }
+            }
+        }
+        for (int k = lane; k < kGroupCells; k += kWave)
+            L.cell[(int64_t)i * kGroupCells + k] = (uint16_t)s.at[k];
+    }
+}
+
+// P2G over the groups of the last sort (see the section head). Each round's fields load two
+// rounds ahead of its fold. A particle that moved out of its cell is queued in LDS and goes by
+// direct atomics once the group's sums are spent (all of the group's particles are looked at
+// again if the queue overflows).
+__global__ __launch_bounds__(256) void p2g3_layer_kernel(Mpm3Args a, Layers L, int waves) {
+    __shared__ float4 tiles[256 / kWave][kTileNodes];
+    __shared__ int queues[256 / kWave][kMovedQueue];
+    float4* tile = tiles[threadIdx.x >> 6];
+    int* queue = queues[threadIdx.x >> 6];
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t below = (1ull << lane) - 1ull;
+    const float dt = bits_f(a.params[DT]), bulk = bits_f(a.params[BULK]), rho0 = bits_f(a.params[RHO0]);
+    const int nl = L.meta[0];
+    const int bxn = (a.nx - 2 + 7) / 8, byn = (a.ny - 2 + 7) / 8;
+    constexpr int kPerBrick = 512 / kGroupCells;
+    const int64_t plane = (int64_t)a.nx * a.ny;
+    for (;;) {
+        const int i = take_ticket(L.meta + 2, waves, nl);
+        if (i < 0) return;
+        const uint32_t S = (uint32_t)L.start[i];
+        const int group = L.layer[i];
+        const int brick = group / kPerBrick;
+        const int x0 = (brick % bxn) * 8, y0 = ((brick / bxn) % byn) * 8;
+        const int z0 = (brick / (bxn * byn)) * 8 + (group % kPerBrick) * kGroupLayers;
+        lds_order();  // the previous group's tile and queue reads are behind us
+        for (int t = lane; t < kTileNodes; t += kWave) tile[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        uint32_t pos = S;  // the first particle of the next round to issue
+        int moved = 0;     // the group's moved particles so far, the same in every lane
+        for (int g = 0; g < kGroupGens; ++g) {
+            const int c = L.cell[(int64_t)i * kGroupCells + kWave * g + lane];
+            const int nc = L.count[(int64_t)i * kGroupCells + c];
+            const int maxn = wave_max(nc);
+            if (maxn == 0) break;
+            const int hx = c & 7, hy = (c >> 3) & 7, hz = c >> 6;
+            auto issue = [&](int r, float (&dst)[17]) {
+                const uint64_t b = __ballot(nc > r);
+                const uint32_t q = pos + (uint32_t)__popcll(b & below);
+                pos += (uint32_t)__popcll(b);
+                if (nc > r) {
+#pragma unroll
+                    for (int k = 0; k < 17; ++k) dst[k] = a.p[k][q];
+                }
+                return q;
+            };
+            float n1[17], n2[17];
+            uint32_t q1 = issue(0, n1), q2 = 0;
+            if (maxn > 1) q2 = issue(1, n2);
+            float acc[108];
+#pragma unroll
+            for (int k = 0; k < 108; ++k) acc[k] = 0.0f;
+            for (int r = 0; r < maxn; ++r) {
+                float f[17];
+#pragma unroll
+                for (int k = 0; k < 17; ++k) {
+                    f[k] = n1[k];
+                    n1[k] = n2[k];
+                }
+                const uint32_t q = q1;
+                q1 = q2;
+                if (r + 2 < maxn) q2 = issue(r + 2, n2);
+                // `home` stays true for a lane without a particle this round; the ballot below is
+                // taken by every lane, so `moved` stays the same in all of them
+                bool home = true;
+                const float m = f[6];
+                if (nc > r && m > 0.0f) {
+                    Axis ax[3] = {axis(f[0], a.nx), axis(f[1], a.ny), axis(f[2], a.nz)};
+                    home = ax[0].cell == x0 + hx && ax[1].cell == y0 + hy && ax[2].cell == z0 + hz;
+                    if (home) {
+                        const float vel[3] = {f[3], f[4], f[5]};
+                        float cc[9];
+#pragma unroll
+                        for (int k = 0; k < 9; ++k) cc[k] = f[8 + k];
+                        const float s = (4.0f * (dt * (m / rho0))) * (bulk * (1.0f - f[7]));
+                        p2g_fold(ax, m, vel, cc, s, acc);
+                    }
+                }
+                const uint64_t away = __ballot(!home);
+                if (away) {
+                    if (!home) {
+                        const int at = moved + __popcll(away & below);
+                        if (at < kMovedQueue) queue[at] = (int)q;
+                    }
+                    moved += __popcll(away);
+                }
+            }
+            // the generation's sums into the tile: one stencil offset a step, the lanes' cells distinct
+            lds_order();
+            if (nc > 0) {
+                const int t0 = (hz * kTileSide + hy) * kTileSide + hx;
+#pragma unroll
+                for (int k = 0; k < 27; ++k) {
+                    const int t = t0 + ((k / 9) * kTileSide + (k / 3) % 3) * kTileSide + k % 3;
+                    float4 v = tile[t];
+                    v.x += acc[4 * k];
+                    v.y += acc[4 * k + 1];
+                    v.z += acc[4 * k + 2];
+                    v.w += acc[4 * k + 3];
+                    tile[t] = v;
+                    lds_order();
+                }
+            }
+        }
+        {
+            const int lo[3] = {x0, y0, z0};
+            const int hi[3] = {min(x0 + kTileSide - 1, a.nx - 1), min(y0 + kTileSide - 1, a.ny - 1),
+                               min(z0 + kTileZ - 1, a.nz - 1)};
+            mark_bricks(a, lo, hi, lane, kWave);
+        }
+        lds_order();
+        for (int t = lane; t < kTileNodes; t += kWave) {
+            const float4 v = tile[t];
+            if (v.x == 0.0f) continue;  // no live particle weighs on the node
+            const int tx = t % kTileSide, ty = (t / kTileSide) % kTileSide, tz = t / (kTileSide * kTileSide);
+            const int gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
+            if (gx >= a.nx || gy >= a.ny || gz >= a.nz) continue;
+            const int64_t node = (int64_t)gz * plane + (int64_t)gy * a.nx + gx;
+            atomic_add(a.g[0] + node, v.x);
+            atomic_add(a.g[1] + node, v.y);
+            atomic_add(a.g[2] + node, v.z);
+            atomic_add(a.g[3] + node, v.w);
+        }
+        if (moved == 0) continue;
+        // the particles that left their cell since the sort, now that the sums are spent
+        if (moved <= kMovedQueue) {
+            for (int j = lane; j < moved; j += kWave) {
+                Particle p;
+                load_particle(a, queue[j], dt, bulk, rho0, p);
+                mark_one(a, p.ax);
+                atomic_particle(a, p.ax, p.m, p.vel, p.c, p.s);
+            }
+            continue;
+        }
+        pos = S;
+        for (int g = 0; g < kGroupGens; ++g) {
+            const int c = L.cell[(int64_t)i * kGroupCells + kWave * g + lane];
+            const int nc = L.count[(int64_t)i * kGroupCells + c];
+            const int maxn = wave_max(nc);
+            if (maxn == 0) break;
+            const int hx = x0 + (c & 7), hy = y0 + ((c >> 3) & 7), hz = z0 + (c >> 6);
+            for (int r = 0; r < maxn; ++r) {
+                const uint64_t b = __ballot(nc > r);
+                const uint32_t q = pos + (uint32_t)__popcll(b & below);
+                pos += (uint32_t)__popcll(b);
+                if (nc <= r || !(a.p[6][q] > 0.0f)) continue;
+                if (base_cell(a.p[0][q], a.nx) == hx && base_cell(a.p[1][q], a.ny) == hy &&
+                    base_cell(a.p[2][q], a.nz) == hz)
+                    continue;
+                Particle p;
+                load_particle(a, q, dt, bulk, rho0, p);
+                mark_one(a, p.ax);
+                atomic_particle(a, p.ax, p.m, p.vel, p.c, p.s);
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------------ grid update, Flip3.java:267-311
 
 // Flip3.grid for node `node` at `at` with its sums.
@@ -1036,6 +1488,54 @@ int mpm3_scatter_tiled(const Launch& l, const Mpm3Args& a, int run) {
     else launch_p2g<1>(l, a);
     timed_end(l, "p2g3_seg");
     return check_launch("p2g3_seg");
+}
+
+int64_t mpm3_layer_capacity(int nx, int ny, int nz) {
+    // groups: every sort brick's 512 cells, kGroupCells at a time
+    return (int64_t)((nx - 2 + 7) / 8) * ((ny - 2 + 7) / 8) * ((nz - 2 + 7) / 8) * (512 / kGroupCells);
+}
+
+int mpm3_layer_cells() { return kGroupCells; }
+
+int64_t mpm3_layer_scratch_words(int64_t capacity) {
+    return 2 * capacity + (capacity + kGroupPer - 1) / kGroupPer + 1;
+}
+
+int mpm3_layers(const Launch& l, const Mpm3Args& a, int32_t* keys, const int32_t* index, int32_t* scratch,
+                const Layers& L) {
+    const int64_t n = a.count;
+    const int32_t inactive = mpm3_inactive_key(a.nx, a.ny, a.nz);
+    const int blocks = (int)((n + kGroupPer - 1) / kGroupPer);
+    int32_t* new_index = scratch;
+    int32_t* new_keys = scratch + a.capacity;
+    int32_t* heads = scratch + 2 * a.capacity;
+    timed_begin(l, "layers3");
+    if (blocks > 0)
+        hipLaunchKernelGGL(group_count_kernel, dim3(blocks), dim3(256), 0, l.stream, keys, n, inactive, index,
+                           new_index, new_keys, heads);
+    hipLaunchKernelGGL(group_scan_kernel, dim3(1), dim3(256), 0, l.stream, heads, blocks, L.meta, L.cap);
+    if (blocks > 0) {
+        hipLaunchKernelGGL(group_list_kernel, dim3(blocks), dim3(256), 0, l.stream, keys, n, inactive, heads, L.layer,
+                           L.start, L.meta, L.cap);
+        const int waves = (int)std::min<int64_t>(L.cap, kGroupWaves);
+        const int wb = (waves + 3) / 4;
+        hipLaunchKernelGGL(group_order_kernel, dim3(wb), dim3(256), 0, l.stream, keys, index, new_index, new_keys, L,
+                           wb * 4);
+        if (hipMemcpyAsync(keys, new_keys, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, l.stream) != hipSuccess)
+            return check_launch("layers3 keys");
+    }
+    timed_end(l, "layers3");
+    return check_launch("layers3");
+}
+
+int mpm3_scatter_layers(const Launch& l, const Mpm3Args& a, const Layers& L) {
+    if (a.count == 0) return 0;
+    const int waves = (int)std::min<int64_t>(L.cap, kGroupWaves);
+    const int wb = (waves + 3) / 4;
+    timed_begin(l, "p2g3_layer");
+    hipLaunchKernelGGL(p2g3_layer_kernel, dim3(wb), dim3(256), 0, l.stream, a, L, wb * 4);
+    timed_end(l, "p2g3_layer");
+    return check_launch("p2g3_layer");
 }
 
 int mpm3_flags(const Launch& l, const Mpm3Args& a) {

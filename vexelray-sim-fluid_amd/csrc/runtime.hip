@@ -161,6 +161,11 @@ struct vxr_sim {
     uint8_t* touched = nullptr;  // 3D: bricks P2G reached, and bricks holding velocities
     uint8_t* live = nullptr;
     int64_t bricks = 0;
+    // 3D P2G by layers (option p2g = 3): the tables of the last sort, which laid the particles out
+    // layer by layer, and whether they still cover every particle with mass (no host write of
+    // positions, masses or slots, no lod pass, no clear since)
+    vxr::Layers layers{};
+    bool layers_valid = false;
 
     // SWE
     float* state[2][3] = {};
@@ -410,9 +415,17 @@ int sort_pass(vxr_sim* s) {
     }
     if ((rc = vxr::radix_sort(l, s->keys, s->index, s->count, bits, (int)opt(s, "radix_bits"), s->scratch))) return rc;
     const int nf = three ? 18 : 10;  // 3D: the 17 fields and the slot
-    if ((rc = vxr::permute_fields(l, s->p.data(), s->p_alt.data(), nf, s->index, s->count))) return rc;
+    const bool by_layers = three && opt(s, "p2g") == 3;
+    const int32_t* order = s->index;
+    if (by_layers) {
+        // the radix sort is done with its scratch: the new order's index and keys go there
+        if ((rc = vxr::mpm3_layers(l, mpm3_args(s), s->keys, s->index, s->scratch, s->layers))) return rc;
+        order = s->scratch;
+    }
+    if ((rc = vxr::permute_fields(l, s->p.data(), s->p_alt.data(), nf, order, s->count))) return rc;
     std::swap(s->p, s->p_alt);
     s->flipped = !s->flipped;
+    if (three) s->layers_valid = by_layers;
     if (three && (rc = vxr::mpm3_after_sort(l, mpm3_args(s), s->keys))) return rc;
     if (!three) {
         const int32_t cells = (s->nx - 1) * (s->ny - 1);
@@ -427,8 +440,12 @@ int run_pass(vxr_sim* s, const std::string& pass) {
     if (s->kind == VXR_KIND_MPM3) {
         const vxr::Mpm3Args a = mpm3_args(s);
         if (pass == "clear") return vxr::mpm3_clear(l, a);
-        if (pass == "scatter")
-            return opt(s, "p2g") == 0 ? vxr::mpm3_scatter_atomic(l, a) : vxr::mpm3_scatter_tiled(l, a, (int)opt(s, "run"));
+        if (pass == "scatter") {
+            const int64_t p2g = opt(s, "p2g");
+            if (p2g == 0) return vxr::mpm3_scatter_atomic(l, a);
+            if (p2g == 3 && s->layers_valid) return vxr::mpm3_scatter_layers(l, a, s->layers);
+            return vxr::mpm3_scatter_tiled(l, a, (int)opt(s, "run"));
+        }
         if (pass == "grid") {
             if (!s->tension) return vxr::mpm3_grid(l, a);
             // the colour the last blur wrote: blur k writes c[k % 2], and there are six
@@ -456,7 +473,10 @@ int run_pass(vxr_sim* s, const std::string& pass) {
             if (dense) s->stale = true;
             return vxr::tension3_stress(l, a, s->colour[1], s->stress, dense);
         }
-        if (pass == "lod") return vxr::mpm3_lod(l, a, s->level, s->target, s->stay);
+        if (pass == "lod") {
+            s->layers_valid = false;  // it moves mass between slots
+            return vxr::mpm3_lod(l, a, s->level, s->target, s->stay);
+        }
         if (pass == "lod_target") return vxr::mpm3_lod_target(l, a, s->level, s->target, s->view);
         if (pass == "advect") {
             const int64_t g2p = opt(s, "g2p");
@@ -520,11 +540,11 @@ int replay_segment(vxr_sim* s, bool sort, int n) {
     // segment recorded while it is stale would replay whole-grid passes for good. Run eagerly
     // until a step has rebuilt the region.
     if (s->tension && s->stale) return run_segment(s, sort, n);
-    const auto key = std::make_tuple((int)sort, n, (int)s->flipped);
+    const auto key = std::make_tuple((int)sort, n, (int)s->flipped | (s->layers_valid ? 2 : 0));
     auto it = s->graphs.find(key);
     if (it == s->graphs.end()) {
         const std::vector<float*> p0 = s->p, alt0 = s->p_alt;
-        const bool flipped0 = s->flipped;
+        const bool flipped0 = s->flipped, layers0 = s->layers_valid;
         HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         const int rc = run_segment(s, sort, n);
         hipGraph_t g = nullptr;
@@ -533,6 +553,7 @@ int replay_segment(vxr_sim* s, bool sort, int n) {
         s->p = p0;
         s->p_alt = alt0;
         s->flipped = flipped0;
+        s->layers_valid = layers0;
         if (rc) {
             if (g) hipGraphDestroy(g);
             return rc;
@@ -548,6 +569,7 @@ int replay_segment(vxr_sim* s, bool sort, int n) {
     if (sort) {
         std::swap(s->p, s->p_alt);
         s->flipped = !s->flipped;
+        if (s->kind == VXR_KIND_MPM3) s->layers_valid = opt(s, "p2g") == 3;
     }
     return 0;
 }
@@ -687,8 +709,11 @@ int clear_all(vxr_sim* s) {
         for (float* f : s->g) HIP_TRY(hipMemsetAsync(f, 0, round_up(s->nodes, 4) * 4, s->stream));
         HIP_TRY(hipMemsetAsync(s->params, 0, s->param_words * 4, s->stream));
         if (int rc = mark_all_bricks(s)) return rc;
-        if (s->kind == VXR_KIND_MPM3)
+        if (s->kind == VXR_KIND_MPM3) {
             if (int rc = vxr::mpm3_reset_slots(s->launch(), mpm3_args(s))) return rc;
+            HIP_TRY(hipMemsetAsync(s->layers.meta, 0, 8 * 4, s->stream));
+            s->layers_valid = false;
+        }
         if (s->tension) {
             // what the full passes give a grid without mass: +0 colour and diagonal stress, -0
             // off-diagonal stress
@@ -731,7 +756,7 @@ struct OptionSpec {
     int64_t lo, hi, def;
 };
 const OptionSpec kOptions[] = {
-    {"p2g", 0, 2, 1},   {"g2p", 0, 2, 2},        {"run", 1, 4, 4},   {"radix_bits", 4, 8, 8},
+    {"p2g", 0, 3, 1},   {"g2p", 0, 2, 2},        {"run", 1, 4, 4},   {"radix_bits", 4, 8, 8},
     {"rows", 4, 256, 32}, {"blocks", 0, 1 << 20, 0}, {"graph", 0, 1, 0},
 };
 
@@ -923,7 +948,10 @@ int vxr_create(int kind, const int32_t dims[3], int64_t particles, uint32_t flag
         if (!rc) rc = alloc(s, s->param_words, &ptr), s->params = (int32_t*)ptr;
         if (!rc) rc = alloc(s, s->capacity, &ptr), s->keys = (int32_t*)ptr;
         if (!rc) rc = alloc(s, s->capacity, &ptr), s->index = (int32_t*)ptr;
-        if (!rc) rc = alloc(s, vxr::radix_scratch_words(s->capacity, 8), &ptr), s->scratch = (int32_t*)ptr;
+        // 3D: the radix scratch also holds the layer passes' new index and block counts afterwards
+        const int64_t sort_words = std::max<int64_t>(vxr::radix_scratch_words(s->capacity, 8),
+                                                     three ? vxr::mpm3_layer_scratch_words(s->capacity) : 0);
+        if (!rc) rc = alloc(s, sort_words, &ptr), s->scratch = (int32_t*)ptr;
         if (!rc && three && (flags & VXR_FLAG_TENSION)) {
             s->tension = true;
             for (int k = 0; k < 2 && !rc; ++k) rc = alloc(s, round_up(s->nodes, 4), &ptr), s->colour[k] = (float*)ptr;
@@ -940,6 +968,16 @@ int vxr_create(int kind, const int32_t dims[3], int64_t particles, uint32_t flag
         if (!rc && three) {
             rc = alloc(s, s->capacity, &ptr), s->home = (int32_t*)ptr;
             if (!rc) rc = alloc(s, 1, &ptr), s->active = (int32_t*)ptr;
+        }
+        if (!rc && three) {
+            vxr::Layers& L = s->layers;
+            L.cap = vxr::mpm3_layer_capacity(s->nx, s->ny, s->nz);
+            const int64_t cells = L.cap * vxr::mpm3_layer_cells();
+            rc = alloc(s, L.cap + 1, &ptr), L.layer = (int32_t*)ptr;
+            if (!rc) rc = alloc(s, L.cap + 1, &ptr), L.start = (int32_t*)ptr;
+            if (!rc) rc = alloc(s, cells, &ptr), L.count = (int32_t*)ptr;
+            if (!rc) rc = alloc(s, (cells + 1) / 2, &ptr), L.cell = (uint16_t*)ptr;
+            if (!rc) rc = alloc(s, 8, &ptr), L.meta = (int32_t*)ptr;
         }
         if (!rc && three) {
             s->bricks = vxr::brick_count3(s->nx, s->ny, s->nz);
@@ -1094,6 +1132,9 @@ int vxr_write(vxr_sim* s, const char* name, const void* words, int64_t n) {
     if (three && (b.ptr == (void*)s->p[17] || b.ptr == (void*)s->p[6]))
         // new slots, or new masses: home from the slots, and P2G and G2P visit every particle
         if (int rc = vxr::mpm3_rebuild_home(s->launch(), mpm3_args(s))) return rc;
+    if (three && (b.ptr == (void*)s->p[0] || b.ptr == (void*)s->p[1] || b.ptr == (void*)s->p[2] ||
+                  b.ptr == (void*)s->p[6] || b.ptr == (void*)s->p[17] || b.ptr == (void*)s->keys))
+        s->layers_valid = false;  // P2G goes by the sorted-segmented schedule until the next sort
     bool tension_buffer = false;
     for (float* f : s->stress) tension_buffer = tension_buffer || (f && b.ptr == (void*)f);
     if (s->tension && (b.ptr == s->colour[0] || b.ptr == s->colour[1] || tension_buffer)) {
