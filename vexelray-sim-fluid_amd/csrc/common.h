@@ -91,6 +91,11 @@ struct SweArgs {
     const int32_t* budget_in;
     int32_t* budget_out;
     int32_t* clamped;
+    // A step of fixed_ticks ticks (> 0) instead of the tick rule: speed_in and budget_in are not
+    // read, and speed_out, speed_clear, budget_out and clamped may be null (not written then).
+    int32_t fixed_ticks;
+    // When set, the step runs only if *gate is nonzero (every wave leaves at once otherwise).
+    const int32_t* gate;
 };
 int swe_step(const Launch& l, const SweArgs& a, int rows);
 

@@ -116,6 +116,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void hll_step_kernel(SweArg
     const int band = blockIdx.y * kWavesPerBlock + (threadIdx.x >> 6);
     const int y_begin = band * rows;
     if (y_begin >= ny) return;  // uniform across the wave; no barrier follows
+    if (a.gate && *a.gate == 0) return;  // a gated step not called for
     const int y_end = min(y_begin + rows, ny);
     const int x = blockIdx.x * kStrip + lane - 1;
     const int xc = min(max(x, 0), nx - 1);
@@ -126,12 +127,15 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void hll_step_kernel(SweArg
     const float dx = bits_f(a.params[2]);
     const float dy = bits_f(a.params[3]);
     const float dry = bits_f(a.params[4]);
-    // :184-192, the step in whole ticks, capped by the budget
-    const float fastest = bits_f(a.speed_in[0]);
-    const float allowed = mn((courant * mn(dx, dy)) / mx(fastest, 1e-30f), 2000.0f);
-    int32_t ticks = (int32_t)(allowed * 1000000.0f);
-    const int32_t budget = a.budget_in[0];
-    if (budget < ticks) ticks = budget;
+    // :184-192, the step in whole ticks, capped by the budget (or a fixed number of ticks)
+    int32_t ticks = a.fixed_ticks, budget = 0;
+    if (ticks <= 0) {
+        const float fastest = bits_f(a.speed_in[0]);
+        const float allowed = mn((courant * mn(dx, dy)) / mx(fastest, 1e-30f), 2000.0f);
+        ticks = (int32_t)(allowed * 1000000.0f);
+        budget = a.budget_in[0];
+        if (budget < ticks) ticks = budget;
+    }
     const float dt = (float)ticks * (float)(1.0 / 1000000.0);
     const float rx = dt / dx;
     const float ry = dt / dy;
@@ -193,7 +197,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void hll_step_kernel(SweArg
             const float speed = mx(fabsf(u), fabsf(w)) + sqrtf(g * new_h);
             const int32_t bits = __float_as_int(speed);
             speed_max = bits > speed_max ? bits : speed_max;
-            if (x == 0 && y == 0) {
+            if (x == 0 && y == 0 && a.budget_out) {
                 // :237-240, invocation zero
                 a.speed_clear[0] = 0;
                 a.budget_out[0] = budget - ticks;
@@ -209,7 +213,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void hll_step_kernel(SweArg
         speed_max = o > speed_max ? o : speed_max;
         clamps += __shfl_xor(clamps, off);
     }
-    if (lane == 0) {
+    if (lane == 0 && a.speed_out) {
         atomicMax(a.speed_out, speed_max);
         if (clamps) atomicAdd(a.clamped, clamps);
     }

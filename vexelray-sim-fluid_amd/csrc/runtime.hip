@@ -654,7 +654,7 @@ int autotune(vxr_sim* s, int reps) {
     if (s->kind == VXR_KIND_SWE) {
         int32_t clamped = 0;
         HIP_TRY(hipMemcpy(&clamped, s->clamped, 4, hipMemcpyDeviceToHost));
-        vxr::SweArgs a;
+        vxr::SweArgs a{};
         a.nx = s->nx;
         a.ny = s->ny;
         a.edges[0] = (s->flags & VXR_FLAG_OPEN_WEST) ? 1 : 0;
@@ -1273,7 +1273,7 @@ int vxr_swe_dispatch(vxr_sim* s, int count) {
     if (!s) return fail(VXR_EINVAL, "null simulation");
     if (s->kind != VXR_KIND_SWE) return fail(VXR_ESTATE, "vxr_swe_dispatch is for a shallow-water patch");
     if (count < 0) return fail(VXR_EINVAL, "negative count");
-    vxr::SweArgs a;
+    vxr::SweArgs a{};
     a.nx = s->nx;
     a.ny = s->ny;
     a.edges[0] = (s->flags & VXR_FLAG_OPEN_WEST) ? 1 : 0;

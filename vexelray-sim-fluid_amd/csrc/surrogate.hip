@@ -628,6 +628,61 @@ __global__ void dam_bumps_kernel(int n, int bumps, const double* __restrict__ bp
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
+// The fallback rule, on the device: an application is refused if any depth it gives is negative
+// or not finite, or if its volume (sum of depths) moved by more than `tolerance` relative. The
+// first pass folds strided shares of the cells per block (sums in double), the second combines
+// the blocks in a fixed order, so the verdict is the same on every run.
+constexpr int kCheckParts = 256;
+
+struct CheckPart {
+    double before, after;
+    long long bad;
+};
+
+__global__ __launch_bounds__(kBlock) void check_kernel(const float* __restrict__ h0, const float* __restrict__ h1,
+                                                       int64_t cells, CheckPart* __restrict__ parts) {
+    __shared__ CheckPart lds[kBlock];
+    CheckPart p{0.0, 0.0, 0};
+    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < cells; k += (int64_t)gridDim.x * kBlock) {
+        const float a = h0[k], b = h1[k];
+        p.before += a;
+        p.after += b;
+        if (!(b >= 0.0f) || !isfinite(b)) p.bad++;
+    }
+    lds[threadIdx.x] = p;
+    __syncthreads();
+    for (int half = kBlock / 2; half > 0; half >>= 1) {
+        if (threadIdx.x < half) {
+            lds[threadIdx.x].before += lds[threadIdx.x + half].before;
+            lds[threadIdx.x].after += lds[threadIdx.x + half].after;
+            lds[threadIdx.x].bad += lds[threadIdx.x + half].bad;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) parts[blockIdx.x] = lds[0];
+}
+
+// flag[0] = 1 when the application is refused, else 0; flag[1] counts refusals; out (if given)
+// = volume before, volume after, bad cells.
+__global__ __launch_bounds__(kBlock) void check_final_kernel(const CheckPart* __restrict__ parts, double tolerance,
+                                                             int32_t* __restrict__ flag, double* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    CheckPart t{0.0, 0.0, 0};
+    for (int i = 0; i < kCheckParts; ++i) {
+        t.before += parts[i].before;
+        t.after += parts[i].after;
+        t.bad += parts[i].bad;
+    }
+    const bool refuse = t.bad > 0 || !(fabs(t.after - t.before) <= tolerance * fabs(t.before));
+    flag[0] = refuse ? 1 : 0;
+    if (refuse) flag[1] += 1;
+    if (out) {
+        out[0] = t.before;
+        out[1] = t.after;
+        out[2] = (double)t.bad;
+    }
+}
+
 }  // namespace
 }  // namespace vxr
 
@@ -809,6 +864,50 @@ int vxs_dam_bumps(void* stream, int n, int bumps, const double* bump_params, dou
     hipLaunchKernelGGL(dam_bumps_kernel, dim3(blocks_for((int64_t)n * n)), dim3(kBlock), 0, (hipStream_t)stream, n,
                        bumps, bump_params, radius, h, hu, hv);
     return serr("dam_bumps");
+}
+
+// One FP32 HLL step of exactly `ticks` ticks (dt = ticks * 1e-6 s, as the tick rule rounds it):
+// the shallow-water step's kernel and arithmetic (bit for bit the CPU restatement's step whose
+// budget caps it at `ticks`), walls or open edges as `edges` (west, east, south, north: 0 wall,
+// 1 open), params the 5 device words of ShallowWater.params. With `gate` (a device word) the step
+// runs only if *gate is nonzero. No speed, budget or clamp words are kept.
+int vxs_hll32(void* stream, int nx, int ny, const int* edges, const int32_t* params, int32_t ticks,
+              const int32_t* gate, const float* h, const float* hu, const float* hv, float* oh, float* ohu,
+              float* ohv) {
+    if (nx < 1 || ny < 1 || !edges || !params || ticks < 1 || !h || !hu || !hv || !oh || !ohu || !ohv)
+        return sfail("hll32: bad argument");
+    vxr::SweArgs a{};
+    a.nx = nx;
+    a.ny = ny;
+    for (int k = 0; k < 4; ++k) a.edges[k] = edges[k] ? 1 : 0;
+    a.in[0] = h;
+    a.in[1] = hu;
+    a.in[2] = hv;
+    a.out[0] = oh;
+    a.out[1] = ohu;
+    a.out[2] = ohv;
+    a.params = params;
+    a.fixed_ticks = ticks;
+    a.gate = gate;
+    const vxr::Launch l{(hipStream_t)stream, nullptr};
+    if (vxr::swe_step(l, a, 32)) return serr("hll32");
+    return serr("hll32");
+}
+
+// Words of scratch vxs_fallback_check takes.
+int64_t vxs_check_scratch(void) { return (int64_t)kCheckParts * (int64_t)(sizeof(CheckPart) / 4); }
+
+// The surrogate's fallback rule on the device, for an application that took depths h0 to h1
+// (cells each): flag[0] = 1 if any h1 is negative or not finite, or the volume moved by more than
+// `tolerance` relative, else 0; flag[1] counts the refusals. out (device, optional) = the two
+// volumes (sums of depths) and the bad cells. Asynchronous; a fixed summation order.
+int vxs_fallback_check(void* stream, const float* h0, const float* h1, int64_t cells, double tolerance,
+                       void* scratch, int32_t* flag, double* out) {
+    if (!h0 || !h1 || cells < 1 || !scratch || !flag || !(tolerance >= 0)) return sfail("fallback_check: bad argument");
+    CheckPart* parts = (CheckPart*)scratch;
+    hipLaunchKernelGGL(check_kernel, dim3(kCheckParts), dim3(kBlock), 0, (hipStream_t)stream, h0, h1, cells, parts);
+    hipLaunchKernelGGL(check_final_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, parts, tolerance, flag, out);
+    return serr("fallback_check");
 }
 
 int vxs_f64_to_f32(void* stream, const double* a, float* b, int64_t n) {

@@ -100,6 +100,10 @@ def lib() -> ctypes.CDLL:
                                          ctypes.c_double]),
             "vxs_dam_bumps": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_double, vp, vp, vp]),
             "vxs_f64_to_f32": (ctypes.c_int, [vp, vp, vp, i64]),
+            "vxs_hll32": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), vp, i32, vp,
+                                         vp, vp, vp, vp, vp, vp]),
+            "vxs_check_scratch": (i64, []),
+            "vxs_fallback_check": (ctypes.c_int, [vp, vp, vp, i64, ctypes.c_double, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(l, name)

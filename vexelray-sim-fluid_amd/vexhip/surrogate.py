@@ -1,5 +1,5 @@
 """A convolutional surrogate of the HLL shallow-water step, its training on FP64 HLL
-trajectories, and the fallback to HLL.
+trajectories, and its fallback to plain HLL.
 
 No reference counterpart: the reference has only the HLL solver (core/stencil/ShallowWater.java).
 The network maps a state (h, hu, hv) on an n x n grid to the state ``STEPS`` HLL steps of a fixed
@@ -7,7 +7,11 @@ The network maps a state (h, hu, hv) on an n x n grid to the state ``STEPS`` HLL
 state channels padded to 16), ReLU between, bf16 weights and activations, FP32 accumulation on
 the matrix cores (csrc/surrogate.hip). The last layer's output, scaled back, is added to the
 state. Its targets and its judge are the FP64 HLL step (vxs_hll64), the CPU restatement's FP64
-twin on the device.
+twin on the device. ``Surrogate.advance`` applies it under the fallback rule: a device check
+(vxs_fallback_check) refuses an application that gives a negative or non-finite depth or moves the
+volume by more than 1e-3, and then STEPS plain FP32 HLL steps of the same dt (vxs_hll32, the
+shallow-water step's own kernel) replace it -- queued behind the check and gated on its flag word,
+so no host round trip decides it.
 
 Every kernel is libvexelhip.so's; torch only allocates device memory and names the stream.
 """
@@ -27,6 +31,7 @@ G = 9.81
 DRY = 1e-6
 COURANT = 0.3        # of the fixed dt against the fastest wave the scenes reach
 FASTEST = 6.0        # m/s: sqrt(g * 2.6) plus a current of about 1 m/s
+TOLERANCE = 1e-3     # the fallback rule's relative volume drift
 BLOCKS = 16          # blocks of STEPS steps a training trajectory runs
 WGRAD_CHUNK = 4096   # pixels a weight-gradient block sums
 
@@ -56,9 +61,22 @@ def _stream() -> int:
     return _t().cuda.current_stream().cuda_stream
 
 
+def fixed_ticks(n: int) -> int:
+    """The fixed step on an n x n unit square in whole ticks of 1e-6 s (the shallow-water clock's
+    unit), so that the FP32 fallback's step is the tick rule's own."""
+    return max(1, int(COURANT * (1.0 / n) / FASTEST * 1e6))
+
+
 def fixed_dt(n: int) -> float:
     """The fixed step of the training and judging HLL runs on an n x n unit square."""
-    return COURANT * (1.0 / n) / FASTEST
+    return fixed_ticks(n) * 1e-6
+
+
+def swe_words(n: int) -> np.ndarray:
+    """ShallowWater.params' five words for the unit square at n x n (the Courant number is not
+    read by a fixed-tick step)."""
+    f = np.array([G, 0.45, 1.0 / n, 1.0 / n, DRY], np.float32)
+    return f.view(np.int32)
 
 
 def bump_params(seed: int, bumps: int = 32) -> np.ndarray:
@@ -154,6 +172,44 @@ class Surrogate:
         self.scratch = torch.empty(words, dtype=torch.float32, device="cuda")
         self.loss_acc = torch.zeros(1, dtype=torch.float32, device="cuda")
         self.grads = True
+
+    def _alloc_fallback(self) -> None:
+        torch = _t()
+        px = self.images * self.n * self.n
+        self.flag = torch.zeros(2, dtype=torch.int32, device="cuda")
+        self.check_out = torch.zeros(3, dtype=torch.float64, device="cuda")
+        self.check_scratch = torch.empty(_native.lib().vxs_check_scratch(), dtype=torch.int32, device="cuda")
+        self.params32 = torch.from_numpy(swe_words(self.n)).to("cuda")
+        self.hll_tmp = [[torch.empty(px, dtype=torch.float32, device="cuda") for _ in range(3)] for _ in range(2)]
+        self.edges = (ctypes.c_int * 4)(0, 0, 0, 0)
+
+    def advance(self, h, hu, hv, out=None):
+        """One application under the fallback rule (an image at a time, walls all round): the
+        network's state STEPS steps on, unless the device check refuses it -- then STEPS plain
+        FP32 HLL steps of fixed_ticks(n) each from the same state, bit for bit. Asynchronous."""
+        if self.images != 1:
+            raise ValueError("advance takes one image at a time")
+        if getattr(self, "flag", None) is None:
+            self._alloc_fallback()
+        lib = _native.lib()
+        st = _stream()
+        out = self.forward(h, hu, hv, out)
+        _native.vxs_check(lib.vxs_fallback_check(st, _p(h), _p(out[0]), self.n * self.n, TOLERANCE,
+                                                 _p(self.check_scratch), _p(self.flag), _p(self.check_out)))
+        ticks = fixed_ticks(self.n)
+        src = (h, hu, hv)
+        for k in range(STEPS):
+            dst = out if k == STEPS - 1 else self.hll_tmp[k % 2]
+            _native.vxs_check(lib.vxs_hll32(st, self.n, self.n, self.edges, _p(self.params32), ticks,
+                                            _p(self.flag), *map(_p, src), *map(_p, dst)))
+            src = dst
+        return out
+
+    def fallbacks(self) -> int:
+        """Applications the fallback rule has refused so far (a readback)."""
+        if getattr(self, "flag", None) is None:
+            return 0
+        return int(self.flag[1].item())
 
     def set_output_scale(self, scales) -> None:
         self.norm[6:9] = _t().tensor([float(s) for s in scales], dtype=_t().float32)
@@ -284,34 +340,25 @@ def train(n=1024, seeds=range(2000), iters=None, lr=1e-3, seed=0, log=None, mode
     return model, losses
 
 
-def fallback_needed(state, pred, tolerance=1e-3) -> bool:
-    """The fallback rule: any negative depth, or the volume off by more than `tolerance` relative."""
-    torch = _t()
-    if bool((pred[0] < 0).any().item()):
-        return True
-    v0 = float(torch.sum(state[0], dtype=torch.float64).item())
-    v1 = float(torch.sum(pred[0], dtype=torch.float64).item())
-    return abs(v1 - v0) > tolerance * abs(v0)
-
-
 def evaluate(model: Surrogate, n=1024, seeds=range(10000, 10200), seed=1):
-    """Relative L2 error in h against the FP64 HLL, one random block per scene, the surrogate
-    applied once; with the error of keeping the state (persistence) for scale, and the fallbacks."""
+    """Relative L2 error in h against the FP64 HLL, one random block per scene, one application
+    under the fallback rule (what Surrogate.advance returns); with the error of keeping the state
+    (persistence) for scale, and the applications the rule refused."""
     torch = _t()
     rng = np.random.default_rng(seed)
     hll = Hll64(n)
-    errs, keep, falls = [], [], 0
+    errs, keep = [], []
+    before = model.fallbacks()
     for s in seeds:
         hll.scene(int(s))
         hll.step(STEPS * int(rng.integers(BLOCKS)))
         state = hll.state32()
         hll.step(STEPS)
         truth = hll.a[0]
-        pred = model.forward(*state)
-        if fallback_needed(state, pred):
-            falls += 1
+        pred = model.advance(*state)
         ref = torch.linalg.vector_norm(truth).item()
         errs.append(torch.linalg.vector_norm(pred[0].double() - truth).item() / ref)
         keep.append(torch.linalg.vector_norm(state[0].double() - truth).item() / ref)
     return {"rel_l2_h_mean": float(np.mean(errs)), "rel_l2_h_max": float(np.max(errs)),
-            "persistence_rel_l2_h_mean": float(np.mean(keep)), "fallbacks": falls, "scenes": len(errs)}
+            "persistence_rel_l2_h_mean": float(np.mean(keep)), "fallbacks": model.fallbacks() - before,
+            "scenes": len(errs)}
