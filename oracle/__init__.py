@@ -14,7 +14,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# VEXEL_ORACLE_LIB names another build of the same sources (the CPU baseline's -march=native one)
+LIB_PATH = os.environ.get("VEXEL_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")

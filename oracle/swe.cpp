@@ -282,10 +282,11 @@ void orc_particle_diagnostics(int64_t count, const float* j, double speed, doubl
     }
     std::sort(valid.begin(), valid.end());
     auto percentile = [&](double p) { return std::min<int64_t>(n - 1, (int64_t)(p * (double)n)); };
-    out[0] = 1 / (double)valid[n - 1];
-    out[1] = 1 / (double)valid[0];
-    out[2] = 1 / (double)valid[percentile(0.99)];
-    out[3] = 1 / (double)valid[percentile(0.01)];
+    // 1 / valid[i] on a float[]: a binary32 quotient, widened (ParticleDiagnostics.java:50-51)
+    out[0] = (double)(1.0f / valid[n - 1]);
+    out[1] = (double)(1.0f / valid[0]);
+    out[2] = (double)(1.0f / valid[percentile(0.99)]);
+    out[3] = (double)(1.0f / valid[percentile(0.01)]);
 }
 
 }  // extern "C"

@@ -42,5 +42,13 @@ def test_percentiles_are_order_statistics():
     j = r.uniform(0.8, 1.2, 5000).astype(np.float32)
     d = oracle.particle_diagnostics(j, 0, 1, 1)
     s = np.sort(j)
-    assert d["low_density"] == 1 / float(s[min(4999, int(0.99 * 5000))])
-    assert d["high_density"] == 1 / float(s[min(4999, int(0.01 * 5000))])
+    assert d["low_density"] == float(np.float32(1) / s[min(4999, int(0.99 * 5000))])
+    assert d["high_density"] == float(np.float32(1) / s[min(4999, int(0.01 * 5000))])
+
+
+def test_densities_are_binary32_quotients():
+    """ParticleDiagnostics divides 1 by a float (ParticleDiagnostics.java:50-51): the binary32
+    quotient, widened to double -- for J = 3 that is 0.3333333432674408, not 1/3 in binary64."""
+    d = oracle.particle_diagnostics(np.array([3.0, 3.0], np.float32), 0, 1, 1)
+    assert d["min_density"] == float(np.float32(1) / np.float32(3))
+    assert d["min_density"] != 1.0 / 3.0

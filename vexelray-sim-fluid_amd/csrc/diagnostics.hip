@@ -185,7 +185,7 @@ int particle_diagnostics(const Launch& l, const float* j, int64_t count, int64_t
     for (int i = 0; i < 4; ++i) {
         float v;
         memcpy(&v, &host[i], 4);
-        out[i] = 1.0 / (double)v;
+        out[i] = (double)(1.0f / v);  // the reference divides on a float[] and widens the quotient
     }
     return 0;
 }
