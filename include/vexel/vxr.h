@@ -147,6 +147,16 @@ int vxr_swe_diagnostics(vxr_sim* sim, double g, double dry, double dt, double dx
  * For a batch, over every scene's particles together. Synchronous. */
 int vxr_particle_diagnostics(vxr_sim* sim, double speed, double sound, double dt, double out[7]);
 
+/* vxr_particle_diagnostics without allocation. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). ParticleDiagnostics.of of `count` device floats `j` (J), queued on
+ * `stream` (a hipStream_t; 0 the null stream) and read back synchronously into out[7] as
+ * vxr_particle_diagnostics gives it. Its sort lives in a fixed device array of the library
+ * (16.3 MB, held by each GPU it runs on until the process ends); count must be 1 to 1,048,576
+ * (VXR_EINVAL past that). Calls that overlap on one GPU share that array and race. */
+int vxr_particle_diagnostics_noalloc(void* stream, const float* j, int64_t count, double speed, double sound,
+                                     double dt, double out[7]);
+
 /* The fixed-point bilinear scatter (core/particle/FixedScatter.java:151-183: FixedScatter.kernel,
  * schedule 0, and FixedScatter.segmented, schedule 1) of `count` particles (device arrays x y u v m,
  * node units) into the i32 grids gridMq, gridMuq, gridMvq of an n x n grid (device arrays, added to),

@@ -263,6 +263,12 @@ int particle_diagnostics(const Launch& l, const float* j, int64_t count, int64_t
                          int32_t* index, int32_t* sort_scratch, int digit_bits, int32_t* small, double speed,
                          double sound, double dt, double out[7]);
 
+// ParticleDiagnostics.of with its sort in the library's fixed device array (no allocation): at
+// most particle_diagnostics_static_limit() particles; `why` says what failed. Synchronous.
+int64_t particle_diagnostics_static_limit();
+int particle_diagnostics_static(const Launch& l, const float* j, int64_t count, double speed, double sound,
+                                double dt, double out[7], const char** why);
+
 // ------------------------------------------------------------------ sort
 
 // Stable LSD radix sort of (key, index) pairs, keys in [0, 2^bits). Scratch sized by

@@ -8,6 +8,8 @@
 // exactly as the sequential loop has them, the volume to double rounding of another summation
 // order. Particles: the valid J (finite, above zero) sorted by their bits (positive floats order
 // as integers), the extremes and the 1st and 99th percentile picked from the sorted keys.
+#include <mutex>
+
 #include "common.h"
 
 namespace vxr {
@@ -162,7 +164,7 @@ int particle_diagnostics(const Launch& l, const float* j, int64_t count, int64_t
                          double sound, double dt, double out[7]) {
     int* invalid = (int*)small;
     int32_t* picks = small + 8;
-    if (hipMemsetAsync(invalid, 0, 4, l.stream) != hipSuccess) return check_launch("particle_diagnostics");
+    if (int rc = fill_zero(l, invalid, 1)) return rc;  // a kernel, not a memset: safe to capture
     timed_begin(l, "particle_diagnostics");
     hipLaunchKernelGGL(jkeys_kernel, dim3((unsigned)((capacity + kBlock - 1) / kBlock)), dim3(kBlock), 0, l.stream, j,
                        count, capacity, keys, index, invalid);
@@ -188,6 +190,67 @@ int particle_diagnostics(const Launch& l, const float* j, int64_t count, int64_t
         out[i] = (double)(1.0f / v);  // the reference divides on a float[] and widens the quotient
     }
     return 0;
+}
+
+// ------------------------------------------------------------------ without allocation
+
+// ParticleDiagnostics.of for a caller that cannot allocate (code under graph capture, serving
+// loops, harnesses that forbid hipMalloc): the J keys, their index and the radix sort's scratch
+// live in `scratch`, a file-scope device array of fixed size that each GPU holds once, for the
+// life of the process. At most kStaticParticles particles a call. Every call on one GPU shares
+// that GPU's array, so calls that overlap on one GPU (two streams, two host threads) race.
+constexpr int64_t kStaticParticles = 1 << 20;
+constexpr int kStaticDigits = 8;
+constexpr int64_t kStaticSortWords =
+    2 * kStaticParticles + (1 << kStaticDigits) * ((kStaticParticles + 4095) / 4096) + (1 << kStaticDigits);
+constexpr int64_t kStaticSmallWords = 16;
+constexpr int64_t kStaticWords = 2 * kStaticParticles + kStaticSortWords + kStaticSmallWords;  // 16.3 MB
+
+static __device__ __align__(256) int32_t scratch[kStaticWords];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int32_t* g_scratch[kMaxDevices] = {};  // each device's address of `scratch`, taken once
+}  // namespace
+
+int64_t particle_diagnostics_static_limit() { return kStaticParticles; }
+
+// The current device's copy of `scratch`. In host code the array's name stands for no device
+// address: hipGetSymbolAddress gives this device's, once, with the device current.
+int static_scratch(int32_t** out, const char** why) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        *why = "no current device, or a device number past 63";
+        return 3;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            *why = "hipGetSymbolAddress of the diagnostics scratch failed";
+            return 3;
+        }
+        g_scratch[dev] = (int32_t*)p;
+    }
+    *out = g_scratch[dev];
+    return 0;
+}
+
+int particle_diagnostics_static(const Launch& l, const float* j, int64_t count, double speed, double sound,
+                                double dt, double out[7], const char** why) {
+    if (radix_scratch_words(kStaticParticles, kStaticDigits) > kStaticSortWords) {
+        *why = "the diagnostics scratch is smaller than the sort needs";
+        return 3;
+    }
+    int32_t* base = nullptr;
+    if (int rc = static_scratch(&base, why)) return rc;
+    int32_t* keys = base;
+    int32_t* index = keys + kStaticParticles;
+    int32_t* sort = index + kStaticParticles;
+    int32_t* small = sort + kStaticSortWords;
+    return particle_diagnostics(l, j, count, count, keys, index, sort, kStaticDigits, small, speed, sound, dt, out);
 }
 
 }  // namespace vxr

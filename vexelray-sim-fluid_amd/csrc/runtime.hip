@@ -1269,6 +1269,20 @@ int vxr_particle_diagnostics(vxr_sim* s, double speed, double sound, double dt, 
     return rc;
 }
 
+int vxr_particle_diagnostics_noalloc(void* stream, const float* j, int64_t count, double speed, double sound,
+                                     double dt, double out[7]) {
+    if (!j || !out) return fail(VXR_EINVAL, "null argument");
+    const int64_t limit = vxr::particle_diagnostics_static_limit();
+    if (count < 1 || count > limit)
+        return fail(VXR_EINVAL, "vxr_particle_diagnostics_noalloc takes 1 to %lld particles (its fixed scratch), got %lld",
+                    (long long)limit, (long long)count);
+    const char* why = "";
+    const int rc = vxr::particle_diagnostics_static(vxr::Launch{(hipStream_t)stream, nullptr}, j, count, speed, sound,
+                                                    dt, out, &why);
+    if (rc == 3 && *why) return fail(VXR_EDEVICE, "%s", why);
+    return rc;
+}
+
 int vxr_swe_dispatch(vxr_sim* s, int count) {
     if (!s) return fail(VXR_EINVAL, "null simulation");
     if (s->kind != VXR_KIND_SWE) return fail(VXR_ESTATE, "vxr_swe_dispatch is for a shallow-water patch");

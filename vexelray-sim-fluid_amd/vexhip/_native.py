@@ -80,6 +80,8 @@ def lib() -> ctypes.CDLL:
                                                    ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]),
             "vxr_particle_diagnostics": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                                         ctypes.POINTER(ctypes.c_double)]),
+            "vxr_particle_diagnostics_noalloc": (ctypes.c_int, [vp, vp, i64, ctypes.c_double, ctypes.c_double,
+                                                                ctypes.c_double, ctypes.POINTER(ctypes.c_double)]),
             "vxr_fixed_scatter": (ctypes.c_int, [vp, ctypes.c_int, i64, vp, vp, vp, vp, vp, ctypes.c_int,
                                                  ctypes.c_int, ctypes.c_int, vp, vp, vp]),
             # the shallow-water surrogate's kernels (csrc/surrogate.hip)
