@@ -47,6 +47,8 @@ def _load() -> ctypes.CDLL:
         "orc_rng_shuffle_order": (None, [_vp, ctypes.c_int32, _i32p]),
         "orc_swe_step": (None, [_c_i, _c_i, _i32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i32p,
                                 _i32p, _i32p, _i32p, _i32p, _i32p, _i32p]),
+        "orc_swe_step_omp": (None, [_c_i, _c_i, _i32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i32p,
+                                _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _c_i]),
         "orc_swe_fastest_wave": (ctypes.c_int32, [_c_i64, _f32p, _f32p, _f32p, ctypes.c_double,
                                                   ctypes.c_double]),
         "orc_swe_diagnostics": (None, [_c_i64, _f32p, _f32p, _f32p, ctypes.c_double, ctypes.c_double,
@@ -189,10 +191,11 @@ class SweStepper:
     """The reference test harness's CPU stepper (test/stencil/Stepper.java:26-221) over the
     oracle: two states ping-pong, three speed words rotate, two budget words ping-pong."""
 
-    def __init__(self, nx: int, ny: int, edges=(WALL, WALL, WALL, WALL)):
+    def __init__(self, nx: int, ny: int, edges=(WALL, WALL, WALL, WALL), threads: int = 1):
         if nx < 1 or ny < 1:
             raise ValueError(f"a patch needs at least one cell each way, got {nx} x {ny}")
         self.nx, self.ny = nx, ny
+        self.threads = threads  # > 1: orc_swe_step_omp, the same bits on that many host threads
         self.edges = np.asarray(edges, dtype=np.int32)
         self.dispatched = 0
         self.state = [[np.zeros(nx * ny, np.float32) for _ in range(3)] for _ in range(2)]
@@ -212,9 +215,13 @@ class SweStepper:
     def dispatch(self):
         k = self.dispatched
         out, inn = self.state[(k + 1) % 2], self.state[k % 2]
-        lib.orc_swe_step(self.nx, self.ny, self.edges, out[0], out[1], out[2], inn[0], inn[1], inn[2],
-                         self.params, self.speed[k % 3], self.speed[(k + 1) % 3], self.speed[(k + 2) % 3],
-                         self.budget[k % 2], self.budget[(k + 1) % 2], self._clamped)
+        args = (self.nx, self.ny, self.edges, out[0], out[1], out[2], inn[0], inn[1], inn[2], self.params,
+                self.speed[k % 3], self.speed[(k + 1) % 3], self.speed[(k + 2) % 3], self.budget[k % 2],
+                self.budget[(k + 1) % 2], self._clamped)
+        if self.threads > 1:
+            lib.orc_swe_step_omp(*args, int(self.threads))
+        else:
+            lib.orc_swe_step(*args)
         self.dispatched += 1
 
     def step(self, n: int):

@@ -51,6 +51,14 @@ void orc_swe_step(int nx, int ny, const int edges[4],
                   const int32_t params[5],
                   const int32_t* speed_in, int32_t* speed_out, int32_t* speed_clear,
                   const int32_t* budget_in, int32_t* budget_out, int32_t* clamped);
+/* The same step over `threads` host threads (OpenMP): bit-identical to orc_swe_step, as each
+ * cell is computed alone and the speed maximum and clamp count do not depend on the order. */
+void orc_swe_step_omp(int nx, int ny, const int edges[4],
+                      float* out_h, float* out_hu, float* out_hv,
+                      const float* in_h, const float* in_hu, const float* in_hv,
+                      const int32_t params[5],
+                      const int32_t* speed_in, int32_t* speed_out, int32_t* speed_clear,
+                      const int32_t* budget_in, int32_t* budget_out, int32_t* clamped, int threads);
 int32_t orc_swe_fastest_wave(int64_t cells, const float* h, const float* hu, const float* hv,
                              double g, double dry);
 /* core/stencil/Diagnostics.java:84-122; out = volume, minDepth, maxDepth, maxSpeed,

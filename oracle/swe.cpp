@@ -12,6 +12,8 @@
 #include <cstring>
 #include <limits>
 
+#include <omp.h>
+
 namespace {
 
 inline float mn(float a, float b) { return b < a ? b : a; }
@@ -97,27 +99,13 @@ State neighbour(const float* h, const float* hu, const float* hv, int64_t cell, 
 
 extern "C" {
 
-void orc_swe_step(int nx, int ny, const int edges[4], float* out_h, float* out_hu, float* out_hv,
-                  const float* in_h, const float* in_hu, const float* in_hv, const int32_t params[5],
-                  const int32_t* speed_in, int32_t* speed_out, int32_t* speed_clear, const int32_t* budget_in,
-                  int32_t* budget_out, int32_t* clamped) {
-    const float g = bits_to_float(params[0]);
-    const float courant = bits_to_float(params[1]);
-    const float dx = bits_to_float(params[2]);
-    const float dy = bits_to_float(params[3]);
-    const float dry = bits_to_float(params[4]);
-    // :184-192, the step in whole ticks, capped by the budget.
-    const float fastest = bits_to_float(speed_in[0]);
-    const float allowed = mn((courant * mn(dx, dy)) / mx(fastest, 1e-30f), 2000.0f);
-    int32_t ticks = (int32_t)(allowed * 1000000.0f);
-    const int32_t budget = budget_in[0];
-    if (budget < ticks) ticks = budget;
-    const float dt = (float)ticks * (float)(1.0 / 1000000.0);
-    const float rx = dt / dx;
-    const float ry = dt / dy;
+namespace {
 
-    const int64_t cells = (int64_t)nx * ny;
-    for (int64_t cell = 0; cell < cells; cell++) {
+// One step's cells [first, last) in cell order; returns the largest speed bits and adds the clamps.
+int32_t swe_cells(int nx, int ny, const int edges[4], float* out_h, float* out_hu, float* out_hv, const float* in_h,
+                  const float* in_hu, const float* in_hv, float g, float dry, float rx, float ry, int64_t first,
+                  int64_t last, int32_t speed_max, int64_t* clamps) {
+    for (int64_t cell = first; cell < last; cell++) {
         const int64_t x = cell % nx;
         const int64_t y = cell / nx;
         State here{in_h[cell], in_hu[cell], in_hv[cell]};
@@ -134,7 +122,7 @@ void orc_swe_step(int nx, int ny, const int edges[4], float* out_h, float* out_h
         float new_hu = (here.hu - rx * (fe.normal - fw.normal)) - ry * (fn.tangent - fs.tangent);
         float new_hv = (here.hv - rx * (fe.tangent - fw.tangent)) - ry * (fn.normal - fs.normal);
         // :218-225
-        if (raw_h < 0.0f) clamped[0] += 1;
+        if (raw_h < 0.0f) *clamps += 1;
         float new_h = raw_h < 0.0f ? 0.0f : raw_h;
         out_h[cell] = new_h;
         out_hu[cell] = new_hu;
@@ -144,11 +132,70 @@ void orc_swe_step(int nx, int ny, const int edges[4], float* out_h, float* out_h
         float w = velocity(new_h, new_hv, dry);
         float speed = mx(std::fabs(u), std::fabs(w)) + std::sqrt(g * new_h);
         int32_t bits = float_to_bits(speed);
-        if (bits > speed_out[0]) speed_out[0] = bits;
+        if (bits > speed_max) speed_max = bits;
     }
+    return speed_max;
+}
+
+// The step size of :184-192 (whole ticks, capped by the budget) and the step over the cells on
+// `threads` threads; each cell is computed alone, and the speed maximum and the clamp count do not
+// depend on the order, so every thread count gives the same bits.
+void swe_step(int nx, int ny, const int edges[4], float* out_h, float* out_hu, float* out_hv, const float* in_h,
+              const float* in_hu, const float* in_hv, const int32_t params[5], const int32_t* speed_in,
+              int32_t* speed_out, int32_t* speed_clear, const int32_t* budget_in, int32_t* budget_out,
+              int32_t* clamped, int threads) {
+    const float g = bits_to_float(params[0]);
+    const float courant = bits_to_float(params[1]);
+    const float dx = bits_to_float(params[2]);
+    const float dy = bits_to_float(params[3]);
+    const float dry = bits_to_float(params[4]);
+    const float fastest = bits_to_float(speed_in[0]);
+    const float allowed = mn((courant * mn(dx, dy)) / mx(fastest, 1e-30f), 2000.0f);
+    int32_t ticks = (int32_t)(allowed * 1000000.0f);
+    const int32_t budget = budget_in[0];
+    if (budget < ticks) ticks = budget;
+    const float dt = (float)ticks * (float)(1.0 / 1000000.0);
+    const float rx = dt / dx;
+    const float ry = dt / dy;
+    const int64_t cells = (int64_t)nx * ny;
+    int32_t speed = speed_out[0];
+    int64_t clamps = 0;
+    if (threads <= 1) {
+        speed = swe_cells(nx, ny, edges, out_h, out_hu, out_hv, in_h, in_hu, in_hv, g, dry, rx, ry, 0, cells, speed,
+                          &clamps);
+    } else {
+#pragma omp parallel num_threads(threads) reduction(max : speed) reduction(+ : clamps)
+        {
+            const int t = omp_get_thread_num(), n = omp_get_num_threads();
+            const int64_t first = cells * t / n, last = cells * (t + 1) / n;
+            speed = swe_cells(nx, ny, edges, out_h, out_hu, out_hv, in_h, in_hu, in_hv, g, dry, rx, ry, first, last,
+                              speed, &clamps);
+        }
+    }
+    speed_out[0] = speed;
+    clamped[0] += (int32_t)clamps;
     // :237-240
     speed_clear[0] = 0;
     budget_out[0] = budget - ticks;
+}
+
+}  // namespace
+
+void orc_swe_step(int nx, int ny, const int edges[4], float* out_h, float* out_hu, float* out_hv,
+                  const float* in_h, const float* in_hu, const float* in_hv, const int32_t params[5],
+                  const int32_t* speed_in, int32_t* speed_out, int32_t* speed_clear, const int32_t* budget_in,
+                  int32_t* budget_out, int32_t* clamped) {
+    swe_step(nx, ny, edges, out_h, out_hu, out_hv, in_h, in_hu, in_hv, params, speed_in, speed_out, speed_clear,
+             budget_in, budget_out, clamped, 1);
+}
+
+// The same step on `threads` host threads: bit-identical to orc_swe_step.
+void orc_swe_step_omp(int nx, int ny, const int edges[4], float* out_h, float* out_hu, float* out_hv,
+                      const float* in_h, const float* in_hu, const float* in_hv, const int32_t params[5],
+                      const int32_t* speed_in, int32_t* speed_out, int32_t* speed_clear, const int32_t* budget_in,
+                      int32_t* budget_out, int32_t* clamped, int threads) {
+    swe_step(nx, ny, edges, out_h, out_hu, out_hv, in_h, in_hu, in_hv, params, speed_in, speed_out, speed_clear,
+             budget_in, budget_out, clamped, threads);
 }
 
 // ShallowWater.fastestWave, :287-295 (float arithmetic with a double threshold and sqrt).

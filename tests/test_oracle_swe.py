@@ -149,3 +149,29 @@ def test_f64_twin_agrees_with_f32_on_a_smooth_hump():
     out64 = oracle.swe_step_f64(n, n, (0, 0, 0, 0), h, z, z, 9.81, 1.0, 1.0, 1e-6, float(np.float32(dt)))
     np.testing.assert_allclose(out32[0], out64[0], rtol=2e-6, atol=1e-6)
     np.testing.assert_allclose(out32[1], out64[1], rtol=1e-4, atol=2e-6)
+
+
+def test_threaded_step_is_bit_identical():
+    """orc_swe_step_omp (the step over host threads, for the 4096^2 parity run) gives the serial
+    step's bits: state, speed word, budget and clamp count, over 30 steps with a dry band and (at
+    Courant 0.9, past the stable bound) clamps, for several thread counts."""
+    nx, ny = 97, 61
+    y, x = np.mgrid[0:ny, 0:nx]
+    h = (1.0 + 0.6 * np.exp(-((x - 30.0) ** 2 + (y - 20.0) ** 2) / 40.0)).astype(np.float32).ravel()
+    h[(x.ravel() > 70)] = 0.0  # dry land to the east: the front clamps
+    hu = (0.3 * np.sin(x / 5.0)).astype(np.float32).ravel()
+    hv = (0.2 * np.cos(y / 7.0)).astype(np.float32).ravel()
+    params = oracle.swe_params(9.81, 0.9, 1.0 / nx, 1.0 / ny, allow_unstable=True)
+    first = oracle.swe_fastest_wave(h, hu, hv, 9.81, 1e-6)
+    runs = []
+    for threads in (1, 2, 5, 8):
+        s = oracle.SweStepper(nx, ny, (0, 1, 0, 0), threads=threads)
+        s.load(h, hu, hv, params, first)
+        s.write_budget(2**31 - 1)
+        s.step(30)
+        runs.append((s.read(), s.fastest_wave_bits(), s.read_budget(), s.clamped()))
+    for got in runs[1:]:
+        for a, b in zip(got[0], runs[0][0]):
+            np.testing.assert_array_equal(a.view(np.int32), b.view(np.int32))
+        assert got[1:] == runs[0][1:]
+    assert runs[0][3] > 0
