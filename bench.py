@@ -42,7 +42,16 @@ BYTES = {
 }
 
 
+# Passes that visit only the flagged bricks (3D) or the chunks P2G reached (batched 2D): their
+# BYTES are per node of the whole grid, an upper bound on what they move, so no rate is derived.
+SPARSE = {"mpm3_clear", "grid3", "grid3_tension", "tension3_stress", "tension3_blur", "batch2_grid", "batch2_clear"}
+
+
 def kernel_table(sim, names, units, bytes_override=None):
+    """Per-kernel launches and mean duration from the library's HIP-event timers (on its own
+    stream), with the rate algorithmic bytes / duration for dense passes. A sparse pass reports
+    its dense-grid bytes as a bound and no rate; a rate above the HBM peak (a working set served
+    from cache) is reported as none, with the reason."""
     out = {}
     for k in names:
         ms, n = sim.kernel_time(k)
@@ -51,9 +60,19 @@ def kernel_table(sim, names, units, bytes_override=None):
         unit, per = (bytes_override or {}).get(k, BYTES.get(k, ("none", 0)))
         count = units.get(unit, 0)
         avg_ms = ms / n
-        gbs = per * count / (avg_ms * 1e-3) / 1e9 if per and avg_ms > 0 else None
-        out[k] = {"launches": n, "avg_us": round(avg_ms * 1e3, 3), "total_ms": round(ms, 3),
-                  "algorithmic_bytes": per * count, "achieved_gbs": None if gbs is None else round(gbs, 1)}
+        entry = {"launches": n, "avg_us": round(avg_ms * 1e3, 3), "total_ms": round(ms, 3)}
+        if k in SPARSE:
+            entry["dense_bytes_bound"] = per * count
+            entry["achieved_gbs"] = None
+            entry["note"] = "visits flagged bricks or chunks only: no rate from dense-grid bytes"
+        else:
+            gbs = per * count / (avg_ms * 1e-3) / 1e9 if per and avg_ms > 0 else None
+            entry["algorithmic_bytes"] = per * count
+            if gbs is not None and gbs > PEAK_HBM_GBS:
+                entry["note"] = f"{gbs:.0f} GB/s of algorithmic bytes exceeds the HBM peak: served from cache"
+                gbs = None
+            entry["achieved_gbs"] = None if gbs is None else round(gbs, 1)
+        out[k] = entry
     return out
 
 
@@ -262,6 +281,126 @@ def bench_surrogate(iters, n=1024):
             "forward_tflops": round(flop / (fwd_ms * 1e-3) / 1e12, 1)}
 
 
+def oracle_native():
+    """Builds the oracle for the host it runs on (g++ -O3 -march=native -ffp-contract=off
+    -fopenmp, oracle/Makefile's `native` target) into a temporary directory and points the
+    oracle's loader at it, before the oracle is first imported. Returns the build's description;
+    when the build fails the prebuilt x86-64-v3 library is timed instead, and the description
+    says so."""
+    import subprocess
+    import tempfile
+
+    out = tempfile.mkdtemp(prefix="vexel_oracle_")
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native", f"DIR={out}"],
+                       capture_output=True, text=True)
+    if r.returncode == 0:
+        os.environ["VEXEL_ORACLE_LIB"] = os.path.join(out, "liboracle_native.so")
+        return "g++ -O3 -march=native -ffp-contract=off -fopenmp (oracle/Makefile native)"
+    return "prebuilt g++ -O3 -march=x86-64-v3 -ffp-contract=off (native build failed: " + r.stderr[-200:] + ")"
+
+
+def _median_run(run, reps=5):
+    """One warm-up call of `run`, then the median wall time of `reps` more."""
+    import statistics
+
+    run()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
+    return statistics.median(times)
+
+
+def _omp_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+
+
+def cpu_baseline_hll(n, steps_per_run=1):
+    """The oracle's HLL step (ShallowWater.kernel restated, one call per cell as the reference's
+    CPU Stepper makes them) on the same seeded dam-and-bumps scene: `steps_per_run` steps per
+    timed run, one warm-up run then the median of five, on one thread and on the host's CPU share
+    (orc_swe_step_omp: the same bits, rows split over threads)."""
+    import oracle
+    from vexhip import scenes
+
+    h, hu, hv = scenes.swe_dam_bumps(n, seed=0)
+    params = oracle.swe_params(9.81, 0.45, 1.0 / n, 1.0 / n)
+    first = oracle.swe_fastest_wave(h, hu, hv, 9.81, 1e-6)
+    out = []
+    for threads in (1, _omp_threads()):
+        st = oracle.SweStepper(n, n, threads=threads)
+        st.load(h, hu, hv, params, first)
+        st.write_budget(2**31 - 1)
+        t = _median_run(lambda: st.step(steps_per_run))
+        out.append({"value": round(n * n * steps_per_run / t, 1), "unit": "cell-updates/s", "cores": threads,
+                    "kind": "port", "seconds_per_run": round(t, 4),
+                    "sample": f"{steps_per_run} HLL step(s) of the {n}^2 scene per run, one warm-up run, median "
+                              f"of 5, on {threads} thread(s)"})
+    return out
+
+
+def cpu_baseline_small2(ppc=4, substeps=200):
+    """The 2D reference-sized scene (the 32x64-cell column on 128^2) through the oracle's
+    fixed-point substep, sorted every 10 as ParticleSimulation.advance sorts: runs of `substeps`,
+    one warm-up run then the median of five. On one thread; then the host's CPU share stepping
+    one independent copy each (the throughput a CPU gets from this scene size)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import oracle
+    from vexhip import particles, scenes
+
+    s = scenes.mpm2_dam_block(ppc=ppc)
+    count = len(s["x"])
+    h = 1.0 / 127
+    words = particles.params2(1e-4, 0, -9.8 / h, 10.0 / h ** 2, 1.0)
+    ref = oracle.Mpm2(128, 128, s, words, fixed=True)
+    t = _median_run(lambda: ref.advance(substeps, SORT_EVERY))
+    one = {"value": round(count * substeps / t, 1), "unit": "particle-substeps/s", "cores": 1, "kind": "port",
+           "sample": f"{substeps} fixed-point substeps of the {count}-particle scene per run (sort every 10), one "
+                     f"warm-up run, median of 5: {t:.3f} s"}
+    threads = _omp_threads()
+    copies = [oracle.Mpm2(128, 128, s, words, fixed=True) for _ in range(threads)]
+    with ThreadPoolExecutor(threads) as pool:
+        t = _median_run(lambda: list(pool.map(lambda r: r.advance(substeps, SORT_EVERY), copies)))
+    many = {"value": round(threads * count * substeps / t, 1), "unit": "particle-substeps/s", "cores": threads,
+            "kind": "port", "sample": f"{threads} independent copies of the scene, one a thread, {substeps} "
+                                      f"substeps each per run, one warm-up run, median of 5: {t:.3f} s"}
+    return one, many
+
+
+def cpu_baseline_batch(scenes=256, sample_every=4, substeps=2):
+    """The batched 2D workload through the oracle's fixed-point substep, every scene its own
+    state, sorted every 10: runs of `substeps` substeps of every scene, one warm-up run then the
+    median of five. On one thread over every `sample_every`-th scene (a bounded sample of the
+    same mix of sizes); on the host's CPU share over all scenes, a scene a task."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import oracle
+    from vexhip import scenes as sc_mod
+
+    layouts = sc_mod.mpm2_batch_layouts(scenes)
+    states = []
+    for k, l in enumerate(layouts):
+        fields, words = sc_mod.mpm2_batch_scene(l, 1000 + k)
+        states.append(oracle.Mpm2(l["n"], l["n"], fields, words, fixed=True))
+    sample = states[::sample_every]
+    part = sum(r.count for r in sample)
+    t = _median_run(lambda: [r.advance(substeps, SORT_EVERY) for r in sample])
+    one = {"value": round(part * substeps / t, 1), "unit": "particle-substeps/s", "cores": 1, "kind": "port",
+           "sample": f"every {sample_every}th scene ({len(sample)} scenes, {part} particles), {substeps} "
+                     f"fixed-point substeps a run (sort every 10), one warm-up run, median of 5: {t:.3f} s"}
+    threads = _omp_threads()
+    total = sum(r.count for r in states)
+    big_first = sorted(states, key=lambda r: -r.count)
+    with ThreadPoolExecutor(threads) as pool:
+        t = _median_run(lambda: list(pool.map(lambda r: r.advance(substeps, SORT_EVERY), big_first)))
+    many = {"value": round(total * substeps / t, 1), "unit": "particle-substeps/s", "cores": threads,
+            "kind": "port", "sample": f"all {len(states)} scenes ({total} particles) on {threads} threads, a scene "
+                                      f"a task, {substeps} substeps a run, one warm-up run, median of 5: {t:.3f} s"}
+    return one, many
+
+
 def cpu_baseline_mpm3(threads=None):
     """The oracle on whole substeps of the same 8,388,608-particle workload, one warm-up substep
     then the median of five: the scalar restatement on one thread (about 2 s a substep, some
@@ -322,17 +461,32 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 3}), flush=True)
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the counter summary kept beside this file
-    (profiles/traffic.json, written from two rocprofv3 --pmc passes and the copy calibration);
-    None when the kernel has no entry."""
-    path = os.path.join(ROOT, "profiles", "traffic.json")
+TRAFFIC_SOURCE = ("profiles/traffic.json: committed rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over this "
+                  "command, each scaled by a plain copy's reading; not measured in this run")
+
+
+def profile_entry(name, kernel):
+    """A kernel's entry in a counter summary committed under profiles/ (None when absent)."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        table = json.load(f).get("kernels", {})
-    entry = table.get(kernel)
+        return json.load(f).get("kernels", {}).get(kernel)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from profiles/traffic.json (two rocprofv3 --pmc passes
+    and the copy calibration); None when the kernel has no entry."""
+    entry = profile_entry("traffic.json", kernel)
     return None if entry is None else entry.get("hbm_bytes_per_launch")
+
+
+def roofline_of(kernel, entry):
+    gbs = entry.get("achieved_gbs")
+    return {"kernel": kernel, "bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": None if gbs is None else round(gbs / PEAK_HBM_GBS, 4), "traffic": measured_traffic(kernel),
+            "traffic_source": TRAFFIC_SOURCE, "bytes_per_launch": entry.get("algorithmic_bytes"),
+            "avg_us": entry["avg_us"]}
 
 
 def main():
@@ -353,6 +507,8 @@ def main():
     ap.add_argument("--tension-steps", type=int, default=50)
     ap.add_argument("--no-lod", action="store_true")
     ap.add_argument("--no-tension", action="store_true")
+    ap.add_argument("--atomic-steps", type=int, default=20)
+    ap.add_argument("--no-atomic", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (counter calibration) and exit")
     args = ap.parse_args()
@@ -377,10 +533,7 @@ def main():
     value = count * args.steps / secs
     dominant = max(table.items(), key=lambda kv: kv[1]["total_ms"])
     dk, dv = dominant
-    roofline = {"kernel": dk, "bound": "hbm", "achieved": dv["achieved_gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": None if dv["achieved_gbs"] is None else round(dv["achieved_gbs"] / PEAK_HBM_GBS, 4),
-                "traffic": measured_traffic(dk), "bytes_per_launch": dv["algorithmic_bytes"],
-                "avg_us": dv["avg_us"]}
+    roofline = roofline_of(dk, dv)
     line = {
         "metric": "particle-substeps/s (3D MPM) & cell-updates/s (HLL); surrogate vs HLL at ≤1% err",
         "value": round(value, 1), "unit": "particle-substeps/s", "n_gpus": 1, "steps": args.steps,
@@ -393,6 +546,22 @@ def main():
                    "run": args.run},
         "roofline": roofline, "kernels": table, "grid_mass_drift": drift,
     }
+    if not args.no_atomic:
+        print(f"[bench] 3D MPM with the direct atomic P2G: same scene, {args.atomic_steps} steps", file=sys.stderr,
+              flush=True)
+        acount, asecs, atable, adrift = bench_mpm3(args.atomic_steps, 3, 0, args.g2p, args.run)
+        ak = atable["p2g3_atomic"]
+        atomics = 108 * acount
+        counted = profile_entry("atomics.json", "p2g3_atomic")
+        line["p2g_atomic"] = {
+            "workload": "the same 3D substep with P2G as the reference schedules it (Flip3.java:193-246): a "
+                        "thread a particle, its 27 nodes x 4 sums straight into the grid by float atomics; "
+                        "sorted every 10 substeps like the headline",
+            "value": round(acount * args.atomic_steps / asecs, 1), "unit": "particle-substeps/s",
+            "steps": args.atomic_steps, "ms_per_step": round(asecs / args.atomic_steps * 1e3, 4),
+            "roofline": roofline_of("p2g3_atomic", ak),
+            "atomics_per_launch": atomics, "atomics_per_s": round(atomics / (ak["avg_us"] * 1e-6), 1),
+            "atomics_counted": counted, "kernels": atable, "grid_mass_drift": adrift}
     if not args.no_tension:
         print(f"[bench] 3D MPM with tension: same scene, {args.tension_steps} steps", file=sys.stderr, flush=True)
         tcount, tsecs, ttable, tdrift = bench_mpm3(args.tension_steps, 5, args.p2g, args.g2p, args.run, tension=True)
@@ -418,11 +587,7 @@ def main():
         hk = htable["hll_step"]
         line["hll"] = {"value": round(rate, 1), "unit": "cell-updates/s", "steps": args.hll_steps, "rows": hrows,
                        "workload": "HLL 4096^2 circular dam + 32 seeded bumps, walls, CFL 0.45",
-                       "roofline": {"kernel": "hll_step", "bound": "hbm", "achieved": hk["achieved_gbs"],
-                                    "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                                    "frac": round(hk["achieved_gbs"] / PEAK_HBM_GBS, 4),
-                                    "traffic": measured_traffic("hll_step"),
-                                    "avg_us": hk["avg_us"]}}
+                       "roofline": roofline_of("hll_step", hk)}
     if not args.no_batch:
         print(f"[bench] batched 2D: 256 scenes, {args.batch_steps} steps", file=sys.stderr, flush=True)
         line["batch2d"] = bench_batch2(args.batch_steps)
@@ -438,9 +603,28 @@ def main():
               file=sys.stderr, flush=True)
         line["surrogate"] = bench_surrogate(args.surrogate_iters)
     if not args.no_cpu:
-        print("[bench] CPU oracle baseline (median of 5 substeps on one thread, then on the host's CPU share)",
-              file=sys.stderr, flush=True)
+        build = oracle_native()
+        print(f"[bench] CPU oracle baselines ({build}): 3D substeps, median of 5, one thread then the host's "
+              "CPU share", file=sys.stderr, flush=True)
         line["cpu_baseline"], line["cpu_baseline_threaded"] = cpu_baseline_mpm3()
+        line["cpu_baseline"]["build"] = line["cpu_baseline_threaded"]["build"] = build
+        if "hll" in line:
+            print("[bench] CPU oracle baseline: HLL 4096^2", file=sys.stderr, flush=True)
+            line["hll"]["cpu_baseline"], line["hll"]["cpu_baseline_threaded"] = cpu_baseline_hll(4096)
+        if "small2d" in line:
+            print("[bench] CPU oracle baseline: 2D 8,192 particles", file=sys.stderr, flush=True)
+            line["small2d"]["cpu_baseline"], line["small2d"]["cpu_baseline_threaded"] = cpu_baseline_small2()
+        if "batch2d" in line:
+            print("[bench] CPU oracle baseline: batched 2D", file=sys.stderr, flush=True)
+            line["batch2d"]["cpu_baseline"], line["batch2d"]["cpu_baseline_threaded"] = cpu_baseline_batch()
+        if "surrogate" in line:
+            print("[bench] CPU oracle baseline: 8 HLL steps at 1024^2", file=sys.stderr, flush=True)
+            one, many = cpu_baseline_hll(1024, steps_per_run=8)
+            line["surrogate"]["cpu_hll_8_steps"] = {
+                "one_thread_ms": round(one["seconds_per_run"] * 1e3, 3),
+                "threaded_ms": round(many["seconds_per_run"] * 1e3, 3), "threads": many["cores"],
+                "sample": "8 FP32 HLL steps of the 1024^2 dam-and-bumps scene (seed 0) by the oracle, one warm-up "
+                          "run, median of 5"}
     print(json.dumps(line), flush=True)
 
 

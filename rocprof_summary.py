@@ -1,8 +1,9 @@
 """Condenses rocprofv3 CSV output into the short summaries kept under profiles/.
 
-    python -m vexhip.rocprof_summary stats  <rocprof-dir> <out.txt> "<command that made it>"
-    python -m vexhip.rocprof_summary traffic <fetch-dir> <write-dir> <cal-fetch-dir> <cal-write-dir> \
+    python rocprof_summary.py stats  <rocprof-dir> <out.txt> "<command that made it>"
+    python rocprof_summary.py traffic <fetch-dir> <write-dir> <cal-fetch-dir> <cal-write-dir> \
         <copy-bytes> <out.json> "<commands>"
+    python rocprof_summary.py counts <pmc-dir> <out.json> "<command that made it>" [<kernel-trace-dir>]
 
 ``stats`` reads run_kernel_stats.csv (rocprofv3 --kernel-trace --stats --output-format csv) and
 writes one line per kernel: launches, average/min/max duration and share of GPU time.
@@ -11,6 +12,10 @@ writes one line per kernel: launches, average/min/max duration and share of GPU 
 --output-format csv, KiB per dispatch) of the workload and of a plain device copy of
 ``copy-bytes`` each way, scales each counter by (copy bytes / what the counter read for the copy)
 and writes HBM bytes per launch for every kernel, the scale factors beside them.
+
+``counts`` reads one --pmc pass with any counters and writes, per kernel and counter, the mean
+value per dispatch; given the kernel-trace directory of a run of the same command it adds each
+kernel's mean duration and every counter as a rate per second of kernel time.
 """
 import collections
 import csv
@@ -33,6 +38,8 @@ _SHORT = {
     "stress_rows_kernel": "tension3_stress", "region_kernel": "tension3_region", "lod_target_kernel": "lod3_target",
     "lod_kernel": "lod3", "after_sort_kernel": "mpm3_after_sort", "slots_kernel": "mpm3_slots",
     "fill_zero_kernel": "hll_fill_zero", "f64_to_f32_kernel": "f64_to_f32", "batch2_flags_kernel": "batch2_flags",
+    "p2g3_layer_kernel": "p2g3_layer", "group_count_kernel": "layers3_count", "group_scan_kernel": "layers3_scan",
+    "group_list_kernel": "layers3_list", "group_order_kernel": "layers3_order",
 }
 
 
@@ -99,10 +106,37 @@ def traffic(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, copy_bytes, out,
         fh.write("\n")
 
 
+def counts(directory, out, command, trace_dir=None):
+    per = collections.defaultdict(list)
+    with open(os.path.join(directory, "run_counter_collection.csv")) as f:
+        for r in csv.DictReader(f):
+            per[(r["Kernel_Name"], r["Counter_Name"])].append(float(r["Counter_Value"]))
+    durations = {}
+    if trace_dir:
+        with open(os.path.join(trace_dir, "run_kernel_stats.csv")) as f:
+            for r in csv.DictReader(f):
+                durations[r["Name"]] = float(r["AverageNs"]) * 1e-9
+    kernels = {}
+    for (k, counter), values in sorted(per.items()):
+        if "rocclr" in k:
+            continue
+        entry = kernels.setdefault(short_name(k), {"kernel": k, "dispatches": len(values)})
+        mean = sum(values) / len(values)
+        entry[counter] = round(mean, 1)
+        if k in durations:
+            entry["avg_seconds"] = durations[k]
+            entry[counter + "_per_s"] = round(mean / durations[k], 1)
+    with open(out, "w") as fh:
+        json.dump({"command": command, "kernels": kernels}, fh, indent=1)
+        fh.write("\n")
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "stats":
         stats(*sys.argv[2:5])
     elif sys.argv[1] == "traffic":
         traffic(*sys.argv[2:9])
+    elif sys.argv[1] == "counts":
+        counts(*sys.argv[2:6])
     else:
         raise SystemExit(__doc__)
