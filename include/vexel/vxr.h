@@ -187,14 +187,24 @@ int vxr_stream(vxr_sim* sim, void** stream);
  *                registers (3D: then a wave-private LDS node tile), one atomic per node and field;
  *                2 = 2D only, and always for a batch: fixed point, each amount rounded to a 64-bit
  *                integer (mass 2^-40, momentum 2^-32) and summed by integer atomics -- the same
- *                bits in any order, equal to the CPU restatement's
+ *                bits in any order, equal to the CPU restatement's;
+ *                3 = MPM3 only: cell-ranked groups -- the sort also lays each 256-cell group out by
+ *                cells ranked by particle count, a wave takes a group with a lane a cell, folds
+ *                each cell's particles in registers and adds them into an LDS node tile in 27
+ *                conflict-free steps; a particle that changed cell since the sort goes by direct
+ *                atomics, and until the first sort (or after a write of positions, masses, slots
+ *                or keys, or a lod pass) the scatter runs as option 1
  *   "g2p"        MPM3: 0 = per-particle gathers and 1 = an LDS node tile, both in the reference's
  *                arithmetic order (the CPU restatement's bits); 2 = the LDS tile with the sums
  *                contracted axis by axis in fused multiply-adds (default, within a few ulps)
  *   "run"        MPM: consecutive particles a thread takes in the segmented passes: 1, 2 or 4
  *                (default 4)
  *   "radix_bits" sort digit width (4..8, default 8)
- *   "rows"       SWE: rows a wave marches per tile (4..256, default 32)
+ *   "rows"       SWE: rows a wave marches per tile (1..256, default 32)
+ *   "hll_fast"   SWE: 1 = the step with one refined hardware reciprocal per face and per cell in
+ *                place of its correctly rounded divisions, and the hardware square root: fewer
+ *                instructions, within a few ulps a step of the reference's bits (default 0: the
+ *                reference's bits)
  *   "blocks"     batch passes: blocks per launch, each striding over chunks (0 = one per chunk)
  *   "graph"      1 = vxr_advance replays captured hipGraphs (default 0) */
 int vxr_set_option(vxr_sim* sim, const char* key, int64_t value);

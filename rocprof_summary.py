@@ -28,7 +28,7 @@ _SHORT = {
     "g2p3_tiled_kernel": "g2p3_tiled", "g2p3_kernel": "g2p3", "grid3_kernel": "grid3", "clear4_kernel": "mpm3_clear",
     "keys3_kernel": "keys3", "hist_kernel": "radix_hist", "scan_rows_kernel": "radix_scan",
     "scatter_kernel": "radix_scatter", "permute_kernel": "permute", "hll_step_kernel": "hll_step",
-    "clear_rows_kernel": "mpm3_clear", "flags_kernel": "grid3_flags", "batch2_p2g": "batch2_p2g",
+    "clear_rows_kernel": "mpm3_clear", "hll_step_kernel<false>": "hll_step", "hll_step_kernel<true>": "hll_step_fast", "flags_kernel": "grid3_flags", "batch2_p2g": "batch2_p2g",
     "batch2_grid": "batch2_grid", "batch2_g2p": "batch2_g2p", "batch2_clear": "batch2_clear",
     "batch2_keys": "batch2_keys", "batch2_starts": "batch2_starts", "conv3x3_lds_kernel": "conv3x3",
     "conv3x3_wgrad_kernel": "conv3x3_wgrad", "wgrad_reduce_kernel": "wgrad_reduce", "hll64_kernel": "hll64",

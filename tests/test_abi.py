@@ -1,4 +1,4 @@
-"""The C ABI library without a GPU: it loads, exports exactly what include/vexel/vxr.h declares,
+"""The C ABI library without a GPU: it loads, exports exactly what include/vexel/vxr.h and vxs.h declare,
 and refuses bad arguments before touching a device. The runners' host logic is checked here too."""
 import ctypes
 import os
@@ -11,27 +11,33 @@ import pytest
 from vexhip import _native, clock, patch, particles
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "vexel", "vxr.h")
+HEADERS = {"vxr": os.path.join(ROOT, "include", "vexel", "vxr.h"),
+           "vxs": os.path.join(ROOT, "include", "vexel", "vxs.h")}
 
 
-def declared():
-    text = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(vxr_[a-z_]+)\s*\(", text)))
+def declared(prefix):
+    text = open(HEADERS[prefix]).read()
+    return sorted(set(re.findall(rf"\b({prefix}_[a-z0-9_]+)\s*\(", text)))
 
 
-def test_library_exports_every_declared_symbol():
-    names = declared()
-    assert "vxr_create" in names and "vxr_swe_dispatch" in names
+@pytest.mark.parametrize("prefix,must", [("vxr", ("vxr_create", "vxr_swe_dispatch")),
+                                         ("vxs", ("vxs_conv3x3", "vxs_hll32", "vxs_fallback_check"))])
+def test_library_exports_every_declared_symbol(prefix, must):
+    """Every entry point a header declares is exported, and the library exports no other of
+    that prefix."""
+    names = declared(prefix)
+    assert all(m in names for m in must)
     lib = _native.lib()
     for n in names:
         assert hasattr(lib, n), n
     out = subprocess.run(["nm", "-D", "--defined-only", _native.LIB_PATH], capture_output=True, text=True).stdout
-    exported = sorted(set(re.findall(r"\b(vxr_[a-z_]+)\b", out)))
+    exported = sorted(set(re.findall(rf"\b({prefix}_[a-z0-9_]+)\b", out)))
     assert exported == names
 
 
-def test_header_compiles_as_c():
-    src = '#include "vexel/vxr.h"\nint main(void){ vxr_sim* s = 0; return vxr_destroy(s); }\n'
+@pytest.mark.parametrize("name", ["vxr.h", "vxs.h"])
+def test_header_compiles_as_c(name):
+    src = f'#include "vexel/{name}"\nint main(void){{ return 0; }}\n'
     r = subprocess.run(["gcc", "-x", "c", "-std=c99", "-Wall", "-Werror", "-fsyntax-only",
                         "-I", os.path.join(ROOT, "include"), "-"], input=src, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr

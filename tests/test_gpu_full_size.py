@@ -3,7 +3,8 @@
 - 3D MLS-MPM, 8,388,608 particles on 256^3: one substep per kernel against the oracle -- every P2G
   schedule's grid within FP32 reassociation, the grid update bit for bit from the oracle's sums,
   G2P bit for bit (g2p=1) and within 1e-5 (g2p=2) from the oracle's velocities.
-- HLL on 4096^2: 100 steps bit for bit (the oracle's step on host threads).
+- HLL on 4096^2: 100 steps bit for bit (the oracle's step on host threads), and with hll_fast
+  within 1e-5.
 - 256 batched 2D scenes: 20 steps with graphs and the autotuner; every scene's mass, and the
   largest, the largest-grid and the smallest scene bit for bit against single-scene runs and the
   oracle.
@@ -105,28 +106,47 @@ def test_mpm3_two_cubes_substep_per_kernel(gpu):
     sim.close()
 
 
+def _worst(got, exp):
+    err = np.abs(got.astype(np.float64) - exp.astype(np.float64))
+    return float(np.max(err / np.maximum(1.0, np.abs(exp.astype(np.float64)))))
+
+
 def test_hll_4096_hundred_steps_bit_exact(gpu):
+    """100 steps: the default step bit for bit, and the step with hll_fast (refined hardware
+    reciprocals and square roots) within 1e-5 * max(1, |e|) of the same oracle run, on the same
+    clock (its budget word equal to the oracle's)."""
     n = 4096
     h, hu, hv = scenes.swe_dam_bumps(n, seed=0)
     words = patch.params(9.81, 0.45, 1.0 / n, 1.0 / n)
     first = patch.fastest_wave(h, hu, hv, 9.81, 1e-6)
-    sim = patch.PatchSimulation(n, n)
-    sim.load(h, hu, hv, words)
-    sim.write_budget(2**31 - 1)
-    sim.dispatch(100)
-    got = sim.read()
-    got_speed = int(sim.sim.read_words("speed")[0])
-    got_clamped = int(sim.sim.read_words("clamped")[0])
-    sim.close()
+    runs = {}
+    for fast in (0, 1):
+        print(f"[full] HLL 4096^2 hll_fast={fast}", flush=True)
+        sim = patch.PatchSimulation(n, n)
+        sim.sim.set_option("hll_fast", fast)
+        sim.load(h, hu, hv, words)
+        sim.write_budget(2**31 - 1)
+        sim.dispatch(100)
+        runs[fast] = (sim.read(), int(sim.sim.read_words("speed")[0]), int(sim.sim.read_words("clamped")[0]),
+                      sim.read_budget())
+        sim.close()
     print(f"[full] HLL 4096^2: the oracle on {THREADS} threads", flush=True)
     ref = oracle.SweStepper(n, n, threads=THREADS)
     ref.load(h, hu, hv, words, first)
     ref.write_budget(2**31 - 1)
     ref.step(100)
-    for a, b, name in zip(got, ref.read(), ("h", "hu", "hv")):
+    exp = ref.read()
+    got, got_speed, got_clamped, _ = runs[0]
+    for a, b, name in zip(got, exp, ("h", "hu", "hv")):
         _bits_equal(a, b, name)
     assert got_speed == ref.fastest_wave_bits()
     assert got_clamped == ref.clamped()
+    fast, _, fast_clamped, fast_budget = runs[1]
+    assert fast_budget == ref.read_budget()
+    for a, b, name in zip(fast, exp, ("h", "hu", "hv")):
+        print(f"[full] hll_fast {name}: worst {_worst(a, b):.2e} of max(1, |e|)", flush=True)
+        _close(a, b, f"hll_fast {name}")
+    assert fast_clamped == ref.clamped()
 
 
 def _single2(n, scene, words, steps, sort_every, p2g=2):

@@ -75,7 +75,7 @@ def test_bit_exact_dry_front_1d(gpu):
     _parity(200, 1, (0, 0, 0, 0), h, z, z, 100, dx=0.5)
 
 
-@pytest.mark.parametrize("rows", [4, 7, 32, 256])
+@pytest.mark.parametrize("rows", [1, 2, 4, 7, 32, 256])
 def test_bit_exact_bumps_mixed_edges(gpu, rows):
     """A ragged grid (not a multiple of the 62-column strip or of the rows a wave marches),
     open and wall edges mixed, currents in both directions, a dry corner."""
@@ -100,6 +100,55 @@ def test_bit_exact_dam_and_bumps_512(gpu):
     100 steps, against the oracle bit for bit."""
     h, hu, hv = scenes.swe_dam_bumps(512, seed=0)
     _parity(512, 512, (0, 0, 0, 0), h, hu, hv, 100, courant=0.45, dx=1.0 / 512)
+
+
+def _fast_within(nx, ny, edges, h, hu, hv, steps, dx, rows=None, rel=1e-5):
+    """hll_fast (refined hardware reciprocals and square roots in place of the correctly rounded
+    divisions and roots) against the oracle's exact step: every field within rel * max(1, |e|)
+    after `steps` steps. The clock may slip: a step's ticks are the floor of a quotient of the
+    fastest wave speed, which the fast step gives to within a few ulps, so where that quotient
+    lies within a few ulps of a whole tick the two steps round apart by one tick (1e-6 s)."""
+    params = oracle.swe_params(9.81, 0.45, dx, dx)
+    first = oracle.swe_fastest_wave(h, hu, hv, 9.81, 1e-6)
+    cpu = oracle.SweStepper(nx, ny, edges)
+    cpu.load(h, hu, hv, params, first)
+    cpu.write_budget(2**31 - 1)
+    cpu.step(steps)
+    dev = GpuSwe(nx, ny, edges, rows=rows)
+    dev.sim.set_option("hll_fast", 1)
+    dev.load(h, hu, hv, params, first)
+    dev.write_budget(2**31 - 1)
+    dev.step(steps)
+    slip = dev.read_budget() - cpu.read_budget()
+    assert abs(slip) <= steps
+    worst = 0.0
+    for f, (a, b) in enumerate(zip(dev.read(), cpu.read())):
+        err = np.abs(a.astype(np.float64) - b.astype(np.float64)) / np.maximum(1.0, np.abs(b.astype(np.float64)))
+        worst = max(worst, float(err.max()))
+        assert float(err.max()) <= rel, f"field {f}: worst {float(err.max()):.3e}"
+    print(f"[swe fast] {nx}x{ny} {steps} steps: worst {worst:.2e} of max(1, |e|), clock slip {slip} ticks",
+          flush=True)
+    return worst
+
+
+def test_fast_dam_and_bumps_512(gpu):
+    """The benchmark's scene at 512^2, 100 steps with hll_fast: within 1e-5 of the reference's bits."""
+    h, hu, hv = scenes.swe_dam_bumps(512, seed=0)
+    worst = _fast_within(512, 512, (0, 0, 0, 0), h, hu, hv, 100, 1.0 / 512)
+    assert worst > 0.0  # the fast step does differ: the option reaches the kernel
+
+
+@pytest.mark.parametrize("rows", [2, 32])
+def test_fast_mixed_edges_and_dry_band(gpu, rows):
+    """hll_fast on the ragged grid with open and wall edges and a dry band (reciprocals of dry
+    depths discarded), 40 steps."""
+    nx, ny = 131, 77
+    r = np.random.default_rng(3)
+    h = (1 + 0.3 * r.random(nx * ny)).astype(np.float32)
+    h[: nx * 5] = 0
+    hu = (0.2 * r.standard_normal(nx * ny)).astype(np.float32) * (h > 0)
+    hv = (0.2 * r.standard_normal(nx * ny)).astype(np.float32) * (h > 0)
+    _fast_within(nx, ny, (1, 0, 0, 1), h, hu, hv, 40, 1.0, rows=rows)
 
 
 def test_full_size_4096_conserves_and_repeats(gpu):

@@ -97,7 +97,8 @@ struct SweArgs {
     // When set, the step runs only if *gate is nonzero (every wave leaves at once otherwise).
     const int32_t* gate;
 };
-int swe_step(const Launch& l, const SweArgs& a, int rows);
+// One HLL step; `fast` trades the reference's bits for the hardware reciprocal and square root.
+int swe_step(const Launch& l, const SweArgs& a, int rows, bool fast = false);
 
 // ------------------------------------------------------------------ fill
 

@@ -11,6 +11,16 @@
 //
 // The fused step-size reduction (:227-235) is a wave max of the new state's wave speed and
 // one atomic max per wave; the clamp count (:218-219) is a wave sum and one atomic add.
+//
+// The step is bound by its VALU work, not by HBM: about 357 vector instructions a cell, most of
+// them in its 10 correctly rounded divisions and 2 square roots. kFast = true trades the
+// reference's bits for fewer: one reciprocal (the hardware's, refined by a Newton step) per
+// face for its three flux quotients and one per cell for its two velocities, and the hardware
+// square root. Its results differ from the reference's by a few ulps a step.
+//
+// Strips are dealt to workgroups so that each of the 8 XCDs takes a contiguous run of them (the
+// hardware dispatches workgroups to XCDs in turn): the halo columns and rows a strip shares with
+// its neighbours then sit in the same XCD's L2.
 #include "common.h"
 
 namespace vxr {
@@ -30,14 +40,33 @@ __device__ __forceinline__ float bits_f(int32_t w) { return __int_as_float(w); }
 // ShallowWater.velocity, :329-333
 __device__ __forceinline__ float velocity(float h, float q, float dry) { return h > dry ? q / h : 0.0f; }
 
+// 1/d from the hardware reciprocal (about 1 ulp) and one Newton step.
+__device__ __forceinline__ float recip(float d) {
+    const float r = __builtin_amdgcn_rcpf(d);
+    return fmaf(fmaf(-d, r, 1.0f), r, r);
+}
+
+template <bool kFast>
+__device__ __forceinline__ float root(float v) {
+    if constexpr (kFast) return __builtin_amdgcn_sqrtf(v);
+    return sqrtf(v);
+}
+
+template <bool kFast>
 __device__ __forceinline__ Cell make_cell(float h, float hu, float hv, float g, float dry) {
     Cell s;
     s.h = h;
     s.hu = hu;
     s.hv = hv;
-    s.u = velocity(h, hu, dry);
-    s.v = velocity(h, hv, dry);
-    s.c = sqrtf(g * h);
+    if constexpr (kFast) {
+        const float inv = recip(h);
+        s.u = h > dry ? hu * inv : 0.0f;
+        s.v = h > dry ? hv * inv : 0.0f;
+    } else {
+        s.u = velocity(h, hu, dry);
+        s.v = velocity(h, hv, dry);
+    }
+    s.c = root<kFast>(g * h);
     return s;
 }
 
@@ -55,13 +84,16 @@ __device__ __forceinline__ Cell mirror(Cell s, bool normal_is_x, float dry) {
     return s;
 }
 
-// ShallowWater.combine, :381-384
+// ShallowWater.combine, :381-384; with kFast, `span` is 1/span and multiplies
+template <bool kFast>
 __device__ __forceinline__ float combine(float hi, float lo, float lohi, float span, float fl, float fr, float ul,
                                          float ur) {
+    if constexpr (kFast) return ((hi * fl - lo * fr) + lohi * (ur - ul)) * span;
     return ((hi * fl - lo * fr) + lohi * (ur - ul)) / span;
 }
 
 // ShallowWater.hll, :336-378
+template <bool kFast>
 __device__ __forceinline__ Flux hll(const Cell& l, const Cell& r, bool normal_is_x, float g, float dry) {
     const float qnL = normal_is_x ? l.hu : l.hv;
     const float qtL = normal_is_x ? l.hv : l.hu;
@@ -85,14 +117,15 @@ __device__ __forceinline__ Flux hll(const Cell& l, const Cell& r, bool normal_is
     }
     const float lo = mn(sL, 0.0f);
     const float hi = mx(sR, 0.0f);
-    const float span = mx(hi - lo, 1e-30f);
+    float span = mx(hi - lo, 1e-30f);
+    if constexpr (kFast) span = recip(span);
     const float lohi = lo * hi;
     const float pL = (0.5f * g) * (l.h * l.h);
     const float pR = (0.5f * g) * (r.h * r.h);
     Flux f;
-    f.mass = combine(hi, lo, lohi, span, qnL, qnR, l.h, r.h);
-    f.normal = combine(hi, lo, lohi, span, qnL * unL + pL, qnR * unR + pR, qnL, qnR);
-    f.tangent = combine(hi, lo, lohi, span, qnL * utL, qnR * utR, qtL, qtR);
+    f.mass = combine<kFast>(hi, lo, lohi, span, qnL, qnR, l.h, r.h);
+    f.normal = combine<kFast>(hi, lo, lohi, span, qnL * unL + pL, qnR * unR + pR, qnL, qnR);
+    f.tangent = combine<kFast>(hi, lo, lohi, span, qnL * utL, qnR * utR, qtL, qtR);
     return f;
 }
 
@@ -109,16 +142,28 @@ __device__ __forceinline__ Cell shfl_down_cell(const Cell& s) {
 
 constexpr int kStrip = kWave - 2;  // cells a wave owns across
 constexpr int kWavesPerBlock = 4;
+constexpr int kXcds = 8;
 
-__global__ __launch_bounds__(kWave * kWavesPerBlock) void hll_step_kernel(SweArgs a, int rows) {
+// Workgroup `id` of a launch of `per` * kXcds: XCD id % kXcds takes the run of tiles
+// [(id % kXcds) * per, (id % kXcds + 1) * per), strips across first, then bands.
+__device__ __forceinline__ int64_t tile_of_workgroup(int64_t id, int64_t per) {
+    return (id % kXcds) * per + id / kXcds;
+}
+
+template <bool kFast>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void hll_step_kernel(SweArgs a, int rows, int strips,
+                                                                          int64_t tiles, int64_t per) {
     const int nx = a.nx, ny = a.ny;
     const int lane = threadIdx.x & (kWave - 1);
-    const int band = blockIdx.y * kWavesPerBlock + (threadIdx.x >> 6);
+    const int64_t tile = tile_of_workgroup(blockIdx.x, per);
+    if (tile >= tiles) return;  // the last XCD's run may be short; uniform across the block
+    const int strip = (int)(tile % strips);
+    const int band = (int)(tile / strips) * kWavesPerBlock + (threadIdx.x >> 6);
     const int y_begin = band * rows;
     if (y_begin >= ny) return;  // uniform across the wave; no barrier follows
     if (a.gate && *a.gate == 0) return;  // a gated step not called for
     const int y_end = min(y_begin + rows, ny);
-    const int x = blockIdx.x * kStrip + lane - 1;
+    const int x = strip * kStrip + lane - 1;
     const int xc = min(max(x, 0), nx - 1);
     const bool owner = lane >= 1 && lane <= kStrip && x < nx;
 
@@ -148,12 +193,12 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void hll_step_kernel(SweArg
 
     auto load = [&](int y) {
         const size_t i = (size_t)y * nx + xc;
-        return make_cell(ih[i], ihu[i], ihv[i], g, dry);
+        return make_cell<kFast>(ih[i], ihu[i], ihv[i], g, dry);
     };
 
     Cell cur = load(y_begin);
     Cell south = y_begin == 0 ? (wall_s ? mirror(cur, false, dry) : cur) : load(y_begin - 1);
-    Flux fs = hll(south, cur, false, g, dry);
+    Flux fs = hll<kFast>(south, cur, false, g, dry);
     Cell nrt = y_begin + 1 < ny ? load(y_begin + 1) : cur;
 
     int32_t speed_max = 0;
@@ -169,16 +214,16 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void hll_step_kernel(SweArg
             phv = ihv[i];
         }
         const Cell north = y + 1 < ny ? nrt : (wall_n ? mirror(cur, false, dry) : cur);
-        const Flux fn = hll(cur, north, false, g, dry);
+        const Flux fn = hll<kFast>(cur, north, false, g, dry);
 
         Cell right = shfl_down_cell(cur);
         if (x == nx - 1) right = wall_e ? mirror(cur, true, dry) : cur;
-        const Flux fe = hll(cur, right, true, g, dry);
+        const Flux fe = hll<kFast>(cur, right, true, g, dry);
         Flux fw;
         fw.mass = __shfl_up(fe.mass, 1);
         fw.normal = __shfl_up(fe.normal, 1);
         fw.tangent = __shfl_up(fe.tangent, 1);
-        if (x == 0) fw = hll(wall_w ? mirror(cur, true, dry) : cur, cur, true, g, dry);
+        if (x == 0) fw = hll<kFast>(wall_w ? mirror(cur, true, dry) : cur, cur, true, g, dry);
 
         if (owner) {
             // :205-225; x-faces carry (mass, normal, tangent) = (h, hu, hv), y-faces (h, hv, hu)
@@ -192,9 +237,16 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void hll_step_kernel(SweArg
             a.out[1][i] = new_hu;
             a.out[2][i] = new_hv;
             // :230-235
-            const float u = velocity(new_h, new_hu, dry);
-            const float w = velocity(new_h, new_hv, dry);
-            const float speed = mx(fabsf(u), fabsf(w)) + sqrtf(g * new_h);
+            float u, w;
+            if constexpr (kFast) {
+                const float inv = recip(new_h);
+                u = new_h > dry ? new_hu * inv : 0.0f;
+                w = new_h > dry ? new_hv * inv : 0.0f;
+            } else {
+                u = velocity(new_h, new_hu, dry);
+                w = velocity(new_h, new_hv, dry);
+            }
+            const float speed = mx(fabsf(u), fabsf(w)) + root<kFast>(g * new_h);
             const int32_t bits = __float_as_int(speed);
             speed_max = bits > speed_max ? bits : speed_max;
             if (x == 0 && y == 0 && a.budget_out) {
@@ -205,7 +257,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void hll_step_kernel(SweArg
         }
         fs = fn;
         cur = nrt;
-        if (prefetch) nrt = make_cell(ph, phu, phv, g, dry);
+        if (prefetch) nrt = make_cell<kFast>(ph, phu, phv, g, dry);
     }
 
     for (int off = kWave / 2; off > 0; off >>= 1) {
@@ -226,11 +278,19 @@ __global__ void fill_zero_kernel(uint32_t* __restrict__ p, int64_t words) {
 
 }  // namespace
 
-int swe_step(const Launch& l, const SweArgs& a, int rows) {
+int swe_step(const Launch& l, const SweArgs& a, int rows, bool fast) {
     const int bands = (a.ny + rows - 1) / rows;
-    dim3 grid((a.nx + kStrip - 1) / kStrip, (bands + kWavesPerBlock - 1) / kWavesPerBlock);
+    const int strips = (a.nx + kStrip - 1) / kStrip;
+    const int64_t tiles = (int64_t)strips * ((bands + kWavesPerBlock - 1) / kWavesPerBlock);
+    const int64_t per = (tiles + kXcds - 1) / kXcds;
+    const dim3 grid((unsigned)(per * kXcds));
     timed_begin(l, "hll_step");
-    hipLaunchKernelGGL(hll_step_kernel, grid, dim3(kWave * kWavesPerBlock), 0, l.stream, a, rows);
+    if (fast)
+        hipLaunchKernelGGL(hll_step_kernel<true>, grid, dim3(kWave * kWavesPerBlock), 0, l.stream, a, rows, strips,
+                           tiles, per);
+    else
+        hipLaunchKernelGGL(hll_step_kernel<false>, grid, dim3(kWave * kWavesPerBlock), 0, l.stream, a, rows, strips,
+                           tiles, per);
     timed_end(l, "hll_step");
     return check_launch("hll_step");
 }

@@ -673,8 +673,9 @@ int autotune(vxr_sim* s, int reps) {
         a.speed_clear = s->speed[(k + 2) % 3];
         a.budget_in = s->budget[k % 2];
         a.budget_out = s->budget[(k + 1) % 2];
-        int rc = tune_option(s, "rows", {8, 16, 32, 64, 128}, reps,
-                             [&]() { return vxr::swe_step(l, a, (int)opt(s, "rows")); });
+        int rc = tune_option(s, "rows", {2, 4, 8, 16, 32, 48, 64, 128}, reps, [&]() {
+            return vxr::swe_step(l, a, (int)opt(s, "rows"), opt(s, "hll_fast") != 0);
+        });
         HIP_TRY(hipMemcpy(s->clamped, &clamped, 4, hipMemcpyHostToDevice));
         return rc;
     }
@@ -757,7 +758,7 @@ struct OptionSpec {
 };
 const OptionSpec kOptions[] = {
     {"p2g", 0, 3, 1},   {"g2p", 0, 2, 2},        {"run", 1, 4, 4},   {"radix_bits", 4, 8, 8},
-    {"rows", 4, 256, 32}, {"blocks", 0, 1 << 20, 0}, {"graph", 0, 1, 0},
+    {"rows", 1, 256, 32}, {"blocks", 0, 1 << 20, 0}, {"graph", 0, 1, 0}, {"hll_fast", 0, 1, 0},
 };
 
 
@@ -1297,6 +1298,7 @@ int vxr_swe_dispatch(vxr_sim* s, int count) {
     a.params = s->params;
     a.clamped = s->clamped;
     const int rows = (int)opt(s, "rows");
+    const bool fast = opt(s, "hll_fast") != 0;
     for (int i = 0; i < count; ++i) {
         const int64_t k = s->dispatched;
         for (int f = 0; f < 3; ++f) {
@@ -1308,7 +1310,7 @@ int vxr_swe_dispatch(vxr_sim* s, int count) {
         a.speed_clear = s->speed[(k + 2) % 3];
         a.budget_in = s->budget[k % 2];
         a.budget_out = s->budget[(k + 1) % 2];
-        if (int rc = vxr::swe_step(s->launch(), a, rows)) return rc;
+        if (int rc = vxr::swe_step(s->launch(), a, rows, fast)) return rc;
         s->dispatched++;
     }
     return VXR_OK;

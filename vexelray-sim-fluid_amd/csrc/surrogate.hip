@@ -14,6 +14,7 @@
 // weight gradient is a GEMM over pixels, staged through LDS transposed.
 #include <climits>
 
+#include "../../include/vexel/vxs.h"
 #include "common.h"
 
 namespace vxr {
