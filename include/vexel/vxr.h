@@ -34,11 +34,13 @@ extern "C" {
 #define VXR_KIND_SWE 3  /* gui/PatchSimulation.java over core/stencil/ShallowWater.java */
 #define VXR_KIND_MPM2_BATCH 4 /* many ParticleSimulation scenes in one set of buffers (vxr_create_batch) */
 
-/* Creation flags. Of the reference's optional physics, 3D surface tension is built: MPM3 with
+/* Creation flags. Of the reference's optional physics, surface tension is built. MPM3 with
  * VXR_FLAG_TENSION steps as Flip3Step(..., tension) does (core/particle/Flip3Step.java:88-109:
  * clear, scatter, blur0..blur5, stress, grid with the capillary force, advect; sigma is params[6];
- * buffers c0 c1 txx tyy tzz txy txz tyz). FlipStep's tension, heat, convection, relax, pump and
- * Flip3Step's plastic are VXR_EUNSUPPORTED. */
+ * buffers c0 c1 txx tyy tzz txy txz tyz); MPM2 with it as FlipStep(..., tension) does
+ * (core/particle/FlipStep.java:111-143: the same passes; sigma is params[5]; buffers c0 c1 txx
+ * tyy txy). FlipStep's heat, convection, relax, pump and Flip3Step's plastic are
+ * VXR_EUNSUPPORTED. */
 #define VXR_FLAG_TENSION (1u << 0)
 #define VXR_FLAG_HEAT (1u << 1)
 #define VXR_FLAG_CONVECTION (1u << 2)
@@ -97,7 +99,7 @@ int vxr_destroy(vxr_sim* sim);
  * length; "params" also takes a shorter prefix. Names: MPM3 x y z u v w m j c00..c22 gm gmu
  * gmv gmw gu gv gw params keys level target stay view slot active (and with VXR_FLAG_TENSION
  * c0 c1 txx tyy tzz txy txz tyz); MPM2 x y u v m j c00 c01 c10 c11 gm gmu gmv gu gv params keys
- * starts; SWE h hu hv params speed budget clamped (the current slot of each rotating set,
+ * starts (and with VXR_FLAG_TENSION c0 c1 txx tyy txy); SWE h hu hv params speed budget clamped (the current slot of each rotating set,
  * as gui/PatchSimulation.java:147-213 resolves it). MPM3 "slot" (int32) is the slot each
  * particle position holds -- the identity until a sort permutes it with the particles -- and a
  * write of it must be a permutation of 0..count-1; "active" (int32, read-only) is the number of

@@ -82,6 +82,11 @@ def _load() -> ctypes.CDLL:
         "orc_tension3_force": (None, [_c_i, _c_i, _c_i, _f32p, _vp, _c_i64, _f32p]),
         "orc_mpm3_grid_tension": (None, [_c_i, _c_i, _c_i, _i32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
                                          _vp, _f32p]),
+        "orc_tension2_blur": (None, [_c_i, _c_i, _f32p, _f32p, _i32p, _c_i]),
+        "orc_tension2_stress": (None, [_c_i, _c_i, _f32p, _f32p, _f32p, _f32p]),
+        "orc_tension2_force": (None, [_c_i, _c_i, _f32p, _f32p, _f32p, _f32p, _c_i64, _f32p]),
+        "orc_mpm2_grid_tension": (None, [_c_i, _c_i, _i32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                         _f32p]),
         "orc_mpm3_lod": (None, [_c_i, _c_i, _c_i, _c_i64, _vp, _f32p, _f32p, _f32p]),
         "orc_mpm3_lod_target": (None, [_c_i64, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]),
         "orc_particle_diagnostics": (None, [_c_i64, _f32p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -422,13 +427,19 @@ def mpm3_inactive_key(nx, ny, nz) -> int:
     return lib.orc_mpm3_inactive_key(nx, ny, nz)
 
 
+STRESS2 = ("txx", "tyy", "txy")
+
+
 class Mpm2:
     """A 2D particle state on the CPU, stepped as FlipStep lists its passes. With ``fixed`` the
     scatter is the deterministic fixed-point one (orc_mpm2_scatter_fixed) and its sums are
-    decoded into gm, gmu, gmv before the grid update."""
+    decoded into gm, gmu, gmv before the grid update. With ``tension`` the step is
+    FlipStep(..., tension)'s (core/particle/FlipStep.java:111-143): clear, scatter, six blurs
+    (gm -> c0 -> c1 -> ...), stress, grid with tension, advect."""
 
-    def __init__(self, nx, ny, fields: dict, params, fixed=False):
+    def __init__(self, nx, ny, fields: dict, params, fixed=False, tension=False):
         self.fixed = fixed
+        self.tension = tension
         self.n = (nx, ny)
         count = len(fields["x"])
         self.p = {f: _c(fields.get(f, np.zeros(count)), np.float32).copy() for f in FIELDS2}
@@ -436,8 +447,28 @@ class Mpm2:
             self.p["j"][:] = 1
         self.count = count
         self.g = {f: np.zeros(nx * ny, np.float32) for f in GRID2}
+        if tension:
+            for f in ("c0", "c1") + STRESS2:
+                self.g[f] = np.zeros(nx * ny, np.float32)
         self.params = np.asarray(params, np.int32)
         self.steps = 0
+
+    def blur(self, k):
+        """Pass blur k: gm (scaled to a colour) into c0 for k = 0, then c0 and c1 in turn."""
+        src = "gm" if k == 0 else ("c0" if k % 2 == 1 else "c1")
+        dst = "c0" if k % 2 == 0 else "c1"
+        lib.orc_tension2_blur(*self.n, self.g[src], self.g[dst], self.params, 1 if k == 0 else 0)
+
+    def colour(self):
+        return "c0" if (TENSION_BLURS - 1) % 2 == 0 else "c1"
+
+    def stress(self):
+        lib.orc_tension2_stress(*self.n, self.g[self.colour()], *[self.g[f] for f in STRESS2])
+
+    def force(self, node):
+        out = np.zeros(2, np.float32)
+        lib.orc_tension2_force(*self.n, self.g[self.colour()], *[self.g[f] for f in STRESS2], int(node), out)
+        return out
 
     def _pp(self):
         return _ptrs([self.p[f] for f in FIELDS2])
@@ -457,16 +488,24 @@ class Mpm2:
                              self.g["gm"], self.g["gmu"], self.g["gmv"])
 
     def grid(self):
+        if self.tension:
+            lib.orc_mpm2_grid_tension(*self.n, self.params, *[self.g[f] for f in GRID2],
+                                      *[self.g[f] for f in STRESS2], self.g[self.colour()])
+            return
         lib.orc_mpm2_grid(*self.n, self.params, *[self.g[f] for f in GRID2])
 
     def advect(self):
         lib.orc_mpm2_advect(*self.n, self.count, self._pp(), self.params, self.g["gu"], self.g["gv"])
 
     def step(self, n=1):
-        if self.fixed:
+        if self.fixed or self.tension:
             for _ in range(n):
                 self.clear()
                 self.scatter()
+                if self.tension:
+                    for k in range(TENSION_BLURS):
+                        self.blur(k)
+                    self.stress()
                 self.grid()
                 self.advect()
             return

@@ -108,6 +108,16 @@ void orc_tension3_force(int nx, int ny, int nz, const float* colour, const float
 void orc_mpm3_grid_tension(int nx, int ny, int nz, const int32_t params[7], const float* gm, const float* gmu,
                            const float* gmv, const float* gmw, float* gu, float* gv, float* gw,
                            const float* const t[6], const float* colour);
+/* 2D surface tension: Tension.blur (core/particle/Tension.java:98-122; scaled: min(m / rho0, 1)
+ * per tap), Tension.stress (:135-150), Tension.force (:160-185) at one node, and
+ * Flip.gridWithTension (core/particle/Flip.java:342-412); params are Flip's 28 words. */
+void orc_tension2_blur(int nx, int ny, const float* src, float* dst, const int32_t* params, int scaled);
+void orc_tension2_stress(int nx, int ny, const float* colour, float* txx, float* tyy, float* txy);
+void orc_tension2_force(int nx, int ny, const float* colour, const float* txx, const float* tyy, const float* txy,
+                        int64_t node, float out[2]);
+void orc_mpm2_grid_tension(int nx, int ny, const int32_t* params, const float* gm, const float* gmu, const float* gmv,
+                           float* gu, float* gv, const float* txx, const float* tyy, const float* txy,
+                           const float* colour);
 
 /* The 3D level of detail: Flip3.lod (core/particle/Flip3.java:420-586) over `groups` groups of 8
  * consecutive slots, with level (per group), target (per group), stay (per slot); and

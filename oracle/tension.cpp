@@ -1,6 +1,9 @@
-// 3D surface tension restated: Tension3.blur, Tension3.stress and Tension3.force
-// (core/particle/Tension3.java:69-196) and Flip3.gridWithTension (core/particle/Flip3.java:263-311),
-// binary32 in the IR's order, neighbour reads held to the wall ring. Test infrastructure only.
+// Surface tension restated, binary32 in the IR's order, neighbour reads held to the wall ring.
+// Test infrastructure only.
+// 3D: Tension3.blur, Tension3.stress and Tension3.force (core/particle/Tension3.java:69-196) and
+// Flip3.gridWithTension (core/particle/Flip3.java:263-311).
+// 2D: Tension.blur, Tension.stress and Tension.force (core/particle/Tension.java:98-185) and
+// Flip.gridWithTension (core/particle/Flip.java:342-412).
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -161,6 +164,95 @@ void orc_mpm3_grid_tension(int nx, int ny, int nz, const int32_t params[7], cons
             if (c[a] > size[a] - 3) vel[a] = mn(vel[a], 0.0f);
             out[a][node] = vel[a];
         }
+    }
+}
+
+// ---- 2D (core/particle/Tension.java)
+
+// Tension.blur (:98-122): the [1 2 1] x [1 2 1] / 16 blur of src, rows of taps outer; with
+// `scaled` each tap is first taken to a colour, min(m / rho0, 1). params: Flip's 28 words.
+void orc_tension2_blur(int nx, int ny, const float* src, float* dst, const int32_t* params, int scaled) {
+    const int64_t nodes = (int64_t)nx * ny;
+    const float rho0 = bits_to_float(params[4]);  // Flip.RHO0
+    for (int64_t node = 0; node < nodes; node++) {
+        const int ni = (int)(node % nx), nj = (int)(node / nx);
+        float sum = 0.0f;
+        for (int dj = -1; dj <= 1; dj++)
+            for (int di = -1; di <= 1; di++) {
+                const float weight = (float)((2 - std::abs(di)) * (2 - std::abs(dj)) / 16.0);
+                float value = src[(int64_t)held(nj, dj, ny) * nx + held(ni, di, nx)];
+                if (scaled) value = mn(value / rho0, 1.0f);
+                sum = sum + weight * value;
+            }
+        dst[node] = sum;
+    }
+}
+
+// Tension.stress (:135-150): (c_y^2, c_x^2, -c_x c_y) / |grad c| from central differences.
+void orc_tension2_stress(int nx, int ny, const float* colour, float* txx, float* tyy, float* txy) {
+    const int64_t nodes = (int64_t)nx * ny;
+    for (int64_t node = 0; node < nodes; node++) {
+        const int ni = (int)(node % nx), nj = (int)(node / nx);
+        const float cx = 0.5f * (colour[(int64_t)held(nj, 0, ny) * nx + held(ni, 1, nx)] -
+                                 colour[(int64_t)held(nj, 0, ny) * nx + held(ni, -1, nx)]);
+        const float cy = 0.5f * (colour[(int64_t)held(nj, 1, ny) * nx + held(ni, 0, nx)] -
+                                 colour[(int64_t)held(nj, -1, ny) * nx + held(ni, 0, nx)]);
+        const float g = std::sqrt((cx * cx + cy * cy) + kEps2);
+        txx[node] = (cy * cy) / g;
+        tyy[node] = (cx * cx) / g;
+        txy[node] = -((cx * cy) / g);
+    }
+}
+
+// Tension.force (:160-185) at one node: the mean stress across each of its faces east, west,
+// north, south in that order, where the colour on both sides exceeds MIN_COLOUR.
+void orc_tension2_force(int nx, int ny, const float* colour, const float* txx, const float* tyy, const float* txy,
+                        int64_t node, float out[2]) {
+    const int ni = (int)(node % nx), nj = (int)(node / nx);
+    const int faces[4][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+    float fx = 0.0f, fy = 0.0f;
+    const float here = colour[node];
+    for (const auto& face : faces) {
+        const int64_t there = (int64_t)held(nj, face[1], ny) * nx + held(ni, face[0], nx);
+        if (!(here > kMinColour && colour[there] > kMinColour)) continue;
+        const float half = (float)(0.5 * (face[0] + face[1]));
+        const float* along_x = face[0] != 0 ? txx : txy;
+        const float* along_y = face[0] != 0 ? txy : tyy;
+        fx = fx + half * (along_x[node] + along_x[there]);
+        fy = fy + half * (along_y[node] + along_y[there]);
+    }
+    out[0] = fx;
+    out[1] = fy;
+}
+
+// Flip.gridWithTension (Flip.java:342-412 with tension, weight 1): Flip.grid plus
+// dt sigma F / max(m, MASS_FLOOR rho0) on every node with mass, before the wall ring.
+void orc_mpm2_grid_tension(int nx, int ny, const int32_t* params, const float* gm, const float* gmu, const float* gmv,
+                           float* gu, float* gv, const float* txx, const float* tyy, const float* txy,
+                           const float* colour) {
+    const float dt = bits_to_float(params[0]);
+    const float gx = bits_to_float(params[1]), gy = bits_to_float(params[2]);
+    const float rho0 = bits_to_float(params[4]), sigma = bits_to_float(params[5]);
+    const int64_t nodes = (int64_t)nx * ny;
+    for (int64_t node = 0; node < nodes; node++) {
+        const int64_t ni = node % nx, nj = node / nx;
+        const float m = gm[node];
+        float u = 0.0f, w = 0.0f;
+        if (m > kEmpty) {
+            u = gmu[node] / m + dt * (gx * 1.0f);
+            w = gmv[node] / m + dt * (gy * 1.0f);
+            float force[2];
+            orc_tension2_force(nx, ny, colour, txx, tyy, txy, node, force);
+            const float push = (dt * sigma) / mx(m, kMassFloor * rho0);
+            u = u + push * force[0];
+            w = w + push * force[1];
+        }
+        if (!(ni > 1)) u = mx(u, 0.0f);
+        if (ni > nx - 3) u = mn(u, 0.0f);
+        if (!(nj > 1)) w = mx(w, 0.0f);
+        if (nj > ny - 3) w = mn(w, 0.0f);
+        gu[node] = u;
+        gv[node] = w;
     }
 }
 

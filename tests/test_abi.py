@@ -49,8 +49,10 @@ def test_arguments_refused_before_the_device():
     dims = (ctypes.c_int32 * 3)(8, 8, 8)
     assert lib.vxr_create(99, dims, 10, 0, ctypes.byref(h)) == _native.EINVAL
     assert b"unknown kind" in lib.vxr_last_error()
-    # 3D tension is built; 2D tension and 3D plasticity are not
-    assert lib.vxr_create(_native.KIND_MPM2, dims, 10, _native.FLAG_TENSION, ctypes.byref(h)) == _native.EUNSUPPORTED
+    # surface tension is built in 2D and 3D; heat and 3D plasticity are not
+    assert lib.vxr_create(_native.KIND_MPM2, dims, 10, _native.FLAG_HEAT, ctypes.byref(h)) == _native.EUNSUPPORTED
+    assert lib.vxr_create(_native.KIND_MPM2, dims, 10, _native.FLAG_TENSION | _native.FLAG_HEAT,
+                          ctypes.byref(h)) == _native.EUNSUPPORTED
     assert lib.vxr_create(_native.KIND_MPM3, dims, 10, _native.FLAG_PLASTIC, ctypes.byref(h)) == _native.EUNSUPPORTED
     assert lib.vxr_create(_native.KIND_MPM3, dims, 10, _native.FLAG_PLASTIC | _native.FLAG_TENSION,
                           ctypes.byref(h)) == _native.EUNSUPPORTED

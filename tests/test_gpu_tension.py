@@ -168,7 +168,10 @@ def test_tension_without_the_flag_is_refused(gpu):
     with pytest.raises(Exception):
         sim.sim.run_pass("blur0")
     with pytest.raises(Exception):
-        particles.ParticleSimulation(16, 16, 64, tension=True)
+        particles.ParticleSimulation(16, 16, 64, heat=True)
+    plain = particles.ParticleSimulation(16, 16, 64)
+    with pytest.raises(Exception):
+        plain.sim.run_pass("stress")
 
 
 # ------------------------------------------------------------------ Tension3Test

@@ -201,6 +201,12 @@ int mpm2_grid(const Launch& l, const Mpm2Args& a);
 int mpm2_advect(const Launch& l, const Mpm2Args& a);
 int mpm2_keys(const Launch& l, const Mpm2Args& a, int32_t* keys, int32_t* index);
 int mpm2_starts(const Launch& l, const int32_t* sorted_keys, int64_t count, int32_t cells, int32_t* starts);
+// 2D surface tension (core/particle/Tension.java), dense over the grid: blur (scaled: gm to a
+// colour), stress (t = txx tyy txy), and the grid update with the capillary force (reads
+// gm gmu gmv, writes gu gv).
+int tension2_blur(const Launch& l, const Mpm2Args& a, const float* src, float* dst, bool scaled);
+int tension2_stress(const Launch& l, const Mpm2Args& a, const float* colour, float* const t[3]);
+int mpm2_grid_tension(const Launch& l, const Mpm2Args& a, float* const t[3], const float* colour);
 
 // Many 2D scenes in one set of buffers, stepped by one launch per pass (a single scene with the
 // fixed-point P2G is a batch of one). Scene s: particles [poff, poff + count) padded to
@@ -238,6 +244,10 @@ struct Batch2Args {
 // `blocks` > 0 caps the grid (each block then strides over chunks); 0 is a block per chunk.
 int batch2_clear(const Launch& l, const Batch2Args& a, int blocks);
 int batch2_scatter(const Launch& l, const Batch2Args& a, int run, int blocks);
+// The fixed-point sums decoded into gm gmu gmv (no velocities), and, after a grid update that did
+// not go through batch2_grid, the chunk flags moved on (touched -> live, touched cleared).
+int batch2_decode(const Launch& l, const Batch2Args& a, int blocks);
+int batch2_flags(const Launch& l, const Batch2Args& a);
 int batch2_grid(const Launch& l, const Batch2Args& a, int blocks);
 int batch2_advect(const Launch& l, const Batch2Args& a, int blocks);
 int batch2_keys(const Launch& l, const Batch2Args& a, int32_t* keys, int32_t* index, int blocks);

@@ -516,6 +516,21 @@ __global__ __launch_bounds__(kBlock) void batch2_grid_kernel(Batch2Args a) {
     }
 }
 
+// The sums decoded into gm gmu gmv over the chunks flagged touched or live, as batch2_grid_kernel
+// decodes them, without the velocities: for passes that read the mass before the grid update.
+__global__ __launch_bounds__(kBlock) void batch2_decode_kernel(Batch2Args a) {
+    for (int64_t c = blockIdx.x; c < a.gchunks; c += gridDim.x) {
+        if (!(a.touched[c] | a.live[c])) continue;
+        const Scene2 sc = a.scenes[a.gscene[c]];
+        const int64_t node = c * kBlock + threadIdx.x - sc.goff;
+        if (node >= (int64_t)sc.nx * sc.ny) continue;
+        const int64_t at = sc.goff + node;
+        a.g[0][at] = dequantize(a.q[0][at], kMassBits);
+        a.g[1][at] = dequantize(a.q[1][at], kMomentumBits);
+        a.g[2][at] = dequantize(a.q[2][at], kMomentumBits);
+    }
+}
+
 // A block a chunk of kBatchAlign particles (one scene's), a thread kBatchAlign / kBlock of them:
 // every read of the thread's particles in flight, then their sums and stores.
 __global__ __launch_bounds__(kBlock) void batch2_g2p_kernel(Batch2Args a) {
@@ -788,6 +803,22 @@ int batch2_grid(const Launch& l, const Batch2Args& a, int blocks) {
                        a.touched, a.live, a.gchunks);
     timed_end(l, "batch2_grid");
     return check_launch("batch2_grid");
+}
+
+int batch2_decode(const Launch& l, const Batch2Args& a, int blocks) {
+    const unsigned g = grid_of(a.gchunks, blocks);
+    if (!g) return 0;
+    timed_begin(l, "batch2_decode");
+    hipLaunchKernelGGL(batch2_decode_kernel, dim3(g), dim3(kBlock), 0, l.stream, a);
+    timed_end(l, "batch2_decode");
+    return check_launch("batch2_decode");
+}
+
+int batch2_flags(const Launch& l, const Batch2Args& a) {
+    if (a.gchunks <= 0) return 0;
+    hipLaunchKernelGGL(batch2_flags_kernel, dim3((unsigned)((a.gchunks + 255) / 256)), dim3(256), 0, l.stream,
+                       a.touched, a.live, a.gchunks);
+    return check_launch("batch2_flags");
 }
 
 int batch2_advect(const Launch& l, const Batch2Args& a, int blocks) {

@@ -276,6 +276,15 @@ bool lookup(vxr_sim* s, const std::string& name, Buf* b) {
             *b = Buf{s->stay, s->count, s->capacity};
             return true;
         }
+        if (!three && s->tension) {
+            const char* tn[5] = {"c0", "c1", "txx", "tyy", "txy"};
+            for (int k = 0; k < 5; ++k) {
+                if (name == tn[k]) {
+                    *b = Buf{k < 2 ? s->colour[k] : s->stress[k - 2], s->nodes, s->nodes};
+                    return true;
+                }
+            }
+        }
         if (three && s->tension) {
             const char* tn[8] = {"c0", "c1", "txx", "tyy", "tzz", "txy", "txz", "tyz"};
             for (int k = 0; k < 8; ++k) {
@@ -482,7 +491,27 @@ int run_pass(vxr_sim* s, const std::string& pass) {
             const int64_t g2p = opt(s, "g2p");
             return g2p == 0 ? vxr::mpm3_advect(l, a) : vxr::mpm3_advect_tiled(l, a, (int)opt(s, "run"), g2p == 2);
         }
-    } else if (fixed_p2g(s)) {
+    }
+    if (s->kind == VXR_KIND_MPM2 && s->tension) {
+        // FlipStep(..., tension)'s passes (core/particle/FlipStep.java:125-139): blur k reads gm
+        // (k = 0) or the colour blur k - 1 wrote and writes c[k % 2]; the stress reads the last
+        // colour; the grid update adds the capillary force. With the fixed-point P2G the sums are
+        // decoded into gm gmu gmv before the first blur.
+        const vxr::Mpm2Args a = mpm2_args(s);
+        if (pass.size() == 5 && pass.compare(0, 4, "blur") == 0 && pass[4] >= '0' && pass[4] <= '5') {
+            const int k = pass[4] - '0';
+            if (k == 0 && fixed_p2g(s))
+                if (int rc = vxr::batch2_decode(l, batch2_args(s), (int)opt(s, "blocks"))) return rc;
+            const float* src = k == 0 ? a.g[0] : s->colour[(k - 1) % 2];
+            return vxr::tension2_blur(l, a, src, s->colour[k % 2], k == 0);
+        }
+        if (pass == "stress") return vxr::tension2_stress(l, a, s->colour[1], s->stress);
+        if (pass == "grid") {
+            if (int rc = vxr::mpm2_grid_tension(l, a, s->stress, s->colour[1])) return rc;
+            return fixed_p2g(s) ? vxr::batch2_flags(l, batch2_args(s)) : 0;
+        }
+    }
+    if (s->kind != VXR_KIND_MPM3 && fixed_p2g(s)) {
         const vxr::Batch2Args b = batch2_args(s);
         const int blocks = (int)opt(s, "blocks");
         if (pass == "clear") return vxr::batch2_clear(l, b, blocks);
@@ -539,7 +568,7 @@ int replay_segment(vxr_sim* s, bool sort, int n) {
     // The tension passes' region bookkeeping is host state, read while a segment is recorded: a
     // segment recorded while it is stale would replay whole-grid passes for good. Run eagerly
     // until a step has rebuilt the region.
-    if (s->tension && s->stale) return run_segment(s, sort, n);
+    if (s->tension && s->kind == VXR_KIND_MPM3 && s->stale) return run_segment(s, sort, n);
     const auto key = std::make_tuple((int)sort, n, (int)s->flipped | (s->layers_valid ? 2 : 0));
     auto it = s->graphs.find(key);
     if (it == s->graphs.end()) {
@@ -715,7 +744,7 @@ int clear_all(vxr_sim* s) {
             HIP_TRY(hipMemsetAsync(s->layers.meta, 0, 8 * 4, s->stream));
             s->layers_valid = false;
         }
-        if (s->tension) {
+        if (s->tension && s->kind == VXR_KIND_MPM3) {
             // what the full passes give a grid without mass: +0 colour and diagonal stress, -0
             // off-diagonal stress
             for (float* f : s->colour) HIP_TRY(hipMemsetAsync(f, 0, round_up(s->nodes, 4) * 4, s->stream));
@@ -724,6 +753,10 @@ int clear_all(vxr_sim* s) {
                                           round_up(s->nodes, 4), s->stream));
             s->spread[0] = s->spread[1] = 0;
             s->stale = true;
+        } else if (s->tension) {
+            // 2D: the passes are dense and rewrite every node; zero, as the reference's clear leaves them
+            for (float* f : s->colour) HIP_TRY(hipMemsetAsync(f, 0, round_up(s->nodes, 4) * 4, s->stream));
+            for (int k = 0; k < 3; ++k) HIP_TRY(hipMemsetAsync(s->stress[k], 0, round_up(s->nodes, 4) * 4, s->stream));
         }
         if (s->stay) {
             const int64_t groups = std::max<int64_t>(s->count / 8, 1);
@@ -858,10 +891,10 @@ int vxr_create(int kind, const int32_t dims[3], int64_t particles, uint32_t flag
     *out = nullptr;
     const uint32_t physics = VXR_FLAG_TENSION | VXR_FLAG_HEAT | VXR_FLAG_CONVECTION | VXR_FLAG_RELAX | VXR_FLAG_PUMP |
                              VXR_FLAG_PLASTIC;
-    const uint32_t built = kind == VXR_KIND_MPM3 ? VXR_FLAG_TENSION : 0u;
+    const uint32_t built = kind == VXR_KIND_MPM3 || kind == VXR_KIND_MPM2 ? VXR_FLAG_TENSION : 0u;
     if (flags & physics & ~built)
-        return fail(VXR_EUNSUPPORTED, "of the optional physics only 3D tension is built (not heat, convection, relax, "
-                                      "pump, plastic, or 2D tension)");
+        return fail(VXR_EUNSUPPORTED, "of the optional physics only surface tension is built (not heat, convection, "
+                                      "relax, pump or plastic)");
     const uint32_t edges = VXR_FLAG_OPEN_WEST | VXR_FLAG_OPEN_EAST | VXR_FLAG_OPEN_SOUTH | VXR_FLAG_OPEN_NORTH;
     if (flags & ~(physics | edges)) return fail(VXR_EINVAL, "unknown flags 0x%x", flags & ~(physics | edges));
 
@@ -953,6 +986,13 @@ int vxr_create(int kind, const int32_t dims[3], int64_t particles, uint32_t flag
         const int64_t sort_words = std::max<int64_t>(vxr::radix_scratch_words(s->capacity, 8),
                                                      three ? vxr::mpm3_layer_scratch_words(s->capacity) : 0);
         if (!rc) rc = alloc(s, sort_words, &ptr), s->scratch = (int32_t*)ptr;
+        if (!rc && !three && (flags & VXR_FLAG_TENSION)) {
+            // FlipStep(..., tension)'s c0 c1 txx tyy txy (core/particle/FlipStep.java:111-115)
+            s->tension = true;
+            s->stale = false;  // the 2D passes are dense: no region to keep
+            for (int k = 0; k < 2 && !rc; ++k) rc = alloc(s, round_up(s->nodes, 4), &ptr), s->colour[k] = (float*)ptr;
+            for (int k = 0; k < 3 && !rc; ++k) rc = alloc(s, round_up(s->nodes, 4), &ptr), s->stress[k] = (float*)ptr;
+        }
         if (!rc && three && (flags & VXR_FLAG_TENSION)) {
             s->tension = true;
             for (int k = 0; k < 2 && !rc; ++k) rc = alloc(s, round_up(s->nodes, 4), &ptr), s->colour[k] = (float*)ptr;
@@ -1138,7 +1178,7 @@ int vxr_write(vxr_sim* s, const char* name, const void* words, int64_t n) {
         s->layers_valid = false;  // P2G goes by the sorted-segmented schedule until the next sort
     bool tension_buffer = false;
     for (float* f : s->stress) tension_buffer = tension_buffer || (f && b.ptr == (void*)f);
-    if (s->tension && (b.ptr == s->colour[0] || b.ptr == s->colour[1] || tension_buffer)) {
+    if (three && s->tension && (b.ptr == s->colour[0] || b.ptr == s->colour[1] || tension_buffer)) {
         // tension buffers: unknown content, so the next passes run dense
         if (b.ptr == s->colour[0]) s->spread[0] = -1;
         if (b.ptr == s->colour[1]) s->spread[1] = -1;

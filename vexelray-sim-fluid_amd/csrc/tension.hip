@@ -1,7 +1,10 @@
-// 3D surface tension for gfx950: Tension3.blur, Tension3.stress, Tension3.force
-// (core/particle/Tension3.java:69-196) and Flip3.gridWithTension (core/particle/Flip3.java:263-311),
-// one node a thread, neighbour reads held to the wall ring, in the reference's binary32 operations
-// and order (the CPU restatement's results bit for bit).
+// Surface tension for gfx950, in the reference's binary32 operations and order (the CPU
+// restatement's results bit for bit), neighbour reads held to the wall ring:
+// - 3D: Tension3.blur, Tension3.stress, Tension3.force (core/particle/Tension3.java:69-196) and
+//   Flip3.gridWithTension (core/particle/Flip3.java:263-311), below;
+// - 2D: Tension.blur, Tension.stress, Tension.force (core/particle/Tension.java:98-185) and
+//   Flip.gridWithTension (core/particle/Flip.java:342-412), at the end of the file: dense passes
+//   over the grid, a node a thread (2D grids are small, and a pass is one launch).
 //
 // Where they run. The reference sweeps the whole grid each pass. Here the blur and stress passes
 // visit, in each row of nodes, the span of bricks in the region: the 3x3x3-brick neighbourhood
@@ -296,6 +299,106 @@ __global__ __launch_bounds__(kBlock) void grid_tension_rows_kernel(Mpm3Args a, S
     }
 }
 
+// ------------------------------------------------------------------ 2D
+
+namespace t2 {
+enum { DT = 0, GX = 1, GY = 2, BULK = 3, RHO0 = 4, SIGMA = 5 };  // Flip.java:102-107
+}
+
+// Tension.blur (:98-122): the [1 2 1] x [1 2 1] / 16 blur, taps row by row (dj outer, di inner),
+// sum += weight * value; kScaled takes each tap to a colour, min(m / rho0, 1).
+template <bool kScaled>
+__global__ __launch_bounds__(kBlock) void blur2_kernel(int nx, int ny, const float* __restrict__ src,
+                                                       float* __restrict__ dst, const int32_t* __restrict__ params) {
+    const int64_t node = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (node >= (int64_t)nx * ny) return;
+    const int ni = (int)(node % nx), nj = (int)(node / nx);
+    const float rho0 = kScaled ? bits_f(params[t2::RHO0]) : 1.0f;
+    float sum = 0.0f;
+#pragma unroll
+    for (int dj = -1; dj <= 1; dj++) {
+        const int64_t row = (int64_t)held(nj, dj, ny) * nx;
+#pragma unroll
+        for (int di = -1; di <= 1; di++) {
+            const float weight = (float)((2 - abs(di)) * (2 - abs(dj))) / 16.0f;
+            float value = src[row + held(ni, di, nx)];
+            if (kScaled) value = mn(value / rho0, 1.0f);
+            sum = sum + weight * value;
+        }
+    }
+    dst[node] = sum;
+}
+
+// Tension.stress (:135-150): (c_y^2, c_x^2, -c_x c_y) / |grad c|, central differences.
+__global__ __launch_bounds__(kBlock) void stress2_kernel(int nx, int ny, const float* __restrict__ colour,
+                                                         float* __restrict__ txx, float* __restrict__ tyy,
+                                                         float* __restrict__ txy) {
+    const int64_t node = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (node >= (int64_t)nx * ny) return;
+    const int ni = (int)(node % nx), nj = (int)(node / nx);
+    const int64_t row = (int64_t)held(nj, 0, ny) * nx;
+    const float cx = 0.5f * (colour[row + held(ni, 1, nx)] - colour[row + held(ni, -1, nx)]);
+    const int col = held(ni, 0, nx);
+    const float cy = 0.5f * (colour[(int64_t)held(nj, 1, ny) * nx + col] - colour[(int64_t)held(nj, -1, ny) * nx + col]);
+    const float g = sqrtf((cx * cx + cy * cy) + (float)(1e-4 * 1e-4));
+    txx[node] = (cy * cy) / g;
+    tyy[node] = (cx * cx) / g;
+    txy[node] = -((cx * cy) / g);
+}
+
+struct Stress2 {
+    const float* txx;
+    const float* tyy;
+    const float* txy;
+};
+
+// Flip.gridWithTension (Flip.java:342-412, weight 1): Flip.grid, plus on a node with mass
+// dt sigma F / max(m, MASS_FLOOR rho0), F being Tension.force (:160-185): the mean stress across
+// the faces east, west, north and south in that order, each where the colour on both of its
+// sides exceeds MIN_COLOUR; then the wall ring.
+__global__ __launch_bounds__(kBlock) void grid2_tension_kernel(int nx, int ny, const float* __restrict__ gm,
+                                                               const float* __restrict__ gmu,
+                                                               const float* __restrict__ gmv, float* __restrict__ gu,
+                                                               float* __restrict__ gv, const int32_t* __restrict__ params,
+                                                               Stress2 t, const float* __restrict__ colour) {
+    const int64_t node = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (node >= (int64_t)nx * ny) return;
+    const int ni = (int)(node % nx), nj = (int)(node / nx);
+    const float m = gm[node];
+    float u = 0.0f, w = 0.0f;
+    if (m > kEmpty) {
+        const float dt = bits_f(params[t2::DT]);
+        u = gmu[node] / m + dt * (bits_f(params[t2::GX]) * 1.0f);
+        w = gmv[node] / m + dt * (bits_f(params[t2::GY]) * 1.0f);
+        const float here = colour[node];
+        float fx = 0.0f, fy = 0.0f;
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            const int di = f == 0 ? 1 : (f == 1 ? -1 : 0);
+            const int dj = f == 2 ? 1 : (f == 3 ? -1 : 0);
+            const int64_t there = (int64_t)held(nj, dj, ny) * nx + held(ni, di, nx);
+            if (here > kMinColour && colour[there] > kMinColour) {
+                const float half = (float)(di + dj) * 0.5f;
+                const float* along_x = di != 0 ? t.txx : t.txy;
+                const float* along_y = di != 0 ? t.txy : t.tyy;
+                fx = fx + half * (along_x[node] + along_x[there]);
+                fy = fy + half * (along_y[node] + along_y[there]);
+            }
+        }
+        const float push = (dt * bits_f(params[t2::SIGMA])) / mx(m, kMassFloor * bits_f(params[t2::RHO0]));
+        u = u + push * fx;
+        w = w + push * fy;
+    }
+    if (!(ni > 1)) u = mx(u, 0.0f);
+    if (ni > nx - 3) u = mn(u, 0.0f);
+    if (!(nj > 1)) w = mx(w, 0.0f);
+    if (nj > ny - 3) w = mn(w, 0.0f);
+    gu[node] = u;
+    gv[node] = w;
+}
+
+inline unsigned node_blocks(int nx, int ny) { return (unsigned)(((int64_t)nx * ny + kBlock - 1) / kBlock); }
+
 inline unsigned row_blocks(const Mpm3Args& a) {
     return (unsigned)(((int64_t)a.ny * a.nz + kBlock / kWave - 1) / (kBlock / kWave));
 }
@@ -335,6 +438,34 @@ int mpm3_grid_tension(const Launch& l, const Mpm3Args& a, float* const t[6], con
     hipLaunchKernelGGL(grid_tension_rows_kernel, dim3(row_blocks(a)), dim3(kBlock), 0, l.stream, a, st, colour);
     timed_end(l, "grid3_tension");
     return check_launch("grid3_tension");
+}
+
+int tension2_blur(const Launch& l, const Mpm2Args& a, const float* src, float* dst, bool scaled) {
+    timed_begin(l, "tension2_blur");
+    if (scaled)
+        hipLaunchKernelGGL(blur2_kernel<true>, dim3(node_blocks(a.nx, a.ny)), dim3(kBlock), 0, l.stream, a.nx, a.ny,
+                           src, dst, a.params);
+    else
+        hipLaunchKernelGGL(blur2_kernel<false>, dim3(node_blocks(a.nx, a.ny)), dim3(kBlock), 0, l.stream, a.nx, a.ny,
+                           src, dst, a.params);
+    timed_end(l, "tension2_blur");
+    return check_launch("tension2_blur");
+}
+
+int tension2_stress(const Launch& l, const Mpm2Args& a, const float* colour, float* const t[3]) {
+    timed_begin(l, "tension2_stress");
+    hipLaunchKernelGGL(stress2_kernel, dim3(node_blocks(a.nx, a.ny)), dim3(kBlock), 0, l.stream, a.nx, a.ny, colour,
+                       t[0], t[1], t[2]);
+    timed_end(l, "tension2_stress");
+    return check_launch("tension2_stress");
+}
+
+int mpm2_grid_tension(const Launch& l, const Mpm2Args& a, float* const t[3], const float* colour) {
+    timed_begin(l, "grid2_tension");
+    hipLaunchKernelGGL(grid2_tension_kernel, dim3(node_blocks(a.nx, a.ny)), dim3(kBlock), 0, l.stream, a.nx, a.ny,
+                       a.g[0], a.g[1], a.g[2], a.g[3], a.g[4], a.params, Stress2{t[0], t[1], t[2]}, colour);
+    timed_end(l, "grid2_tension");
+    return check_launch("grid2_tension");
 }
 
 }  // namespace vxr
