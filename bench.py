@@ -26,6 +26,7 @@ SORT_EVERY = 10
 # through HBM at least, as derived in the design notes.
 BYTES = {
     "p2g3_seg": ("particle", 68), "p2g3_atomic": ("particle", 68), "p2g3_layer": ("particle", 68),
+    "p2g3_fixed": ("particle", 68),
     "g2p3_fast": ("particle", 84), "g2p3_tiled": ("particle", 84), "g2p3": ("particle", 84),
     "grid3": ("node", 28), "mpm3_clear": ("node", 16),
     "keys3": ("particle", 20), "radix_hist": ("element", 4), "radix_scatter": ("element", 16),
@@ -44,7 +45,7 @@ BYTES = {
 
 # Passes that visit only the flagged bricks (3D) or the chunks P2G reached (batched 2D): their
 # BYTES are per node of the whole grid, an upper bound on what they move, so no rate is derived.
-SPARSE = {"mpm3_clear", "grid3", "grid3_tension", "tension3_stress", "tension3_blur", "batch2_grid", "batch2_clear"}
+SPARSE = {"mpm3_clear", "fixed3_decode", "grid3", "grid3_tension", "tension3_stress", "tension3_blur", "batch2_grid", "batch2_clear"}
 
 
 def kernel_table(sim, names, units, bytes_override=None):
@@ -104,7 +105,8 @@ def bench_mpm3(steps, warmup, p2g, g2p, run, tension=False, lod_level=None):
     sim.advance(steps, sort_every=SORT_EVERY)
     sim.finish()
     t1 = time.perf_counter()
-    names = ["mpm3_clear", "p2g3_seg", "p2g3_atomic", "p2g3_layer", "grid3", "g2p3_fast", "g2p3_tiled", "g2p3",
+    names = ["mpm3_clear", "p2g3_seg", "p2g3_atomic", "p2g3_layer", "p2g3_fixed", "fixed3_decode", "grid3",
+             "g2p3_fast", "g2p3_tiled", "g2p3",
              "keys3", "radix_hist", "radix_scan", "radix_scatter", "layers3", "permute", "tension3_blur",
              "tension3_stress", "grid3_tension"]
     units = {"particle": count, "node": 256 ** 3, "element": count}
@@ -514,6 +516,7 @@ def main():
     ap.add_argument("--no-tension", action="store_true")
     ap.add_argument("--atomic-steps", type=int, default=20)
     ap.add_argument("--no-atomic", action="store_true")
+    ap.add_argument("--no-fixed", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (counter calibration) and exit")
     args = ap.parse_args()
@@ -567,6 +570,16 @@ def main():
             "roofline": roofline_of("p2g3_atomic", ak),
             "atomics_per_launch": atomics, "atomics_per_s": round(atomics / (ak["avg_us"] * 1e-6), 1),
             "atomics_counted": counted, "kernels": atable, "grid_mass_drift": adrift}
+    if not args.no_fixed:
+        print(f"[bench] 3D MPM with the fixed-point P2G: same scene, {args.tension_steps} steps", file=sys.stderr,
+              flush=True)
+        fcount, fsecs, ftable, fdrift = bench_mpm3(args.tension_steps, 5, 2, 1, args.run)
+        line["fixed3"] = {"value": round(fcount * args.tension_steps / fsecs, 1), "unit": "particle-substeps/s",
+                          "steps": args.tension_steps, "ms_per_step": round(fsecs / args.tension_steps * 1e3, 4),
+                          "workload": "the same 3D substep, deterministic: P2G in 64-bit fixed point (option p2g 2) "
+                                      "and the reference-order G2P (g2p 1), so every substep is the oracle's bit "
+                                      "for bit in any particle order",
+                          "kernels": ftable, "grid_mass_drift": fdrift}
     if not args.no_tension:
         print(f"[bench] 3D MPM with tension: same scene, {args.tension_steps} steps", file=sys.stderr, flush=True)
         tcount, tsecs, ttable, tdrift = bench_mpm3(args.tension_steps, 5, args.p2g, args.g2p, args.run, tension=True)

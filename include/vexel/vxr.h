@@ -187,9 +187,11 @@ int vxr_stream(vxr_sim* sim, void** stream);
  *   "p2g"        MPM: 0 = direct float atomics (the reference's Flip3.scatter schedule),
  *                1 = sorted-segmented (default): per-thread runs of consecutive particles folded in
  *                registers (3D: then a wave-private LDS node tile), one atomic per node and field;
- *                2 = 2D only, and always for a batch: fixed point, each amount rounded to a 64-bit
+ *                2 = fixed point (2D, 3D, and always for a batch): each amount rounded to a 64-bit
  *                integer (mass 2^-40, momentum 2^-32) and summed by integer atomics -- the same
- *                bits in any order, equal to the CPU restatement's;
+ *                bits in any order, equal to the CPU restatement's, so that with g2p 0 or 1 whole
+ *                substeps are deterministic (3D: the sums take 32 bytes a node, made when the
+ *                option is first set);
  *                3 = MPM3 only: cell-ranked groups -- the sort also lays each 256-cell group out by
  *                cells ranked by particle count, a wave takes a group with a lane a cell, folds
  *                each cell's particles in registers and adds them into an LDS node tile in 27

@@ -58,6 +58,8 @@ def _load() -> ctypes.CDLL:
                                     ctypes.c_double]),
         "orc_mpm3_clear": (None, [_c_i, _c_i, _c_i, _f32p, _f32p, _f32p, _f32p]),
         "orc_mpm3_scatter": (None, [_c_i, _c_i, _c_i, _c_i64, _vp, _i32p, _f32p, _f32p, _f32p, _f32p]),
+        "orc_mpm3_scatter_fixed": (None, [_c_i, _c_i, _c_i, _c_i64, _vp, _i32p, _i64p, _i64p, _i64p, _i64p]),
+        "orc_mpm3_decode_fixed": (None, [_c_i64, _i64p, _i64p, _i64p, _i64p, _f32p, _f32p, _f32p, _f32p]),
         "orc_mpm3_grid": (None, [_c_i, _c_i, _c_i, _i32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]),
         "orc_mpm3_advect": (None, [_c_i, _c_i, _c_i, _c_i64, _vp, _i32p, _f32p, _f32p, _f32p]),
         "orc_mpm3_step": (None, [_c_i, _c_i, _c_i, _c_i64, _vp, _i32p, _vp]),
@@ -287,10 +289,13 @@ STRESS3 = ("txx", "tyy", "tzz", "txy", "txz", "tyz")
 class Mpm3:
     """A 3D particle state on the CPU, stepped pass by pass as Flip3Step lists them. With
     ``tension`` the step is Flip3Step's tension one (core/particle/Flip3Step.java:88-109): clear,
-    scatter, six blurs (gm -> c0 -> c1 -> ...), stress, grid with tension, advect."""
+    scatter, six blurs (gm -> c0 -> c1 -> ...), stress, grid with tension, advect. With ``fixed``
+    the scatter is the fixed-point one (orc_mpm3_scatter_fixed), its sums (kept in self.q) decoded
+    into gm gmu gmv gmw."""
 
-    def __init__(self, nx, ny, nz, fields: dict, params, tension=False):
+    def __init__(self, nx, ny, nz, fields: dict, params, tension=False, fixed=False):
         self.tension = tension
+        self.fixed = fixed
         self.n = (nx, ny, nz)
         count = len(fields["x"])
         self.p = {f: _c(fields.get(f, np.zeros(count)), np.float32).copy() for f in FIELDS3}
@@ -328,6 +333,12 @@ class Mpm3:
         lib.orc_mpm3_clear(*self.n, self.g["gm"], self.g["gmu"], self.g["gmv"], self.g["gmw"])
 
     def scatter(self):
+        if self.fixed:
+            nodes = self.n[0] * self.n[1] * self.n[2]
+            self.q = [np.zeros(nodes, np.int64) for _ in range(4)]
+            lib.orc_mpm3_scatter_fixed(*self.n, self.count, self._pp(), self.params, *self.q)
+            lib.orc_mpm3_decode_fixed(nodes, *self.q, self.g["gm"], self.g["gmu"], self.g["gmv"], self.g["gmw"])
+            return
         lib.orc_mpm3_scatter(*self.n, self.count, self._pp(), self.params,
                              self.g["gm"], self.g["gmu"], self.g["gmv"], self.g["gmw"])
 
@@ -343,13 +354,14 @@ class Mpm3:
                             self.g["gu"], self.g["gv"], self.g["gw"])
 
     def step(self, n=1):
-        if self.tension:
+        if self.tension or self.fixed:
             for _ in range(n):
                 self.clear()
                 self.scatter()
-                for k in range(TENSION_BLURS):
-                    self.blur(k)
-                self.stress()
+                if self.tension:
+                    for k in range(TENSION_BLURS):
+                        self.blur(k)
+                    self.stress()
                 self.grid()
                 self.advect()
             return

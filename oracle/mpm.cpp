@@ -173,6 +173,40 @@ void orc_mpm3_scatter(int nx, int ny, int nz, int64_t count, float* const p[17],
                      [&](int64_t node, int f, float v) { grid[f][node] += v; });
 }
 
+// Fixed-point amounts: a * 2^bits is exact in binary32 (a power-of-two scaling), and llrintf
+// rounds it to the nearest integer, ties to even, under the default rounding mode.
+static inline int64_t quantize(float a, int bits) { return (int64_t)llrintf(std::ldexp(a, bits)); }
+
+// The 3D fixed-point scatter: each amount scatter3_one computes (the reference's arithmetic,
+// Flip3.java:216-243) rounded to a 64-bit integer, mass in 2^-40 and momentum in 2^-32, and summed
+// with two's-complement wrap-around as the device's 64-bit atomics sum it.
+void orc_mpm3_scatter_fixed(int nx, int ny, int nz, int64_t count, float* const p[17], const int32_t params[7],
+                            int64_t* qm, int64_t* qmu, int64_t* qmv, int64_t* qmw) {
+    int64_t* q[4] = {qm, qmu, qmv, qmw};
+    const int bits[4] = {ORC_FIXED_MASS_BITS, ORC_FIXED_MOMENTUM_BITS, ORC_FIXED_MOMENTUM_BITS,
+                         ORC_FIXED_MOMENTUM_BITS};
+    const int64_t nodes = (int64_t)nx * ny * nz;
+    for (int f = 0; f < 4; f++) std::fill(q[f], q[f] + nodes, 0);
+    const float dt = bits_to_float(params[DT3]);
+    const float bulk = bits_to_float(params[BULK3]);
+    const float rho0 = bits_to_float(params[RHO03]);
+    const auto base_of = [&](const int32_t (&cell)[3]) { return ((int64_t)cell[2] * ny + cell[1]) * nx + cell[0]; };
+    for (int64_t k = 0; k < count; k++)
+        scatter3_one(nx, ny, nz, p, k, dt, bulk, rho0, base_of, nx, (int64_t)nx * ny, [&](int64_t node, int f, float v) {
+            q[f][node] = (int64_t)((uint64_t)q[f][node] + (uint64_t)quantize(v, bits[f]));
+        });
+}
+
+void orc_mpm3_decode_fixed(int64_t nodes, const int64_t* qm, const int64_t* qmu, const int64_t* qmv,
+                           const int64_t* qmw, float* gm, float* gmu, float* gmv, float* gmw) {
+    for (int64_t i = 0; i < nodes; i++) {
+        gm[i] = (float)std::ldexp((double)qm[i], -ORC_FIXED_MASS_BITS);
+        gmu[i] = (float)std::ldexp((double)qmu[i], -ORC_FIXED_MOMENTUM_BITS);
+        gmv[i] = (float)std::ldexp((double)qmv[i], -ORC_FIXED_MOMENTUM_BITS);
+        gmw[i] = (float)std::ldexp((double)qmw[i], -ORC_FIXED_MOMENTUM_BITS);
+    }
+}
+
 // Flip3.grid, Flip3.java:267-311 (the plain branch)
 void orc_mpm3_grid(int nx, int ny, int nz, const int32_t params[7], const float* gm, const float* gmu,
                    const float* gmv, const float* gmw, float* gu, float* gv, float* gw) {
@@ -343,11 +377,7 @@ void orc_mpm2_scatter(int nx, int ny, int64_t count, float* const p[10], const i
     }
 }
 
-// Flip.grid, Flip.java:366-412 (the plain branch: weight 1)
-// Fixed-point amounts: a * 2^bits is exact in binary32 (a power-of-two scaling), and llrintf
-// rounds it to the nearest integer, ties to even, under the default rounding mode.
-static inline int64_t quantize(float a, int bits) { return (int64_t)llrintf(std::ldexp(a, bits)); }
-
+// The 2D fixed-point scatter: each amount orc_mpm2_scatter computes, rounded by quantize.
 void orc_mpm2_scatter_fixed(int nx, int ny, int64_t count, float* const p[10], const int32_t* params,
                             int64_t* qm, int64_t* qmu, int64_t* qmv) {
     const float dt = bits_to_float(params[DT2]);
@@ -393,6 +423,7 @@ void orc_mpm2_decode_fixed(int64_t nodes, const int64_t* qm, const int64_t* qmu,
     }
 }
 
+// Flip.grid, Flip.java:366-412 (the plain branch: weight 1)
 void orc_mpm2_grid(int nx, int ny, const int32_t* params, const float* gm, const float* gmu, const float* gmv,
                    float* gu, float* gv) {
     const float dt = bits_to_float(params[DT2]);

@@ -77,6 +77,13 @@ void orc_swe_step_f64(int nx, int ny, const int edges[4],
 void orc_mpm3_clear(int nx, int ny, int nz, float* gm, float* gmu, float* gmv, float* gmw);
 void orc_mpm3_scatter(int nx, int ny, int nz, int64_t count, float* const p[17],
                       const int32_t params[7], float* gm, float* gmu, float* gmv, float* gmw);
+/* The 3D fixed-point scatter: every amount orc_mpm3_scatter adds, rounded to a 64-bit integer
+ * (mass in 2^-40, momentum in 2^-32, ties to even) and summed with wrap-around into q (zeroed
+ * first); orc_mpm3_decode_fixed turns the sums into gm gmu gmv gmw. */
+void orc_mpm3_scatter_fixed(int nx, int ny, int nz, int64_t count, float* const p[17], const int32_t params[7],
+                            int64_t* qm, int64_t* qmu, int64_t* qmv, int64_t* qmw);
+void orc_mpm3_decode_fixed(int64_t nodes, const int64_t* qm, const int64_t* qmu, const int64_t* qmv,
+                           const int64_t* qmw, float* gm, float* gmu, float* gmv, float* gmw);
 void orc_mpm3_grid(int nx, int ny, int nz, const int32_t params[7],
                    const float* gm, const float* gmu, const float* gmv, const float* gmw,
                    float* gu, float* gv, float* gw);

@@ -39,7 +39,9 @@ _SHORT = {
     "lod_kernel": "lod3", "after_sort_kernel": "mpm3_after_sort", "slots_kernel": "mpm3_slots",
     "fill_zero_kernel": "hll_fill_zero", "f64_to_f32_kernel": "f64_to_f32", "batch2_flags_kernel": "batch2_flags",
     "p2g3_layer_kernel": "p2g3_layer", "group_count_kernel": "layers3_count", "group_scan_kernel": "layers3_scan",
-    "group_list_kernel": "layers3_list", "group_order_kernel": "layers3_order",
+    "group_list_kernel": "layers3_list", "group_order_kernel": "layers3_order", "p2g3_fixed_kernel": "p2g3_fixed",
+    "fixed_rows_kernel": "fixed3_rows", "blur2_kernel": "tension2_blur", "stress2_kernel": "tension2_stress",
+    "grid2_tension_kernel": "grid2_tension", "batch2_decode_kernel": "batch2_decode",
 }
 
 

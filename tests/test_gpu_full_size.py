@@ -1,7 +1,8 @@
 """Parity at the configurations' own sizes.
 
-- 3D MLS-MPM, 8,388,608 particles on 256^3: one substep per kernel against the oracle -- every P2G
-  schedule's grid within FP32 reassociation, the grid update bit for bit from the oracle's sums,
+- 3D MLS-MPM, 8,388,608 particles on 256^3: one substep per kernel against the oracle -- every float
+  P2G schedule's grid within FP32 reassociation, the fixed-point one bit for bit, the grid update bit
+  for bit from the oracle's sums,
   G2P bit for bit (g2p=1) and within 1e-5 (g2p=2) from the oracle's velocities.
 - HLL on 4096^2: 100 steps bit for bit (the oracle's step on host threads), and with hll_fast
   within 1e-5.
@@ -103,6 +104,16 @@ def test_mpm3_two_cubes_substep_per_kernel(gpu):
     sim.sim.run_pass("scatter")
     for name in ("gm", "gmu", "gmv", "gmw"):
         _close(sim.sim.read(name), ref.g[name], f"p2g=3 {name}")
+    print("[full] 3D: P2G schedule 2 (fixed point) against the oracle's fixed-point scatter", flush=True)
+    f3 = {name: sim.sim.read(name).copy() for name in oracle.FIELDS3}
+    del ref, after
+    fixed = oracle.Mpm3(256, 256, 256, f3, words, fixed=True)
+    fixed.scatter()
+    sim.sim.set_option("p2g", 2)
+    sim.sim.run_pass("clear")
+    sim.sim.run_pass("scatter")
+    for name in ("gm", "gmu", "gmv", "gmw"):
+        _bits_equal(sim.sim.read(name), fixed.g[name], f"p2g=2 {name}")
     sim.close()
 
 

@@ -136,6 +136,14 @@ inline int64_t brick_count3(int nx, int ny, int nz) {
     return (int64_t)((nx + kBrick - 1) / kBrick) * ((ny + kBrick - 1) / kBrick) * ((nz + kBrick - 1) / kBrick);
 }
 int mpm3_clear(const Launch& l, const Mpm3Args& a);
+// P2G schedule 2 (fixed point): the sums of the amounts rounded to 64-bit integers (mass 2^-40,
+// momentum 2^-32), the same bits in any order. The clear zeroes them beside the floats; the
+// scatter adds into them and then decodes them into gm gmu gmv gmw.
+struct Fixed3 {
+    unsigned long long* q[4];
+};
+int mpm3_clear_fixed(const Launch& l, const Mpm3Args& a, const Fixed3& fx);
+int mpm3_scatter_fixed(const Launch& l, const Mpm3Args& a, const Fixed3& fx);
 int mpm3_scatter_atomic(const Launch& l, const Mpm3Args& a);
 // Chunks of 256 * run particles, run in {1, 2, 4}.
 int mpm3_scatter_tiled(const Launch& l, const Mpm3Args& a, int run);

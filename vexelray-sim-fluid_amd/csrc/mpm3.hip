@@ -634,6 +634,192 @@ __global__ __launch_bounds__(kWave) void p2g3_wave_kernel(Mpm3Args a) {
     }
 }
 
+// ------------------------------------------------------------------ P2G in fixed point
+//
+// Schedule 2: each of a particle's 108 amounts, computed as the reference computes it
+// (Flip3.java:216-243, the arithmetic of atomic_particle), is rounded to a 64-bit integer -- mass
+// in units of 2^-40, momentum in 2^-32 -- and the amounts are summed as integers, which
+// associate exactly: the sums are the same bits in any order and on any launch shape, equal to
+// the CPU restatement's (orc_mpm3_scatter_fixed), and so is everything the grid update and G2P
+// compute from them. core/particle/FixedScatter.java:24-31 gives the reason for integers in 2D;
+// these are its 3D counterpart on the MLS-MPM amounts. A wave takes 64 * R consecutive particles
+// (a lane R of them in a row); a lane folds each run of its particles that share a stencil base
+// into integer sums and adds them into an LDS tile of the box the wave's stencils cover, by 64-bit
+// LDS atomics (two of the four fields a sweep): within one instruction two lanes meet at a node
+// only where a cell's particles span both. The tile then leaves with one 64-bit global atomic per
+// nonzero node and field. A wave whose box exceeds the tile (unsorted particles) adds straight into the
+// grid. The pass ends by decoding the sums into gm gmu gmv gmw over the touched or live bricks,
+// so the passes after it read floats as before.
+constexpr int kFixedMassBits = 40;
+constexpr int kFixedMomentumBits = 32;
+constexpr int kFixedTile = 512;  // nodes: 4 fields x 8 B x 512 = 16 KB of LDS a wave
+constexpr int kFixedRun = 4;
+
+__device__ __forceinline__ float dequantize(unsigned long long q, int bits) {
+    return (float)ldexp((double)(long long)q, -bits);
+}
+
+// An amount in units of 2^-bits, rounded to the nearest integer (ties to even): the scaling is
+// exact, so this is the one rounding, the same as the CPU's llrintf(ldexpf(a, bits)). The integer
+// is held in a double, and a run's sums of up to kFixedRun amounts stay exact there while each
+// amount is below 2^11 in mass or 2^19 in momentum (a particle of mass 0.125 moving at 10^5 node
+// spacings a second is far inside both); they convert to 64 bits once, at the flush.
+__device__ __forceinline__ double quantize_d(float a, int bits) { return (double)rintf(ldexpf(a, bits)); }
+
+// Where a fixed-point sweep adds: the wave's LDS tile, or (when its box exceeds the tile) the grid.
+struct FixedTarget {
+    bool tiled;
+    Slab sl;
+    int64_t row, lay;
+};
+
+// One sweep of a lane's R particles over fields F0 and F0 + 1 ((m, mu) or (mv, mw)): each run of
+// particles sharing a stencil base folds into 27 x 2 integer sums, added into the tile (or the
+// grid) when the base changes and at the end. The amounts are atomic_particle's (Flip3.java:225-243).
+template <int R, int F0>
+__device__ __forceinline__ void fixed_sweep(const Mpm3Args& a, const Fixed3& fx, unsigned long long (*tile)[kFixedTile],
+                                            const FixedTarget& to, int64_t first, float dt, float bulk, float rho0) {
+    const int size[3] = {a.nx, a.ny, a.nz};
+    double acc[54];
+#pragma unroll
+    for (int i = 0; i < 54; ++i) acc[i] = 0.0;
+    int64_t run = -1;
+#pragma unroll 1
+    for (int r = 0; r <= R; ++r) {
+        // r == R: a pass that only flushes the last run
+        const int64_t q = first + (r < R ? r : 0);
+        const float m = r < R ? a.p[6][q] : 0.0f;
+        const bool live = m > 0.0f;
+        Axis ax[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) ax[i] = axis(a.p[i][q], size[i]);
+        const int64_t base = to.tiled ? ((int64_t)(ax[2].cell - to.sl.lo[2]) * to.sl.dims[1] +
+                                         (ax[1].cell - to.sl.lo[1])) * to.sl.dims[0] +
+                                            (ax[0].cell - to.sl.lo[0])
+                                      : ((int64_t)ax[2].cell * a.ny + ax[1].cell) * a.nx + ax[0].cell;
+        if (run >= 0 && (r == R || (live && base != run))) {
+#pragma unroll
+            for (int k = 0; k < 27; ++k) {
+                const int64_t node = run + (k / 9) * to.lay + ((k / 3) % 3) * to.row + k % 3;
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const unsigned long long v = (unsigned long long)(long long)acc[2 * k + j];
+                    if (to.tiled)
+                        atomicAdd(&tile[F0 + j][node], v);
+                    else
+                        atomicAdd(fx.q[F0 + j] + node, v);
+                    acc[2 * k + j] = 0.0;
+                }
+            }
+            run = -1;
+        }
+        if (!live) continue;
+        if (!to.tiled && F0 == 0) mark_one(a, ax);
+        run = base;
+        const float s = (4.0f * (dt * (m / rho0))) * (bulk * (1.0f - a.p[7][q]));
+        // field F0 + j's velocity and row of C: u and c0x for mu, v and c1x for mv, w and c2x for mw
+        constexpr int kA0 = F0 == 0 ? -1 : 1, kA1 = F0 == 0 ? 0 : 2;
+        float vel[2] = {0.0f, a.p[3 + kA1][q]}, c[2][3] = {};
+        if (kA0 >= 0) vel[0] = a.p[3 + kA0][q];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            if (kA0 >= 0) c[0][e] = a.p[8 + 3 * kA0 + e][q];
+            c[1][e] = a.p[8 + 3 * kA1 + e][q];
+        }
+#pragma unroll
+        for (int k = 0; k < 27; ++k) {
+            const float wgt = ax[0].w[k % 3] * (ax[1].w[(k / 3) % 3] * ax[2].w[k / 9]);
+            const float d[3] = {(float)(k % 3) - ax[0].off, (float)((k / 3) % 3) - ax[1].off,
+                                (float)(k / 9) - ax[2].off};
+            const float wm = wgt * m;
+            if (kA0 < 0) {
+                acc[2 * k] += quantize_d(wm, kFixedMassBits);
+            } else {
+                const float at3 = vel[0] + (c[0][0] * d[0] + (c[0][1] * d[1] + c[0][2] * d[2]));
+                acc[2 * k] += quantize_d(wm * at3 + wgt * (s * d[kA0]), kFixedMomentumBits);
+            }
+            const float at3 = vel[1] + (c[1][0] * d[0] + (c[1][1] * d[1] + c[1][2] * d[2]));
+            acc[2 * k + 1] += quantize_d(wm * at3 + wgt * (s * d[kA1]), kFixedMomentumBits);
+        }
+    }
+}
+
+template <int R>
+__global__ __launch_bounds__(kWave) void p2g3_fixed_kernel(Mpm3Args a, Fixed3 fx) {
+    __shared__ unsigned long long tile[4][kFixedTile];
+    const int lane = threadIdx.x;
+    if ((int64_t)blockIdx.x * kWave * R >= *a.active) return;  // a wave of inactive slots
+    const float dt = bits_f(a.params[DT]), bulk = bits_f(a.params[BULK]), rho0 = bits_f(a.params[RHO0]);
+    const int64_t first = ((int64_t)blockIdx.x * kWave + lane) * R;  // first + R <= capacity
+    const int size[3] = {a.nx, a.ny, a.nz};
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int64_t q = first + r;
+        if (!(a.p[6][q] > 0.0f)) continue;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int c = base_cell(a.p[i][q], size[i]);
+            lo[i] = min(lo[i], c);
+            hi[i] = max(hi[i], c);
+        }
+    }
+    const Box box = wave_box(lo, hi, kFixedTile);
+    if (box.empty) return;  // uniform: every lane holds the same box
+    FixedTarget to{};
+    to.tiled = box.nodes != INT_MAX;
+    if (to.tiled) {
+        const int nlo[3] = {box.lo[0], box.lo[1], box.lo[2]};
+        const int nhi[3] = {box.hi[0] + 2, box.hi[1] + 2, box.hi[2] + 2};
+        mark_bricks(a, nlo, nhi, lane, kWave);
+        to.sl = tile_of(box);
+        for (int t = lane; t < to.sl.nodes; t += kWave)
+#pragma unroll
+            for (int f = 0; f < 4; ++f) tile[f][t] = 0ull;
+        __syncthreads();  // the block is this one wave
+    }
+    to.row = to.tiled ? to.sl.dims[0] : a.nx;
+    to.lay = to.tiled ? (int64_t)to.sl.dims[0] * to.sl.dims[1] : (int64_t)a.nx * a.ny;
+    fixed_sweep<R, 0>(a, fx, tile, to, first, dt, bulk, rho0);
+    fixed_sweep<R, 2>(a, fx, tile, to, first, dt, bulk, rho0);
+    if (!to.tiled) return;
+    __syncthreads();
+    const Slab& sl = to.sl;
+    const int ex = sl.dims[0], ey = sl.dims[1];
+    for (int t = lane; t < sl.nodes; t += kWave) {
+        const int lx = t % ex, ly = (t / ex) % ey, lz = t / (ex * ey);
+        const int64_t node = ((int64_t)(sl.lo[2] + lz) * a.ny + (sl.lo[1] + ly)) * a.nx + (sl.lo[0] + lx);
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            const unsigned long long v = tile[f][t];
+            if (v != 0ull) atomicAdd(fx.q[f] + node, v);
+        }
+    }
+}
+
+// Over the span of touched or live bricks of each row, as the clear: the fixed-point sums zeroed
+// (`decode` false), or decoded into gm gmu gmv gmw (`decode` true).
+__global__ __launch_bounds__(256) void fixed_rows_kernel(Mpm3Args a, Fixed3 fx, int decode) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= (int64_t)a.ny * a.nz) return;
+    const int y = (int)(row % a.ny), z = (int)(row / a.ny);
+    int x0, x1;
+    if (!row_range(a, y, z, lane, &x0, &x1)) return;
+    const int64_t first = row * a.nx;
+    for (int x = x0 + lane; x < x1; x += kWave) {
+        const int64_t i = first + x;
+        if (decode) {
+            a.g[0][i] = dequantize(fx.q[0][i], kFixedMassBits);
+#pragma unroll
+            for (int f = 1; f < 4; ++f) a.g[f][i] = dequantize(fx.q[f][i], kFixedMomentumBits);
+        } else {
+#pragma unroll
+            for (int f = 0; f < 4; ++f) fx.q[f][i] = 0ull;
+        }
+    }
+}
+
 // ------------------------------------------------------------------ P2G by bricks
 //
 // P2G schedule 3. The unit is a group: the 256 cells (8 x 8 by 4 layers, half a sort brick) whose
@@ -1465,6 +1651,27 @@ int mpm3_clear(const Launch& l, const Mpm3Args& a) {
     hipLaunchKernelGGL(clear_rows_kernel, dim3(blocks_for((int64_t)a.ny * a.nz, 4)), dim3(256), 0, l.stream, a);
     timed_end(l, "mpm3_clear");
     return check_launch("mpm3_clear");
+}
+
+int mpm3_clear_fixed(const Launch& l, const Mpm3Args& a, const Fixed3& fx) {
+    timed_begin(l, "mpm3_clear");
+    hipLaunchKernelGGL(clear_rows_kernel, dim3(blocks_for((int64_t)a.ny * a.nz, 4)), dim3(256), 0, l.stream, a);
+    hipLaunchKernelGGL(fixed_rows_kernel, dim3(blocks_for((int64_t)a.ny * a.nz, 4)), dim3(256), 0, l.stream, a, fx, 0);
+    timed_end(l, "mpm3_clear");
+    return check_launch("mpm3_clear");
+}
+
+int mpm3_scatter_fixed(const Launch& l, const Mpm3Args& a, const Fixed3& fx) {
+    if (a.count == 0) return 0;
+    timed_begin(l, "p2g3_fixed");
+    hipLaunchKernelGGL(p2g3_fixed_kernel<kFixedRun>, dim3(blocks_for(a.count, kWave * kFixedRun)), dim3(kWave), 0,
+                       l.stream, a, fx);
+    timed_end(l, "p2g3_fixed");
+    if (int rc = check_launch("p2g3_fixed")) return rc;
+    timed_begin(l, "fixed3_decode");
+    hipLaunchKernelGGL(fixed_rows_kernel, dim3(blocks_for((int64_t)a.ny * a.nz, 4)), dim3(256), 0, l.stream, a, fx, 1);
+    timed_end(l, "fixed3_decode");
+    return check_launch("fixed3_decode");
 }
 
 int mpm3_scatter_atomic(const Launch& l, const Mpm3Args& a) {

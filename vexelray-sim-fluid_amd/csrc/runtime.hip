@@ -166,6 +166,9 @@ struct vxr_sim {
     // positions, masses or slots, no lod pass, no clear since)
     vxr::Layers layers{};
     bool layers_valid = false;
+    // 3D P2G in fixed point (option p2g = 2): the 64-bit sums of gm gmu gmv gmw, made when the
+    // option is first set
+    unsigned long long* q3[4] = {};
 
     // SWE
     float* state[2][3] = {};
@@ -448,10 +451,12 @@ int run_pass(vxr_sim* s, const std::string& pass) {
     if (pass == "sort") return sort_pass(s);
     if (s->kind == VXR_KIND_MPM3) {
         const vxr::Mpm3Args a = mpm3_args(s);
-        if (pass == "clear") return vxr::mpm3_clear(l, a);
+        const vxr::Fixed3 fx{{s->q3[0], s->q3[1], s->q3[2], s->q3[3]}};
+        if (pass == "clear") return opt(s, "p2g") == 2 ? vxr::mpm3_clear_fixed(l, a, fx) : vxr::mpm3_clear(l, a);
         if (pass == "scatter") {
             const int64_t p2g = opt(s, "p2g");
             if (p2g == 0) return vxr::mpm3_scatter_atomic(l, a);
+            if (p2g == 2) return vxr::mpm3_scatter_fixed(l, a, fx);
             if (p2g == 3 && s->layers_valid) return vxr::mpm3_scatter_layers(l, a, s->layers);
             return vxr::mpm3_scatter_tiled(l, a, (int)opt(s, "run"));
         }
@@ -758,6 +763,8 @@ int clear_all(vxr_sim* s) {
             for (float* f : s->colour) HIP_TRY(hipMemsetAsync(f, 0, round_up(s->nodes, 4) * 4, s->stream));
             for (int k = 0; k < 3; ++k) HIP_TRY(hipMemsetAsync(s->stress[k], 0, round_up(s->nodes, 4) * 4, s->stream));
         }
+        for (unsigned long long* q : s->q3)
+            if (q) HIP_TRY(hipMemsetAsync(q, 0, round_up(s->nodes, 4) * 8, s->stream));
         if (s->stay) {
             const int64_t groups = std::max<int64_t>(s->count / 8, 1);
             HIP_TRY(hipMemsetAsync(s->level, 0, groups * 4, s->stream));
@@ -1390,8 +1397,19 @@ int vxr_set_option(vxr_sim* s, const char* key, int64_t value) {
                             (long long)value);
             if (std::string(key) == "run" && value == 3)
                 return fail(VXR_EINVAL, "option 'run' takes 1, 2 or 4, got 3");
-            if (std::string(key) == "p2g" && value == 2 && s->kind != VXR_KIND_MPM2 && s->kind != VXR_KIND_MPM2_BATCH)
-                return fail(VXR_EINVAL, "p2g 2 (fixed point) is built for 2D scenes");
+            if (std::string(key) == "p2g" && value == 2 && s->kind == VXR_KIND_SWE)
+                return fail(VXR_EINVAL, "p2g is for particle simulations");
+            if (std::string(key) == "p2g" && value == 2 && s->kind == VXR_KIND_MPM3 && !s->q3[0]) {
+                // the fixed-point sums, made once and zeroed (the option change below marks every
+                // brick, so the first clear zeroes them again over the whole grid)
+                const int64_t words = 2 * round_up(s->nodes, 4);
+                for (int f = 0; f < 4; ++f) {
+                    void* ptr = nullptr;
+                    if (int rc = alloc(s, words, &ptr)) return rc;
+                    s->q3[f] = (unsigned long long*)ptr;
+                    HIP_TRY(hipMemsetAsync(ptr, 0, words * 4, s->stream));
+                }
+            }
             if (std::string(key) == "p2g" && value != 2 && s->kind == VXR_KIND_MPM2_BATCH)
                 return fail(VXR_EINVAL, "a batch of scenes always scatters in fixed point (p2g 2)");
             drop_graphs(s);
