@@ -456,6 +456,25 @@ __device__ __forceinline__ void flush_run(float4* tile, int* claim, const Slab& 
     }
 }
 
+// Before a wave's last flush: where lanes 2k and 2k+1 hold runs with one base (a cell's
+// particles split over the two lanes, the usual case for sorted input), lane 2k takes lane
+// 2k+1's sums and lane 2k+1 has nothing left to flush, so the flush needs one claim round where
+// it took two. Called by the whole wave (DPP reads every lane).
+__device__ __forceinline__ void merge_pairs(int& run, float (&acc)[108]) {
+    const int next = __builtin_amdgcn_mov_dpp(run, 0xF5, 0xF, 0xF, false);  // quad_perm [1,1,3,3]
+    const int prev = __builtin_amdgcn_mov_dpp(run, 0xA0, 0xF, 0xF, false);  // quad_perm [0,0,2,2]
+    const bool even = (threadIdx.x & 1) == 0;
+    const bool take = even && run >= 0 && next == run;
+    const bool given = !even && run >= 0 && prev == run;
+#pragma unroll
+    for (int i = 0; i < 108; ++i) {
+        const float v = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(acc[i]), 0xF5, 0xF, 0xF, false));
+        if (take) acc[i] += v;
+        if (given) acc[i] = 0.0f;  // handed over: the sums start from zero again, as after a flush
+    }
+    if (given) run = -1;
+}
+
 // The bricks a lone particle's stencil reaches, marked by this lane alone.
 __device__ __forceinline__ void mark_one(const Mpm3Args& a, const Axis (&ax)[3]) {
     const int lo[3] = {ax[0].cell, ax[1].cell, ax[2].cell};
@@ -576,6 +595,7 @@ __global__ __launch_bounds__(kWave) void p2g3_wave_kernel(Mpm3Args a) {
             take(a, law, tile, claim, sl, bk[r] != INT_MAX, f[0][r], f[1][r], f[2][r], f[6][r], f[7][r], vel, c,
                  run_base, acc);
         }
+        merge_pairs(run_base, acc);
         flush_run(tile, claim, sl, run_base, run_base >= 0, acc);
         leave_tile(a, tile, sl);
         return;
@@ -609,6 +629,7 @@ __global__ __launch_bounds__(kWave) void p2g3_wave_kernel(Mpm3Args a) {
             take(a, law, tile, claim, sl, bk[r] == cur, a.p[0][q], a.p[1][q], a.p[2][q], a.p[6][q], a.p[7][q], vel, c,
                  run_base, acc);
         }
+        merge_pairs(run_base, acc);
         flush_run(tile, claim, sl, run_base, run_base >= 0, acc);
         leave_tile(a, tile, sl);
         cur = lanes_min(min_of(bk, cur));
