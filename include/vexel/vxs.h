@@ -44,14 +44,15 @@ const char* vxs_last_error(void);
 int vxs_conv3x3(void* stream, const void* in, int images, int H, int W, int cin, const void* w, const float* bias,
                 int cout, void* out, int out_ch, const void* mask, int flags);
 
-/* Words of scratch vxs_conv3x3_wgrad needs for `pixels` pixels in chunks of `chunk`. */
-int64_t vxs_wgrad_scratch(int64_t pixels, int cin, int cout, int64_t chunk);
+/* Words of FP32 scratch vxs_conv3x3_wgrad needs for channels (cin, cout), at any image size;
+ * -1 for channels it does not take. */
+int64_t vxs_wgrad_scratch(int cin, int cout);
 
-/* The weight and bias gradients of a 3x3 convolution from its input x and output gradient dy:
- * dw[co][tap][ci] and db[co] in FP32, summed in a fixed order (per chunk, then over chunks).
- * chunk: a positive multiple of 32 pixels. */
+/* The weight and bias gradients of a 3x3 convolution from its input x and output gradient dy
+ * (NHWC bf16, images x H x W): dw[co][tap][ci] and db[co] in FP32, summed in an order fixed by
+ * the image size (per block of the launch, then over blocks). (cin, cout) as vxs_conv3x3. */
 int vxs_conv3x3_wgrad(void* stream, const void* x, const void* dy, int images, int H, int W, int cin, int cout,
-                      int64_t chunk, float* scratch, float* dw, float* db);
+                      float* scratch, float* dw, float* db);
 
 /* The weights of the transposed convolution (input gradient): wt[ci][8 - tap][co] = w[co][tap][ci]. */
 int vxs_flip(void* stream, const void* w, int cout, int cin, void* wt);

@@ -83,31 +83,48 @@ def test_conv_masked_input_gradient_matches_autograd(gpu):
     assert float(((got - ref).abs() / (ref.abs() + 1.0)).max()) <= 1e-4
 
 
-@pytest.mark.parametrize("cin,cout", [(16, 64), (64, 64), (64, 16)])
-def test_weight_gradient_matches_autograd(gpu, cin, cout):
+@pytest.mark.parametrize("shape", [(2, 23, 41), (1, 21, 200), (1, 130, 1024), (600, 4, 9)],
+                         ids=["edges", "interior", "long-segments", "many-segments"])
+@pytest.mark.parametrize("cin,cout", [(16, 64), (64, 64), (64, 16), (16, 16)])
+def test_weight_gradient_matches_autograd(gpu, cin, cout, shape):
+    """23 x 41: odd rows, a partial strip, every row on an image edge; 21 x 200: strips clear of
+    the side edges; 130 x 1024: segments of several steps with rows in flight ahead; 600 images
+    of 4 x 9: more segments than the launch has blocks, so blocks walk several. Twice, equal."""
     gen = np.random.default_rng(cin * 3 + cout)
-    n, h, w = 2, 23, 41
+    n, h, w = shape
     x = _bf((n, h, w, cin), gen)
     dy = _bf((n, h, w, cout), gen)
     k = torch.zeros((cout, 9, cin), requires_grad=True)
     b = torch.zeros(cout, requires_grad=True)
     y = _conv_ref(x, k, b, False)
     (y * dy.float()).sum().backward()
-    print(f"[surrogate] wgrad {cin}->{cout}", flush=True)
+    print(f"[surrogate] wgrad {cin}->{cout} {shape}", flush=True)
     s = torch.cuda.current_stream().cuda_stream
     lib = _native.lib()
-    px = n * h * w
-    chunk = 256  # several chunks, the last partial
-    scratch = torch.empty(lib.vxs_wgrad_scratch(px, cin, cout, chunk), dtype=torch.float32, device="cuda")
-    dw = torch.empty(cout * 9 * cin, dtype=torch.float32, device="cuda")
-    db = torch.empty(cout, dtype=torch.float32, device="cuda")
+    words = lib.vxs_wgrad_scratch(cin, cout)
+    assert words > 0
+    scratch = torch.empty(words, dtype=torch.float32, device="cuda")
     dx, ddy = x.cuda(), dy.cuda()
-    _native.vxs_check(lib.vxs_conv3x3_wgrad(s, dx.data_ptr(), ddy.data_ptr(), n, h, w, cin, cout, chunk,
-                                            scratch.data_ptr(), dw.data_ptr(), db.data_ptr()))
-    got = dw.cpu().reshape(cout, 9, cin)
+    runs = []
+    for _ in range(2):
+        dw = torch.empty(cout * 9 * cin, dtype=torch.float32, device="cuda")
+        db = torch.empty(cout, dtype=torch.float32, device="cuda")
+        _native.vxs_check(lib.vxs_conv3x3_wgrad(s, dx.data_ptr(), ddy.data_ptr(), n, h, w, cin, cout,
+                                                scratch.data_ptr(), dw.data_ptr(), db.data_ptr()))
+        runs.append((dw.cpu(), db.cpu()))
+    got, gotb = runs[0]
     ref = k.grad
-    assert float(((got - ref).abs() / (ref.abs().max() + 1e-6)).max()) <= 1e-4
-    assert float(((db.cpu() - b.grad).abs() / (b.grad.abs().max() + 1e-6)).max()) <= 1e-4
+    # FP32 sums of bf16 products in another order than torch's
+    assert float(((got.reshape(cout, 9, cin) - ref).abs() / (ref.abs().max() + 1e-6)).max()) <= 1e-4
+    assert float(((gotb - b.grad).abs() / (b.grad.abs().max() + 1e-6)).max()) <= 1e-4
+    assert torch.equal(runs[0][0], runs[1][0]) and torch.equal(runs[0][1], runs[1][1])
+
+
+def test_weight_gradient_refuses_other_channels(gpu):
+    lib = _native.lib()
+    assert lib.vxs_wgrad_scratch(32, 64) == -1
+    rc = lib.vxs_conv3x3_wgrad(None, 1, 1, 1, 8, 8, 32, 64, 1, 1, 1)
+    assert rc == 1 and b"16 or 64" in lib.vxs_last_error()
 
 
 def test_hll64_bit_exact_with_oracle(gpu):

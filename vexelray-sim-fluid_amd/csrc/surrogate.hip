@@ -104,9 +104,11 @@ __global__ __launch_bounds__(kBlock) void conv3x3_kernel(const __bf16* __restric
 constexpr int kRows = 2;
 constexpr int kCols = 64;
 
-// 16 bytes of zeros in global memory: the source of the halo slots outside the image and of the
-// pad slot after each pixel's channels, so every lane of a staging load has an address.
-__device__ __align__(16) const unsigned kZero16[4] = {0u, 0u, 0u, 0u};
+// 4 KB of zeros in global memory: the source of the halo pixels outside the image. A staging load
+// takes its lane's own 16 bytes of it, so that the zeros of one load do not all queue on one
+// address (one L2 channel).
+constexpr int kZeroChunks = 256;
+__device__ __align__(16) const unsigned kZeros[4 * kZeroChunks] = {};
 
 // Where the lanes of a store with nothing to store (a pixel past the image edge) write, so that
 // every store instruction of the plain epilogue is issued.
@@ -124,42 +126,42 @@ __device__ __forceinline__ void wait_vmcnt() {
 // must not be moved above it or served from values it assumed.
 __device__ __forceinline__ void lds_fence() { asm volatile("" ::: "memory"); }
 
-// Stages input tile `tt` into `buf` with global-to-LDS loads (global_load_lds_dwordx4): the tile
-// is kTileH x kTileW pixels of kCin / 8 16-byte chunks and one pad chunk, chunk e of the tile at
+// Stages tile `tt` of an NHWC image into `buf` with global-to-LDS loads (global_load_lds_dwordx4):
+// the tile is kTileH x kTileW pixels -- the kRows x kCols pixels of the tile and a ring of kHalo
+// pixels about them -- of kCin / 8 16-byte chunks and kPad pad chunks, chunk e of the tile at
 // buf + 16 e; each load instruction fills 64 consecutive chunks, lane l the l-th, and wave w
-// takes instructions w, w + 4, ... Nothing passes through registers, so every load of the tile
-// is in flight at once; the caller waits (vmcnt) and synchronises before reading. A lane's chunk
+// takes instructions w, w + 4, ... The lanes of pad chunks load nothing (no reader looks at a
+// pad: pads only space pixels over the LDS banks), and pixels outside the image are zeros. Nothing passes through registers, so every load
+// of the tile is in flight at once; the caller waits (vmcnt) and synchronises before reading. A lane's chunk
 // (pixel row ty, column tx, slot cg) moves by 256 chunks from one of its instructions to the
 // next, kept up to date by additions rather than divided out each time.
-template <int kCin>
+template <int kCin, int kHalo = 1, int kPad = 1>
 __device__ __forceinline__ void stage_tile(const __bf16* __restrict__ in, int H, int W, int64_t tt, __bf16* buf) {
-    constexpr int kSlots = kCin / 8 + 1;
-    constexpr int kTileW = kCols + 2, kTileH = kRows + 2;
+    constexpr int kSlots = kCin / 8 + kPad;
+    constexpr int kTileW = kCols + 2 * kHalo, kTileH = kRows + 2 * kHalo;
     constexpr int kChunks = kTileH * kTileW * kSlots;
     constexpr int kLoads = (kChunks + 63) / 64;
     constexpr int kPerWave = (kLoads + 3) / 4;
     constexpr int kStepPix = 256 / kSlots, kStepSlot = 256 % kSlots;  // 256 chunks
-    static_assert(kStepPix + 1 < 2 * kTileW, "two wraps of the column at most");
+    static_assert(kPerWave == 1 || kStepPix + 1 < 2 * kTileW, "two wraps of the column at most");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int tiles_x = (W + kCols - 1) / kCols, tiles_y = (H + kRows - 1) / kRows;
     const int n = (int)(tt / ((int64_t)tiles_y * tiles_x));
     const int rem = (int)(tt % ((int64_t)tiles_y * tiles_x));
     const int y0 = (rem / tiles_x) * kRows, x0 = (rem % tiles_x) * kCols;
     const __bf16* base = in + ((int64_t)n * H * W + (int64_t)y0 * W + x0) * kCin;
-    const bool inside = y0 >= 1 && y0 + kRows < H && x0 >= 1 && x0 + kCols < W;
+    const bool inside = y0 >= kHalo && y0 + kRows + kHalo <= H && x0 >= kHalo && x0 + kCols + kHalo <= W;
     const int e0 = 64 * wave + lane;
     int pix = e0 / kSlots, cg = e0 % kSlots;
     int ty = pix / kTileW, tx = pix % kTileW;
 #pragma unroll 1
     for (int jj = 0; jj < kPerWave; ++jj) {
         const int j = wave + 4 * jj;
-        if (j < kLoads && 64 * j + lane < kChunks) {
-            const void* src = kZero16;
-            if (cg < kCin / 8) {
-                const int yy = y0 + ty - 1, xx = x0 + tx - 1;
-                if (inside || (yy >= 0 && yy < H && xx >= 0 && xx < W))
-                    src = base + ((ty - 1) * W + (tx - 1)) * kCin + 8 * cg;
-            }
+        if (j < kLoads && 64 * j + lane < kChunks && cg < kCin / 8) {
+            const void* src = kZeros + 4 * ((64 * j + lane) % kZeroChunks);
+            const int yy = y0 + ty - kHalo, xx = x0 + tx - kHalo;
+            if (inside || (yy >= 0 && yy < H && xx >= 0 && xx < W))
+                src = base + ((ty - kHalo) * W + (tx - kHalo)) * kCin + 8 * cg;
             __builtin_amdgcn_global_load_lds(src, buf + 64 * 8 * j, 16, 0, 0);
         }
         // the next instruction of this wave: 256 chunks on
@@ -364,100 +366,256 @@ __global__ __launch_bounds__(kBlock, 2) void conv3x3_lds_kernel(const __bf16* __
     }
 }
 
-// Weight gradient of one tap over a chunk of pixels:
-//   part[chunk][tap][co][ci] = sum_{p in chunk} dy[p][co] * x[p + off(tap)][ci]
-// (and, for the centre tap, part_b[chunk][co] = sum_p dy[p][co]). 32 pixels at a time are staged
-// into LDS transposed -- dyT[co][p], xT[ci][p] -- so each lane's 8 consecutive pixels are one
-// 16-byte LDS read; a wave takes every fourth of the (co, ci) 16x16 tiles.
+// How the weight gradient splits its work. One side of the product (the output gradient or the
+// input) is read once a K step of 32 pixels, all its channels (kTF 16-channel tiles); the other,
+// the "shifted" side, once a tap, 16 channels a wave: the side with fewer channels, so that the
+// nine-fold reads fall on the narrower one. A block walks a strip of kCols columns down a segment
+// of rows, kRows output rows a step; its four waves split the shifted side's channel tiles (kGs)
+// and a step's K steps (kGk). kDepth steps' rows are in flight ahead of the one computing.
 template <int kCin, int kCout>
-__global__ __launch_bounds__(kBlock) void conv3x3_wgrad_kernel(const __bf16* __restrict__ x,
-                                                               const __bf16* __restrict__ dy, int H, int W,
-                                                               int64_t pixels, int64_t chunk,
-                                                               float* __restrict__ part,
-                                                               float* __restrict__ part_b) {
-    constexpr int kTiles = (kCout / 16) * (kCin / 16);
-    constexpr int kPerWave = kTiles / 4;
-    __shared__ __align__(16) __bf16 dyT[kCout][32 + 8];
-    __shared__ __align__(16) __bf16 xT[kCin][32 + 8];
-    const int tap = blockIdx.y;
-    const int dyo = tap / 3 - 1, dxo = tap % 3 - 1;
+struct WgradShape {
+    static constexpr bool kShiftX = kCin <= kCout;      // the input is the shifted side
+    static constexpr int kCF = kShiftX ? kCout : kCin;  // channels of the side read once a K step
+    static constexpr int kCS = kShiftX ? kCin : kCout;  // and of the shifted side
+    static constexpr int kTF = kCF / 16;
+    static constexpr int kGs = kCS / 16;
+    static constexpr int kGk = 4 / kGs;
+    static constexpr int kRows = kGk == 1 ? 1 : 2;     // output rows a step: 2 K steps each
+    static constexpr int kDepth = kGk == 1 ? 4 : 2;
+    static constexpr int kBlocksPerCu = kGk == 1 ? 1 : 2;  // as the rings' LDS allows
+    static constexpr int kWords = 9 * kCin * kCout + kCout;  // a partial: dw[co][tap][ci], then db[co]
+};
+
+// Pad chunks after a pixel's channels in the weight gradient's rows: pixel strides of 40 banks
+// (64 channels) and 8 banks (16), so the 8 consecutive pixels of a transposing read's half fall on
+// disjoint banks.
+template <int kC>
+constexpr int wgrad_pad() { return kC == 64 ? 2 : 0; }
+
+// One row of a weight-gradient ring: kCols pixels and kHalo more each side, kC / 8 16-byte chunks
+// and kPad pad chunks a pixel, chunk e at 16 e bytes; load instruction j fills chunks 64 j ..
+// 64 j + 63 and is wave j % 4's. kMinLoads: the instructions a row takes from the wave that
+// issues fewest.
+template <int kC, int kHalo, int kPad>
+struct RowStage {
+    static constexpr int kSlots = kC / 8 + kPad;
+    static constexpr int kPix = kCols + 2 * kHalo;
+    static constexpr int kChunks = kPix * kSlots;
+    static constexpr int kLoads = (kChunks + 63) / 64;
+    static constexpr int kElems = kChunks * 8;
+    static constexpr int kMinLoads = kLoads / 4;
+};
+
+// Stages row y of the image at pixel offset img (columns x0 - kHalo on) into `buf` with
+// global-to-LDS loads: pixels outside the image are zeros, pad chunks are not loaded.
+template <int kC, int kHalo, int kPad>
+__device__ __forceinline__ void stage_row(const __bf16* __restrict__ in, int64_t img, int y, int x0, int H, int W,
+                                          __bf16* buf) {
+    typedef RowStage<kC, kHalo, kPad> R;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int i = lane & 15, kg = lane >> 4;
-    const int64_t p_begin = (int64_t)blockIdx.x * chunk;
-    const int64_t p_end = p_begin + chunk < pixels ? p_begin + chunk : pixels;
-    f32x4 acc[kPerWave];
+    const bool row_in = y >= 0 && y < H;
+    const __bf16* base = in + (img + (int64_t)(row_in ? y : 0) * W) * kC;
 #pragma unroll
-    for (int t = 0; t < kPerWave; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    float bsum = 0.0f;
-    const int64_t hw = (int64_t)H * W;
-    for (int64_t p0 = p_begin; p0 < p_end; p0 += 32) {
-        // stage: thread t copies channel group (t % (C/8)) of pixel p0 + t / (C/8), transposed
-        for (int e = threadIdx.x; e < 32 * (kCout / 8); e += kBlock) {
-            const int pl = e / (kCout / 8), cg = e % (kCout / 8);
-            const int64_t p = p0 + pl;
-            bf16x8 v = zero8();
-            if (p < p_end) v = *(const bf16x8*)(dy + p * kCout + 8 * cg);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) dyT[8 * cg + j][pl] = v[j];
+    for (int jj = 0; jj < (R::kLoads + 3) / 4; ++jj) {
+        const int j = wave + 4 * jj;
+        const int e = 64 * j + lane;
+        const int px = e / R::kSlots, cg = e % R::kSlots;
+        if (j < R::kLoads && e < R::kChunks && cg < kC / 8) {
+            const int xx = x0 + px - kHalo;
+            const void* src = kZeros + 4 * (e % kZeroChunks);
+            if (row_in && xx >= 0 && xx < W) src = base + (int64_t)xx * kC + 8 * cg;
+            __builtin_amdgcn_global_load_lds(src, buf + 64 * 8 * j, 16, 0, 0);
         }
-        for (int e = threadIdx.x; e < 32 * (kCin / 8); e += kBlock) {
-            const int pl = e / (kCin / 8), cg = e % (kCin / 8);
-            const int64_t p = p0 + pl;
-            bf16x8 v = zero8();
-            if (p < p_end) {
-                const int64_t n = p / hw, rem = p % hw;
-                const int yy = (int)(rem / W) + dyo, xx = (int)(rem % W) + dxo;
-                if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-                    v = *(const bf16x8*)(x + (n * hw + (int64_t)yy * W + xx) * kCin + 8 * cg);
-            }
-#pragma unroll
-            for (int j = 0; j < 8; ++j) xT[8 * cg + j][pl] = v[j];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int t = 0; t < kPerWave; ++t) {
-            const int tile = wave + 4 * t;
-            const int ct = tile / (kCin / 16), it = tile % (kCin / 16);
-            const bf16x8 a = *(const bf16x8*)&dyT[16 * ct + i][8 * kg];
-            const bf16x8 b = *(const bf16x8*)&xT[16 * it + i][8 * kg];
-            acc[t] = mfma(a, b, acc[t]);
-        }
-        if (tap == 4 && threadIdx.x < kCout) {
-            for (int j = 0; j < 32; ++j) bsum += (float)dyT[threadIdx.x][j];
-        }
-        __syncthreads();
     }
-    // lane holds C[4 kg + r][i]: co = 16 ct + 4 kg + r, ci = 16 it + i
-    float* out = part + ((int64_t)blockIdx.x * 9 + tap) * kCout * kCin;
-#pragma unroll
-    for (int t = 0; t < kPerWave; ++t) {
-        const int tile = wave + 4 * t;
-        const int ct = tile / (kCin / 16), it = tile % (kCin / 16);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) out[(16 * ct + 4 * kg + r) * kCin + 16 * it + i] = acc[t][r];
-    }
-    if (tap == 4 && threadIdx.x < kCout) part_b[(int64_t)blockIdx.x * kCout + threadIdx.x] = bsum;
 }
 
-// dw[co][tap][ci] = sum over chunks of part[chunk][tap][co][ci]; db[co] likewise, in chunk order.
-__global__ __launch_bounds__(kBlock) void wgrad_reduce_kernel(const float* __restrict__ part,
-                                                              const float* __restrict__ part_b, int chunks,
-                                                              int cout, int cin, float* __restrict__ dw,
-                                                              float* __restrict__ db) {
-    const int64_t n = (int64_t)9 * cout * cin;
-    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (e < n) {
-        const int tap = (int)(e / ((int64_t)cout * cin));
-        const int rem = (int)(e % ((int64_t)cout * cin));
-        const int co = rem / cin, ci = rem % cin;
-        float s = 0.0f;
-        for (int c = 0; c < chunks; ++c) s += part[(int64_t)c * n + e];
-        dw[((int64_t)co * 9 + tap) * cin + ci] = s;
+// s_waitcnt vmcnt(min(n, 63)) for a wave-uniform n.
+template <int N = 63>
+__device__ __forceinline__ void wait_vmcnt_at_most(int n) {
+    if constexpr (N == 0) {
+        wait_vmcnt<0>();
+    } else {
+        if (n >= N) wait_vmcnt<N>();
+        else wait_vmcnt_at_most<N - 1>(n);
     }
-    if (e < cout) {
-        float s = 0.0f;
-        for (int c = 0; c < chunks; ++c) s += part_b[(int64_t)c * cout + e];
-        db[e] = s;
+}
+
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+
+// An MFMA operand of 8 pixels of one channel from a pixel-major LDS row, by gfx950's transposing
+// read (ds_read_b64_tr_b16): within each 16-lane group, lane i receives element i % 4 of the
+// 4-element rows the lanes 4 m + i / 4 address, m = 0..3. With lane (i, kg) addressing pixel
+// 4 kg + i / 4, channels c + 4 (i % 4) .. + 3 (`p`), lane i receives channel c + i of pixels
+// 4 kg .. 4 kg + 3; a second read 16 pixels on gives 4 kg + 16 .. + 3. So k slot 4 h + m of lane
+// group kg is pixel 4 kg + 16 h + m, for every operand read this way.
+__device__ __forceinline__ bf16x8 read_pixels(const __bf16* p, int stride) {
+    const s16x4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((s16x4 __attribute__((address_space(3)))*)p);
+    const s16x4 hi =
+        __builtin_amdgcn_ds_read_tr16_b64_v4i16((s16x4 __attribute__((address_space(3)))*)(p + 16 * stride));
+    const s16x8 v = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_bit_cast(bf16x8, v);
+}
+
+// The weight and bias gradients of a 3x3 convolution as a GEMM over pixels on the matrix cores:
+//   part[p][co][tap][ci] = sum over the block's pixels of dy[pixel][co] x[pixel + off(tap)][ci]
+//   part[p][9 cout cin + co] = sum over the block's pixels of dy[pixel][co]
+// with p = block kGk + (the wave's K group). The pixels are segments of seg_rows rows of a
+// kCols-column strip; a block walks its segments a step (kRows rows) at a time, with the rows of
+// the next kDepth steps loading into two LDS rings by global-to-LDS loads: the shifted side's rows
+// with one more pixel each end and the rows above and below (zeros outside the image), the other
+// side's rows as they are. Every staged row is loaded once. The product's K index is the pixel:
+// operands come out of the pixel-major rows by transposing reads. When the output gradient is the
+// shifted side the sum runs as sum over q of dy[q - off(tap)][co] x[q][ci] (the same sum: x is
+// zero outside the image, dy's ring zero). The bias gradient is one more product, of the output
+// gradient against ones.
+template <int kCin, int kCout>
+__global__ __launch_bounds__(kBlock, (WgradShape<kCin, kCout>::kBlocksPerCu)) void conv3x3_wgrad_kernel(
+    const __bf16* __restrict__ x, const __bf16* __restrict__ dy, int images, int H, int W, int seg_rows,
+    float* __restrict__ part) {
+    typedef WgradShape<kCin, kCout> S;
+    constexpr int kTF = S::kTF, P = S::kRows, D = S::kDepth;
+    constexpr int kPF = wgrad_pad<S::kCF>(), kPS = wgrad_pad<S::kCS>();
+    typedef RowStage<S::kCS, 1, kPS> RS;
+    typedef RowStage<S::kCF, 0, kPF> RF;
+    constexpr int kSS = RS::kSlots * 8, kSF = RF::kSlots * 8;  // LDS pixel strides
+    // ring rows: a step reads P + 2 shifted rows and P others, and D steps' rows are in flight
+    constexpr int kNs = P * (D + 1) + 2, kNf = P * (D + 1);
+    // load instructions a step's rows take from the wave issuing fewest: waiting until this many
+    // times the steps issued since are outstanding waits for a step's rows in every wave
+    constexpr int kUnitLoads = P * (RS::kMinLoads + RF::kMinLoads);
+    __shared__ __align__(16) __bf16 ring_s[kNs * RS::kElems];
+    __shared__ __align__(16) __bf16 ring_f[kNf * RF::kElems];
+    const __bf16* src_s = S::kShiftX ? x : dy;
+    const __bf16* src_f = S::kShiftX ? dy : x;
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = lane & 15, kg = lane >> 4;
+    const int gs = wave % S::kGs, gk = wave / S::kGs;
+    const int lp = 4 * kg + (i >> 2), lc = 4 * (i & 3);  // this lane's pixel and channels in a read
+
+    f32x4 acc[9][kTF];
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+        for (int t = 0; t < kTF; ++t) acc[tap][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    constexpr int kTB = S::kShiftX ? kTF : 1;  // output-channel tiles of the bias
+    f32x4 accb[kTB];
+#pragma unroll
+    for (int t = 0; t < kTB; ++t) accb[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    bf16x8 ones;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) ones[e] = (__bf16)1.0f;
+
+    const int strips = (W + kCols - 1) / kCols, segs_y = (H + seg_rows - 1) / seg_rows;
+    const int64_t segs = (int64_t)images * strips * segs_y;
+    for (int64_t sg = blockIdx.x; sg < segs; sg += gridDim.x) {
+        const int n = (int)(sg / ((int64_t)strips * segs_y));
+        const int rem = (int)(sg % ((int64_t)strips * segs_y));
+        const int ya = (rem / strips) * seg_rows, x0 = (rem % strips) * kCols;
+        const int ye = ya + seg_rows < H ? ya + seg_rows : H;
+        const int steps = (ye - ya + P - 1) / P;
+        const int64_t img = (int64_t)n * H * W;
+        // every wave is done with the rings
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_s_barrier();
+        lds_fence();
+        // shifted row ya - 1 + j in slot j % kNs; row ya + j of the other side in slot j % kNf.
+        // Step k's rows: shifted ya + P k + 1 .. + P, the other side ya + P k .. + P - 1.
+        auto issue = [&](int k) {
+#pragma unroll
+            for (int r = 0; r < P; ++r) {
+                const int js = P * k + r + 2, jf = P * k + r;
+                stage_row<S::kCS, 1, kPS>(src_s, img, ya - 1 + js, x0, H, W, ring_s + (js % kNs) * RS::kElems);
+                stage_row<S::kCF, 0, kPF>(src_f, img, ya + jf, x0, H, W, ring_f + (jf % kNf) * RF::kElems);
+            }
+        };
+        stage_row<S::kCS, 1, kPS>(src_s, img, ya - 1, x0, H, W, ring_s);
+        stage_row<S::kCS, 1, kPS>(src_s, img, ya, x0, H, W, ring_s + RS::kElems);
+        for (int k = 0; k < D && k < steps; ++k) issue(k);
+        for (int k = 0; k < steps; ++k) {
+            // this step's rows have landed in every wave, and every wave is done with the slots
+            // the step D on takes (the previous step's)
+            const int after = steps - 1 - k < D - 1 ? steps - 1 - k : D - 1;
+            wait_vmcnt_at_most(after * kUnitLoads);
+            __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+            __builtin_amdgcn_s_barrier();
+            lds_fence();
+            if (k + D < steps) issue(k + D);
+#pragma unroll
+            for (int m = 0; m < 2 * P / S::kGk; ++m) {
+                const int ks = gk + S::kGk * m;  // K step: row r of the step, columns c0 .. c0 + 31
+                const int r = ks >> 1, c0 = 32 * (ks & 1);
+                const int j = P * k + r;  // the output row ya + j
+                if (ya + j >= ye) continue;
+                const __bf16* fr = ring_f + (j % kNf) * RF::kElems;
+                bf16x8 fo[kTF];
+#pragma unroll
+                for (int t = 0; t < kTF; ++t) fo[t] = read_pixels(fr + (c0 + lp) * kSF + 16 * t + lc, kSF);
+                if (S::kShiftX && gs == 0) {
+#pragma unroll
+                    for (int t = 0; t < kTB; ++t) accb[t] = mfma(fo[t], ones, accb[t]);
+                }
+#pragma unroll
+                for (int tr = 0; tr < 3; ++tr) {
+                    // the shifted side's row for tap row tr (its ring index: row + 1 - ya)
+                    const int js = S::kShiftX ? j + tr : j + 2 - tr;
+                    const __bf16* sr = ring_s + (js % kNs) * RS::kElems;
+#pragma unroll
+                    for (int dxi = 0; dxi < 3; ++dxi) {
+                        const int sc = S::kShiftX ? c0 + dxi : c0 + 2 - dxi;  // ring column: image x0 - 1 + sc
+                        const bf16x8 so = read_pixels(sr + (sc + lp) * kSS + 16 * gs + lc, kSS);
+                        const int tap = 3 * tr + dxi;
+#pragma unroll
+                        for (int t = 0; t < kTF; ++t)
+                            acc[tap][t] = S::kShiftX ? mfma(fo[t], so, acc[tap][t]) : mfma(so, fo[t], acc[tap][t]);
+                        if (!S::kShiftX && tap == 4) accb[0] = mfma(so, ones, accb[0]);
+                    }
+                }
+            }
+        }
+    }
+    // lane holds rows 4 kg + q, column i of each tile: with the input shifted, rows are output
+    // channels 16 t + 4 kg + q and the column input channel 16 gs + i; otherwise rows are output
+    // channels 4 kg + q and the column input channel 16 t + i
+    float* out = part + ((int64_t)blockIdx.x * S::kGk + gk) * S::kWords;
+#pragma unroll
+    for (int t = 0; t < kTF; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int co = S::kShiftX ? 16 * t + 4 * kg + q : 4 * kg + q;
+            const int ci = S::kShiftX ? 16 * gs + i : 16 * t + i;
+#pragma unroll
+            for (int tap = 0; tap < 9; ++tap) out[((int64_t)co * 9 + tap) * kCin + ci] = acc[tap][t][q];
+        }
+    if (gs == 0 && i == 0) {
+#pragma unroll
+        for (int t = 0; t < kTB; ++t)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) out[9 * kCin * kCout + 16 * t + 4 * kg + q] = accb[t][q];
+    }
+}
+
+// out = sum over the parts of part[p][e] for each of the n words of a partial (the first nw are
+// dw, the rest db): four interleaved groups of parts, each summed in order, then the groups in
+// order, so the result does not depend on the run.
+__global__ __launch_bounds__(kBlock) void wgrad_reduce_kernel(const float* __restrict__ part, int parts, int nw,
+                                                              int n, float* __restrict__ dw,
+                                                              float* __restrict__ db) {
+    __shared__ float s[4][64];
+    const int t = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int64_t e = (int64_t)blockIdx.x * 64 + t;
+    float sum = 0.0f;
+    if (e < n) {
+#pragma unroll 8
+        for (int p = g; p < parts; p += 4) sum += part[(int64_t)p * n + e];
+    }
+    s[g][t] = sum;
+    __syncthreads();
+    if (g == 0 && e < n) {
+        const float v = ((s[0][t] + s[1][t]) + s[2][t]) + s[3][t];
+        if (e < nw) dw[e] = v;
+        else db[e - nw] = v;
     }
 }
 
@@ -727,6 +885,50 @@ static int launch_conv(hipStream_t s, int64_t tiles, const __bf16* x, int images
     return 0;
 }
 
+// Blocks of a weight-gradient launch: the device's CU count times what a CU holds, taken at the
+// first call; the partials and their summation order then depend only on the image size.
+template <int kCin, int kCout>
+static int wgrad_blocks() {
+    static int blocks = 0;
+    if (!blocks) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            return 0;
+        blocks = cus * WgradShape<kCin, kCout>::kBlocksPerCu;
+    }
+    return blocks;
+}
+
+template <int kCin, int kCout>
+static int64_t wgrad_words() {
+    const int blocks = wgrad_blocks<kCin, kCout>();
+    return blocks ? (int64_t)blocks * WgradShape<kCin, kCout>::kGk * WgradShape<kCin, kCout>::kWords : -1;
+}
+
+// The weight gradient: its blocks' partials into `part`, then their sum. Segments of rows about
+// as many as the launch has blocks (a whole number of steps each).
+template <int kCin, int kCout>
+static int launch_wgrad(hipStream_t s, const __bf16* x, const __bf16* dy, int images, int H, int W, float* part,
+                        float* dw, float* db) {
+    typedef WgradShape<kCin, kCout> S;
+    const int blocks = wgrad_blocks<kCin, kCout>();
+    if (!blocks) return sfail("wgrad: cannot size the launch");
+    const int64_t strip_rows = (int64_t)images * ((W + kCols - 1) / kCols) * H;
+    int64_t seg = (strip_rows + blocks - 1) / blocks;
+    seg = (seg + S::kRows - 1) / S::kRows * S::kRows;
+    if (seg > H) seg = (H + S::kRows - 1) / S::kRows * S::kRows;
+    const int64_t segs = (int64_t)images * ((W + kCols - 1) / kCols) * ((H + seg - 1) / seg);
+    const int grid = (int)(segs < blocks ? segs : blocks);
+    hipLaunchKernelGGL((conv3x3_wgrad_kernel<kCin, kCout>), dim3(grid), dim3(kBlock), 0, s, x, dy, images, H, W,
+                       (int)seg, part);
+    if (int rc = serr("wgrad")) return rc;
+    const int nw = 9 * kCin * kCout;
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((S::kWords + 63) / 64), dim3(kBlock), 0, s, part, grid * S::kGk,
+                       nw, S::kWords, dw, db);
+    return serr("wgrad reduce");
+}
+
 extern "C" {
 
 const char* vxs_last_error(void) { return g_serr; }
@@ -774,37 +976,28 @@ int vxs_conv3x3(void* stream, const void* in, int images, int H, int W, int cin,
     return serr("conv3x3");
 }
 
-// Scratch words the weight gradient needs for `pixels` pixels: partial sums per chunk.
-int64_t vxs_wgrad_scratch(int64_t pixels, int cin, int cout, int64_t chunk) {
-    const int64_t chunks = (pixels + chunk - 1) / chunk;
-    return chunks * 9 * cin * cout + chunks * cout;
+// Scratch words the weight gradient needs: a partial per block and wave group.
+int64_t vxs_wgrad_scratch(int cin, int cout) {
+    if (cin == 16 && cout == 64) return wgrad_words<16, 64>();
+    if (cin == 64 && cout == 64) return wgrad_words<64, 64>();
+    if (cin == 64 && cout == 16) return wgrad_words<64, 16>();
+    if (cin == 16 && cout == 16) return wgrad_words<16, 16>();
+    return -1;
 }
 
 // dw[co][tap][ci] and db[co] of a 3x3 convolution from its input x and output gradient dy (both
 // NHWC bf16 over `images` images of H x W), reduced in a fixed order.
 int vxs_conv3x3_wgrad(void* stream, const void* x, const void* dy, int images, int H, int W, int cin, int cout,
-                      int64_t chunk, float* scratch, float* dw, float* db) {
-    if (!x || !dy || !scratch || !dw || !db || chunk < 32 || chunk % 32) return sfail("wgrad: bad argument");
-    const int64_t pixels = (int64_t)images * H * W;
-    const int64_t chunks = (pixels + chunk - 1) / chunk;
-    float* part = scratch;
-    float* part_b = scratch + chunks * 9 * cin * cout;
+                      float* scratch, float* dw, float* db) {
+    if (!x || !dy || !scratch || !dw || !db || images < 1 || H < 1 || W < 1) return sfail("wgrad: bad argument");
     hipStream_t s = (hipStream_t)stream;
-    const dim3 grid((unsigned)chunks, 9);
     const __bf16* a = (const __bf16*)x;
     const __bf16* g = (const __bf16*)dy;
-    if (cin == 16 && cout == 64)
-        hipLaunchKernelGGL((conv3x3_wgrad_kernel<16, 64>), grid, dim3(kBlock), 0, s, a, g, H, W, pixels, chunk, part, part_b);
-    else if (cin == 64 && cout == 64)
-        hipLaunchKernelGGL((conv3x3_wgrad_kernel<64, 64>), grid, dim3(kBlock), 0, s, a, g, H, W, pixels, chunk, part, part_b);
-    else if (cin == 64 && cout == 16)
-        hipLaunchKernelGGL((conv3x3_wgrad_kernel<64, 16>), grid, dim3(kBlock), 0, s, a, g, H, W, pixels, chunk, part, part_b);
-    else
-        return sfail("wgrad: channels (cin, cout) must be (16, 64), (64, 64) or (64, 16)");
-    if (int rc = serr("wgrad")) return rc;
-    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(blocks_for((int64_t)9 * cin * cout)), dim3(kBlock), 0, s, part,
-                       part_b, (int)chunks, cout, cin, dw, db);
-    return serr("wgrad reduce");
+    if (cin == 16 && cout == 64) return launch_wgrad<16, 64>(s, a, g, images, H, W, scratch, dw, db);
+    if (cin == 64 && cout == 64) return launch_wgrad<64, 64>(s, a, g, images, H, W, scratch, dw, db);
+    if (cin == 64 && cout == 16) return launch_wgrad<64, 16>(s, a, g, images, H, W, scratch, dw, db);
+    if (cin == 16 && cout == 16) return launch_wgrad<16, 16>(s, a, g, images, H, W, scratch, dw, db);
+    return sfail("wgrad: channels (cin, cout) must be 16 or 64 each");
 }
 
 int vxs_flip(void* stream, const void* w, int cout, int cin, void* wt) {

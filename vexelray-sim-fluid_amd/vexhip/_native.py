@@ -88,9 +88,9 @@ def lib() -> ctypes.CDLL:
             "vxs_last_error": (cs, []),
             "vxs_conv3x3": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp,
                                            ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int]),
-            "vxs_wgrad_scratch": (i64, [i64, ctypes.c_int, ctypes.c_int, i64]),
+            "vxs_wgrad_scratch": (i64, [ctypes.c_int, ctypes.c_int]),
             "vxs_conv3x3_wgrad": (ctypes.c_int, [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                                 ctypes.c_int, i64, vp, vp, vp]),
+                                                 ctypes.c_int, vp, vp, vp]),
             "vxs_flip": (ctypes.c_int, [vp, vp, ctypes.c_int, ctypes.c_int, vp]),
             "vxs_adam": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, i64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                         ctypes.c_float, ctypes.c_int]),

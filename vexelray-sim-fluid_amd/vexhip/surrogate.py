@@ -33,7 +33,6 @@ COURANT = 0.3        # of the fixed dt against the fastest wave the scenes reach
 FASTEST = 6.0        # m/s: sqrt(g * 2.6) plus a current of about 1 m/s
 TOLERANCE = 1e-3     # the fallback rule's relative volume drift
 BLOCKS = 16          # blocks of STEPS steps a training trajectory runs
-WGRAD_CHUNK = 4096   # pixels a weight-gradient block sums
 
 _torch = None
 
@@ -168,7 +167,7 @@ class Surrogate:
         self.dw = [torch.empty_like(w) for w in self.w32]
         self.db = [torch.empty_like(b) for b in self.b32]
         self.wt = [torch.empty_like(w) for w in self.w16]
-        words = max(_native.lib().vxs_wgrad_scratch(px, cin, cout, WGRAD_CHUNK) for cin, cout in LAYERS)
+        words = max(_native.lib().vxs_wgrad_scratch(cin, cout) for cin, cout in LAYERS)
         self.scratch = torch.empty(words, dtype=torch.float32, device="cuda")
         self.loss_acc = torch.zeros(1, dtype=torch.float32, device="cuda")
         self.grads = True
@@ -258,8 +257,7 @@ class Surrogate:
             cin, cout = LAYERS[li]
             x = self.x0 if li == 0 else self.acts[li - 1]
             _native.vxs_check(lib.vxs_conv3x3_wgrad(st, _p(x), _p(g), self.images, self.n, self.n, cin, cout,
-                                                    WGRAD_CHUNK, _p(self.scratch), _p(self.dw[li]),
-                                                    _p(self.db[li])))
+                                                    _p(self.scratch), _p(self.dw[li]), _p(self.db[li])))
             if li > 0:
                 _native.vxs_check(lib.vxs_flip(st, _p(self.w16[li]), cout, cin, _p(self.wt[li])))
                 dst = self.g64[li % 2]
