@@ -281,7 +281,7 @@ def bench_surrogate(iters, n=1024):
             "hll_fp32_cell_updates_per_s": round(cells / (hll32_ms * 1e-3), 1),
             "hll_fp64_ms_per_8_steps": round(hll64_ms, 4),
             "surrogate_speed_vs_hll_fp32": round(hll32_ms / fwd_ms, 4),
-            "roofline": {"kernel": "conv3x3_lds 64->64", "bound": "mfma",
+            "roofline": {"kernel": "conv3x3_rows 64->64", "bound": "mfma",
                          "achieved": round(conv_flop / (conv_ms * 1e-3) / 1e12, 1), "peak": 2500.0,
                          "unit": "TFLOP/s", "frac": round(conv_flop / (conv_ms * 1e-3) / 1e12 / 2500.0, 4),
                          "avg_us": round(conv_ms * 1e3, 2)},

@@ -122,6 +122,13 @@ __device__ __forceinline__ void wait_vmcnt() {
     __builtin_amdgcn_s_waitcnt((n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8));
 }
 
+// s_waitcnt with lgkmcnt = n and the other counters not waited on.
+template <int n>
+__device__ __forceinline__ void wait_lgkmcnt() {
+    static_assert(n >= 0 && n < 16, "lgkmcnt holds 4 bits");
+    __builtin_amdgcn_s_waitcnt(15 | (3 << 14) | (7 << 4) | (n << 8));
+}
+
 // The compiler does not see the LDS stores of a global-to-LDS load: LDS reads after the wait
 // must not be moved above it or served from values it assumed.
 __device__ __forceinline__ void lds_fence() { asm volatile("" ::: "memory"); }
@@ -370,8 +377,8 @@ __global__ __launch_bounds__(kBlock, 2) void conv3x3_lds_kernel(const __bf16* __
 // input) is read once a K step of 32 pixels, all its channels (kTF 16-channel tiles); the other,
 // the "shifted" side, once a tap, 16 channels a wave: the side with fewer channels, so that the
 // nine-fold reads fall on the narrower one. A block walks a strip of kCols columns down a segment
-// of rows, kRows output rows a step; its four waves split the shifted side's channel tiles (kGs)
-// and a step's K steps (kGk). kDepth steps' rows are in flight ahead of the one computing.
+// of rows, kRows output rows a step; its waves split the shifted side's channel tiles (kGs) and a
+// step's K steps (kGk). kDepth steps' rows are in flight ahead of the one computing.
 template <int kCin, int kCout>
 struct WgradShape {
     static constexpr bool kShiftX = kCin <= kCout;      // the input is the shifted side
@@ -379,10 +386,12 @@ struct WgradShape {
     static constexpr int kCS = kShiftX ? kCin : kCout;  // and of the shifted side
     static constexpr int kTF = kCF / 16;
     static constexpr int kGs = kCS / 16;
-    static constexpr int kGk = 4 / kGs;
-    static constexpr int kRows = kGk == 1 ? 1 : 2;     // output rows a step: 2 K steps each
-    static constexpr int kDepth = kGk == 1 ? 4 : 2;
-    static constexpr int kBlocksPerCu = kGk == 1 ? 1 : 2;  // as the rings' LDS allows
+    static constexpr bool kWide = kGs == 4;            // 64 channels each side
+    static constexpr int kGk = kWide ? 2 : 4;
+    static constexpr int kWaves = kGs * kGk;           // 8 (two a SIMD) or 4 (two blocks a CU)
+    static constexpr int kRows = kWide ? 1 : 2;        // output rows a step: 2 K steps each
+    static constexpr int kDepth = kWide ? 4 : 2;
+    static constexpr int kBlocksPerCu = kWide ? 1 : 2;  // as the rings' LDS allows
     static constexpr int kWords = 9 * kCin * kCout + kCout;  // a partial: dw[co][tap][ci], then db[co]
 };
 
@@ -392,32 +401,32 @@ struct WgradShape {
 template <int kC>
 constexpr int wgrad_pad() { return kC == 64 ? 2 : 0; }
 
-// One row of a weight-gradient ring: kCols pixels and kHalo more each side, kC / 8 16-byte chunks
-// and kPad pad chunks a pixel, chunk e at 16 e bytes; load instruction j fills chunks 64 j ..
-// 64 j + 63 and is wave j % 4's. kMinLoads: the instructions a row takes from the wave that
-// issues fewest.
-template <int kC, int kHalo, int kPad>
+// One row of an LDS ring: kCols pixels and kHalo more each side, kC / 8 16-byte chunks and kPad
+// pad chunks a pixel, chunk e at 16 e bytes; load instruction j fills chunks 64 j .. 64 j + 63
+// and is wave j % kWaves's. kMinLoads: the instructions a row takes from the wave that issues
+// fewest.
+template <int kC, int kHalo, int kPad, int kWaves = 4>
 struct RowStage {
     static constexpr int kSlots = kC / 8 + kPad;
     static constexpr int kPix = kCols + 2 * kHalo;
     static constexpr int kChunks = kPix * kSlots;
     static constexpr int kLoads = (kChunks + 63) / 64;
     static constexpr int kElems = kChunks * 8;
-    static constexpr int kMinLoads = kLoads / 4;
+    static constexpr int kMinLoads = kLoads / kWaves;
 };
 
 // Stages row y of the image at pixel offset img (columns x0 - kHalo on) into `buf` with
 // global-to-LDS loads: pixels outside the image are zeros, pad chunks are not loaded.
-template <int kC, int kHalo, int kPad>
+template <int kC, int kHalo, int kPad, int kWaves = 4>
 __device__ __forceinline__ void stage_row(const __bf16* __restrict__ in, int64_t img, int y, int x0, int H, int W,
                                           __bf16* buf) {
-    typedef RowStage<kC, kHalo, kPad> R;
+    typedef RowStage<kC, kHalo, kPad, kWaves> R;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const bool row_in = y >= 0 && y < H;
     const __bf16* base = in + (img + (int64_t)(row_in ? y : 0) * W) * kC;
 #pragma unroll
-    for (int jj = 0; jj < (R::kLoads + 3) / 4; ++jj) {
-        const int j = wave + 4 * jj;
+    for (int jj = 0; jj < (R::kLoads + kWaves - 1) / kWaves; ++jj) {
+        const int j = wave + kWaves * jj;
         const int e = 64 * j + lane;
         const int px = e / R::kSlots, cg = e % R::kSlots;
         if (j < R::kLoads && e < R::kChunks && cg < kC / 8) {
@@ -470,14 +479,16 @@ __device__ __forceinline__ bf16x8 read_pixels(const __bf16* p, int stride) {
 // zero outside the image, dy's ring zero). The bias gradient is one more product, of the output
 // gradient against ones.
 template <int kCin, int kCout>
-__global__ __launch_bounds__(kBlock, (WgradShape<kCin, kCout>::kBlocksPerCu)) void conv3x3_wgrad_kernel(
+__global__ __launch_bounds__((64 * WgradShape<kCin, kCout>::kWaves),
+                             (WgradShape<kCin, kCout>::kBlocksPerCu)) void conv3x3_wgrad_kernel(
     const __bf16* __restrict__ x, const __bf16* __restrict__ dy, int images, int H, int W, int seg_rows,
     float* __restrict__ part) {
     typedef WgradShape<kCin, kCout> S;
     constexpr int kTF = S::kTF, P = S::kRows, D = S::kDepth;
     constexpr int kPF = wgrad_pad<S::kCF>(), kPS = wgrad_pad<S::kCS>();
-    typedef RowStage<S::kCS, 1, kPS> RS;
-    typedef RowStage<S::kCF, 0, kPF> RF;
+    constexpr int kW = S::kWaves;
+    typedef RowStage<S::kCS, 1, kPS, kW> RS;
+    typedef RowStage<S::kCF, 0, kPF, kW> RF;
     constexpr int kSS = RS::kSlots * 8, kSF = RF::kSlots * 8;  // LDS pixel strides
     // ring rows: a step reads P + 2 shifted rows and P others, and D steps' rows are in flight
     constexpr int kNs = P * (D + 1) + 2, kNf = P * (D + 1);
@@ -526,12 +537,12 @@ __global__ __launch_bounds__(kBlock, (WgradShape<kCin, kCout>::kBlocksPerCu)) vo
 #pragma unroll
             for (int r = 0; r < P; ++r) {
                 const int js = P * k + r + 2, jf = P * k + r;
-                stage_row<S::kCS, 1, kPS>(src_s, img, ya - 1 + js, x0, H, W, ring_s + (js % kNs) * RS::kElems);
-                stage_row<S::kCF, 0, kPF>(src_f, img, ya + jf, x0, H, W, ring_f + (jf % kNf) * RF::kElems);
+                stage_row<S::kCS, 1, kPS, kW>(src_s, img, ya - 1 + js, x0, H, W, ring_s + (js % kNs) * RS::kElems);
+                stage_row<S::kCF, 0, kPF, kW>(src_f, img, ya + jf, x0, H, W, ring_f + (jf % kNf) * RF::kElems);
             }
         };
-        stage_row<S::kCS, 1, kPS>(src_s, img, ya - 1, x0, H, W, ring_s);
-        stage_row<S::kCS, 1, kPS>(src_s, img, ya, x0, H, W, ring_s + RS::kElems);
+        stage_row<S::kCS, 1, kPS, kW>(src_s, img, ya - 1, x0, H, W, ring_s);
+        stage_row<S::kCS, 1, kPS, kW>(src_s, img, ya, x0, H, W, ring_s + RS::kElems);
         for (int k = 0; k < D && k < steps; ++k) issue(k);
         for (int k = 0; k < steps; ++k) {
             // this step's rows have landed in every wave, and every wave is done with the slots
@@ -593,6 +604,223 @@ __global__ __launch_bounds__(kBlock, (WgradShape<kCin, kCout>::kBlocksPerCu)) vo
         for (int t = 0; t < kTB; ++t)
 #pragma unroll
             for (int q = 0; q < 4; ++q) out[9 * kCin * kCout + 16 * t + 4 * kg + q] = accb[t][q];
+    }
+}
+
+// The streamed 3x3 convolution: resident blocks walk strips of kCols columns down segments of
+// seg_rows rows, kR output rows a step, with the input rows of the next kDepth steps loading into
+// an LDS ring by global-to-LDS loads (each input row loaded once, with one more pixel each end).
+// Each wave keeps the weights of kNT output-channel tiles in registers for the whole launch and
+// computes kMT 16-pixel column tiles of the step's rows. With 64 input channels a K step (32
+// channels of one tap) lies in one input row, so each pixel operand read from LDS serves every
+// output row of the step that reads that input row (up to 3) and every channel tile; with 16 a K
+// step spans two taps, and a step is one row. Either way each output accumulates over K = 9 taps x
+// cin in the order of conv3x3_lds_kernel (the results are the same bits). With kMasked the mask
+// rows stream through a second ring and the epilogue reads them from LDS. Every wave issues the
+// same stores every step (pixels and rows past the image edge store to kSink), so the waits can
+// count the loads still in flight behind them.
+template <int kCin, int kCoTiles, bool kMasked>
+struct ConvRows {
+    static constexpr int kCo = 16 * kCoTiles;
+    static constexpr bool kWide = kCin == 64 && kCoTiles == 4;
+    static constexpr int kWaves = kWide ? 8 : 4;
+    static constexpr int kNT = kWide ? 2 : kCoTiles;
+    static constexpr int kGroupsN = kCoTiles / kNT;           // waves over the channel tiles
+    static constexpr int kMT = 4 * kGroupsN / kWaves;         // column tiles a wave computes
+    static constexpr int kR = kCin == 64 ? ((kWide && !kMasked) ? 4 : 2) : 1;
+    static constexpr int kDepth = kCin == 64 ? 2 : (kMasked ? 5 : 8);
+    // two waves a SIMD (so one's epilogue, waits and barriers overlap the other's products): eight
+    // waves a block, or two blocks a CU where the rings fit twice in LDS
+    static constexpr int kBlocksPerCu = (kWide || (kCin == 64 && kMasked)) ? 1 : 2;
+};
+
+template <int kCin, int kCoTiles, bool kMasked>
+__global__ __launch_bounds__((64 * ConvRows<kCin, kCoTiles, kMasked>::kWaves),
+                             (ConvRows<kCin, kCoTiles, kMasked>::kBlocksPerCu)) void conv3x3_rows_kernel(const __bf16* __restrict__ in, int images, int H,
+                                                                 int W, int seg_rows, const __bf16* __restrict__ w,
+                                                                 const float* __restrict__ bias, void* out,
+                                                                 const __bf16* __restrict__ mask, int flags) {
+    typedef ConvRows<kCin, kCoTiles, kMasked> C;
+    constexpr int D = C::kDepth, R = C::kR, kNT = C::kNT, kMT = C::kMT;
+    constexpr int kW = C::kWaves;
+    typedef RowStage<kCin, 1, 1, kW> RX;  // input rows: the pad chunk spaces 16 pixels over the banks
+    typedef RowStage<C::kCo, 0, 0, kW> RM;  // mask rows
+    constexpr int kStride = RX::kSlots * 8;
+    constexpr int kNx = R * (D + 1) + 2, kNm = R * (D + 1);
+    constexpr int kK = 9 * kCin;
+    constexpr int kSteps = (kK + 31) / 32;
+    constexpr int kStepLoads = R * (RX::kMinLoads + (kMasked ? RM::kMinLoads : 0));  // fewest of any wave
+    constexpr int kStepStores = R * kMT * kNT;
+    __shared__ __align__(16) __bf16 ring_x[kNx * RX::kElems];
+    __shared__ __align__(16) __bf16 ring_m[kMasked ? kNm * RM::kElems : 8];
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = lane & 15, kg = lane >> 4;
+    const int wn = wave % C::kGroupsN, wm = wave / C::kGroupsN;
+    const bool relu = flags & kRelu, f32 = flags & kOutF32;
+
+    // this wave's weights: B[k][n] for its channel tiles, every K step
+    bf16x8 b[kNT][kSteps];
+#pragma unroll
+    for (int t = 0; t < kNT; ++t) {
+        const int co = 16 * (wn * kNT + t) + i;
+#pragma unroll
+        for (int s = 0; s < kSteps; ++s) {
+            const int k = 32 * s + 8 * kg;
+            const int tap = k / kCin, ci = k % kCin;
+            b[t][s] = tap < 9 ? *(const bf16x8*)(w + ((int64_t)co * 9 + tap) * kCin + ci) : zero8();
+        }
+    }
+    float bias_r[kNT][4];
+#pragma unroll
+    for (int t = 0; t < kNT; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bias_r[t][q] = bias ? bias[16 * (wn * kNT + t) + 4 * kg + q] : 0.0f;
+
+    const int strips = (W + kCols - 1) / kCols, segs_y = (H + seg_rows - 1) / seg_rows;
+    const int64_t segs = (int64_t)images * strips * segs_y;
+    for (int64_t sg = blockIdx.x; sg < segs; sg += gridDim.x) {
+        const int n = (int)(sg / ((int64_t)strips * segs_y));
+        const int rem = (int)(sg % ((int64_t)strips * segs_y));
+        const int ya = (rem / strips) * seg_rows, x0 = (rem % strips) * kCols;
+        const int ye = ya + seg_rows < H ? ya + seg_rows : H;
+        const int steps = (ye - ya + R - 1) / R;
+        const int64_t img = (int64_t)n * H * W;
+        // every load and store of this wave is done (the weights included), and every wave is done
+        // with the rings
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_s_barrier();
+        lds_fence();
+        // input row ya - 1 + j in slot j % kNx; mask row ya + j in slot j % kNm. Step k's rows:
+        // input ya + R k + 1 .. + R, mask ya + R k .. + R - 1.
+        auto issue = [&](int k) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int jx = R * k + r + 2, jm = R * k + r;
+                stage_row<kCin, 1, 1, kW>(in, img, ya - 1 + jx, x0, H, W, ring_x + (jx % kNx) * RX::kElems);
+                if (kMasked)
+                    stage_row<C::kCo, 0, 0, kW>(mask, img, ya + jm, x0, H, W, ring_m + (jm % kNm) * RM::kElems);
+            }
+        };
+        stage_row<kCin, 1, 1, kW>(in, img, ya - 1, x0, H, W, ring_x);
+        stage_row<kCin, 1, 1, kW>(in, img, ya, x0, H, W, ring_x + RX::kElems);
+        for (int k = 0; k < D && k < steps; ++k) issue(k);
+        for (int k = 0; k < steps; ++k) {
+            // this step's rows have landed in every wave: behind its loads this wave issued the
+            // loads of the steps after it and the stores of the steps before
+            const int loads_after = steps - 1 - k < D - 1 ? steps - 1 - k : D - 1;
+            const int stores_after = k < D - 1 ? k : D - 1;
+            wait_vmcnt_at_most(loads_after * kStepLoads + stores_after * kStepStores);
+            __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+            __builtin_amdgcn_s_barrier();
+            lds_fence();
+            f32x4 acc[R][kMT][kNT];
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int m = 0; m < kMT; ++m)
+#pragma unroll
+                    for (int t = 0; t < kNT; ++t) acc[r][m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+            if constexpr (kCin == 64) {
+                // group g = (input row rho of the step, ring index R k + rho; tap column dx): its
+                // operands serve output rows r = rho - dy. Each group's LDS reads are issued before
+                // the previous group's products, so their latency hides behind them.
+                constexpr int kGroups = (R + 2) * 3;
+                bf16x8 ab[2][2][kMT];
+                auto load_group = [&](int g, bf16x8(&dst)[2][kMT]) {
+                    const int rho = g / 3, dx = g % 3;
+                    const __bf16* row = ring_x + ((R * k + rho) % kNx) * RX::kElems;
+#pragma unroll
+                    for (int ch = 0; ch < 2; ++ch)
+#pragma unroll
+                        for (int m = 0; m < kMT; ++m)
+                            dst[ch][m] =
+                                *(const bf16x8*)&row[(16 * (wm * kMT + m) + i + dx) * kStride + 32 * ch + 8 * kg];
+                };
+                load_group(0, ab[0]);
+#pragma unroll
+                for (int g = 0; g < kGroups; ++g) {
+                    // this group's reads (issued before the previous group's products) have landed
+                    // before the next group's are issued, so no wait is left for the products
+                    wait_lgkmcnt<0>();
+                    __builtin_amdgcn_sched_barrier(0);
+                    if (g + 1 < kGroups) load_group(g + 1, ab[(g + 1) & 1]);
+                    __builtin_amdgcn_sched_barrier(0);
+                    const int rho = g / 3, dx = g % 3;
+#pragma unroll
+                    for (int ch = 0; ch < 2; ++ch)
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            const int dy = rho - r;
+                            if (dy < 0 || dy > 2) continue;
+                            const int s = (3 * dy + dx) * 2 + ch;
+#pragma unroll
+                            for (int m = 0; m < kMT; ++m)
+#pragma unroll
+                                for (int t = 0; t < kNT; ++t)
+                                    acc[r][m][t] = mfma(b[t][s], ab[g & 1][ch][m], acc[r][m][t]);
+                        }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < kSteps; ++s) {
+                    const int k32 = 32 * s + 8 * kg;
+                    const int tap = k32 / kCin, ci = k32 % kCin;
+                    const int dy = tap / 3, dx = tap % 3;
+#pragma unroll
+                    for (int m = 0; m < kMT; ++m) {
+                        const bf16x8 a = tap < 9 ? *(const bf16x8*)&ring_x[((R * k + dy) % kNx) * RX::kElems +
+                                                                           (16 * (wm * kMT + m) + i + dx) * kStride + ci]
+                                                 : zero8();
+#pragma unroll
+                        for (int t = 0; t < kNT; ++t) acc[0][m][t] = mfma(b[t][s], a, acc[0][m][t]);
+                    }
+                }
+            }
+            // lane holds channel 16 (wn kNT + t) + 4 kg + q of pixel (ya + R k + r, x0 + 16 (wm kMT + m) + i)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int yy = ya + R * k + r;
+#pragma unroll
+                for (int m = 0; m < kMT; ++m) {
+                    const int xx = x0 + 16 * (wm * kMT + m) + i;
+                    const bool inside = yy < ye && xx < W;
+                    const int64_t at = (img + (int64_t)yy * W + xx) * C::kCo;
+#pragma unroll
+                    for (int t = 0; t < kNT; ++t) {
+                        const int co = 16 * (wn * kNT + t) + 4 * kg;
+                        float v[4];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) v[q] = acc[r][m][t][q] + bias_r[t][q];
+                        if (kMasked) {
+                            const bf16x4 mk = *(const bf16x4*)&ring_m[((R * k + r) % kNm) * RM::kElems +
+                                                                       (16 * (wm * kMT + m) + i) * C::kCo + co];
+#pragma unroll
+                            for (int q = 0; q < 4; ++q)
+                                if (!((float)mk[q] > 0.0f)) v[q] = 0.0f;
+                        }
+                        if (relu) {
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) v[q] = v[q] > 0.0f ? v[q] : 0.0f;
+                        }
+                        if (f32) {
+                            float* o = inside ? (float*)out + at + co : (float*)kSink + 4 * (lane & 15);
+                            *(float4*)o = make_float4(v[0], v[1], v[2], v[3]);
+                        } else {
+                            bf16x4 rr;
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) rr[q] = (__bf16)v[q];
+                            __bf16* o = inside ? (__bf16*)out + at + co : kSink + 4 * lane;
+                            *(bf16x4*)o = rr;
+                        }
+                    }
+                }
+            }
+            // the rows D steps on, into the slots the previous step read (every wave has passed
+            // this step's barrier, so is done with them)
+            if (k + D < steps) issue(k + D);
+        }
     }
 }
 
@@ -920,13 +1148,50 @@ static int launch_wgrad(hipStream_t s, const __bf16* x, const __bf16* dy, int im
     if (seg > H) seg = (H + S::kRows - 1) / S::kRows * S::kRows;
     const int64_t segs = (int64_t)images * ((W + kCols - 1) / kCols) * ((H + seg - 1) / seg);
     const int grid = (int)(segs < blocks ? segs : blocks);
-    hipLaunchKernelGGL((conv3x3_wgrad_kernel<kCin, kCout>), dim3(grid), dim3(kBlock), 0, s, x, dy, images, H, W,
+    hipLaunchKernelGGL((conv3x3_wgrad_kernel<kCin, kCout>), dim3(grid), dim3(64 * S::kWaves), 0, s, x, dy, images, H, W,
                        (int)seg, part);
     if (int rc = serr("wgrad")) return rc;
     const int nw = 9 * kCin * kCout;
     hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((S::kWords + 63) / 64), dim3(kBlock), 0, s, part, grid * S::kGk,
                        nw, S::kWords, dw, db);
     return serr("wgrad reduce");
+}
+
+// The streamed convolution over resident blocks (as many as the device holds at once), in
+// segments of rows about as many as there are blocks.
+template <int kCin, int kCoTiles, bool kMasked>
+static int launch_conv_rows(hipStream_t s, const __bf16* x, int images, int H, int W, const __bf16* k,
+                            const float* bias, void* out, const __bf16* mk, int f) {
+    static int resident = 0;
+    if (!resident) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, conv3x3_rows_kernel<kCin, kCoTiles, kMasked>,
+                                                         64 * ConvRows<kCin, kCoTiles, kMasked>::kWaves,
+                                                         0) != hipSuccess ||
+            cus < 1 || per_cu < 1)
+            return 1;
+        resident = cus * per_cu;
+    }
+    constexpr int R = ConvRows<kCin, kCoTiles, kMasked>::kR;
+    const int64_t strips = (W + kCols - 1) / kCols;
+    int64_t seg = (images * strips * H + resident - 1) / resident;
+    seg = (seg + R - 1) / R * R;
+    if (seg > H) seg = (H + R - 1) / R * R;
+    const int64_t segs = images * strips * ((H + seg - 1) / seg);
+    const unsigned g = (unsigned)(segs < resident ? segs : resident);
+    hipLaunchKernelGGL((conv3x3_rows_kernel<kCin, kCoTiles, kMasked>), dim3(g),
+                       dim3(64 * ConvRows<kCin, kCoTiles, kMasked>::kWaves), 0, s, x, images, H, W,
+                       (int)seg, k, bias, out, mk, f);
+    return 0;
+}
+
+template <int kCin, int kCoTiles>
+static int launch_conv_rows(hipStream_t s, const __bf16* x, int images, int H, int W, const __bf16* k,
+                            const float* bias, void* out, const __bf16* mk, int f) {
+    return (f & kMask) ? launch_conv_rows<kCin, kCoTiles, true>(s, x, images, H, W, k, bias, out, mk, f)
+                       : launch_conv_rows<kCin, kCoTiles, false>(s, x, images, H, W, k, bias, out, mk, f);
 }
 
 extern "C" {
@@ -944,6 +1209,22 @@ int vxs_conv3x3(void* stream, const void* in, int images, int H, int W, int cin,
     const __bf16* x = (const __bf16*)in;
     const __bf16* k = (const __bf16*)w;
     const __bf16* mk = (const __bf16*)mask;
+    if (!(flags & (16 | kAccumulate)) && out_ch == cout) {  // the streamed kernel
+        const int f = flags & 15;
+        int rc = 1;
+        if (cin == 16 && cout == 64)
+            rc = launch_conv_rows<16, 4>(s, x, images, H, W, k, bias, out, mk, f);
+        else if (cin == 64 && cout == 64)
+            rc = launch_conv_rows<64, 4>(s, x, images, H, W, k, bias, out, mk, f);
+        else if (cin == 64 && cout == 16)
+            rc = launch_conv_rows<64, 1>(s, x, images, H, W, k, bias, out, mk, f);
+        else if (cin == 16 && cout == 16)
+            rc = launch_conv_rows<16, 1>(s, x, images, H, W, k, bias, out, mk, f);
+        else
+            return sfail("conv3x3: channels (cin, cout) must be 16 or 64 each");
+        if (rc) return sfail("conv3x3: cannot size the launch");
+        return serr("conv3x3");
+    }
     if (!(flags & 16)) {  // the staged kernel, resident blocks over 2 x 64 tiles
         const int64_t tiles = (int64_t)images * ((H + kRows - 1) / kRows) * ((W + kCols - 1) / kCols);
         const int f = flags & 15;
