@@ -77,7 +77,7 @@ def kernel_table(sim, names, units, bytes_override=None):
     return out
 
 
-def bench_mpm3(steps, warmup, p2g, g2p, run, tension=False, lod_level=None):
+def bench_mpm3(steps, warmup, p2g, g2p, run, tension=False, lod_level=None, p2g_ints=0):
     """The 3D substep on the two-cubes scene; with ``tension`` Flip3Step's tension step (six
     blurs, the stress and the grid with the capillary force), sigma = 0.01 B 8, the scale
     Tension3Test gives a drop of radius 8 nodes. With ``lod_level`` every cell's group of 8 is
@@ -93,6 +93,7 @@ def bench_mpm3(steps, warmup, p2g, g2p, run, tension=False, lod_level=None):
     sim.sim.set_option("p2g", p2g)
     sim.sim.set_option("g2p", g2p)
     sim.sim.set_option("run", run)
+    sim.sim.set_option("p2g_ints", p2g_ints)
     z = np.zeros(count, np.float32)
     sim.load((s["x"], s["y"], s["z"]), (z, z, z), s["m"], words)
     if lod_level is not None:
@@ -504,6 +505,7 @@ def main():
     ap.add_argument("--p2g", type=int, default=1)
     ap.add_argument("--g2p", type=int, default=2)
     ap.add_argument("--run", type=int, default=4)
+    ap.add_argument("--p2g-ints", type=int, default=0)
     ap.add_argument("--hll-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hll", action="store_true")
@@ -537,7 +539,8 @@ def main():
 
     print(f"[bench] 3D MPM: 8,388,608 particles, 256^3, {args.warmup} warmup + {args.steps} steps", file=sys.stderr,
           flush=True)
-    count, secs, table, drift = bench_mpm3(args.steps, args.warmup, args.p2g, args.g2p, args.run)
+    count, secs, table, drift = bench_mpm3(args.steps, args.warmup, args.p2g, args.g2p, args.run,
+                                           p2g_ints=args.p2g_ints)
     value = count * args.steps / secs
     dominant = max(table.items(), key=lambda kv: kv[1]["total_ms"])
     dk, dv = dominant

@@ -203,6 +203,11 @@ int vxr_stream(vxr_sim* sim, void** stream);
  *                contracted axis by axis in fused multiply-adds (default, within a few ulps)
  *   "run"        MPM: consecutive particles a thread takes in the segmented passes: 1, 2 or 4
  *                (default 4)
+ *   "p2g_ints"   MPM3 with p2g 1: 1 = each wave's LDS node tile sums in 32-bit integers, in
+ *                power-of-two units the wave chooses from bounds on its particles' amounts (mass
+ *                and momentum apart), each run of particles rounded once to the nearest unit and
+ *                added by LDS integer atomics; 0 = float tiles added by read-add-write rounds.
+ *                Both within FP32 reassociation of the reference's sums (default 0)
  *   "radix_bits" sort digit width (4..8, default 8)
  *   "rows"       SWE: rows a wave marches per tile (1..256, default 32)
  *   "hll_fast"   SWE: 1 = the step with one refined hardware reciprocal per face and per cell in

@@ -50,11 +50,13 @@ class GpuSwe:
 class GpuRig:
     """mpm_cases' rig over a vxr MPM handle."""
 
-    def __init__(self, kind, dims, count, p2g=1, g2p=1, run=4, tension=False):
+    def __init__(self, kind, dims, count, p2g=1, g2p=1, run=4, tension=False, **options):
         self.sim = _native.Sim(kind, dims, count, _native.FLAG_TENSION if tension else 0)
         self.sim.set_option("p2g", p2g)
         self.sim.set_option("g2p", g2p)
         self.sim.set_option("run", run)
+        for key, value in options.items():
+            self.sim.set_option(key, value)
 
     def write(self, name, values):
         self.sim.write(name, np.asarray(values, np.float32))

@@ -14,6 +14,13 @@ pytestmark = pytest.mark.gpu
 SCHEDULES = [dict(p2g=0, g2p=0), dict(p2g=1, g2p=1, run=4), dict(p2g=1, g2p=2, run=4), dict(p2g=1, g2p=2, run=1),
              dict(p2g=1, g2p=2, run=2)]
 IDS = ["atomic", "seg-exact", "seg-fast", "seg-run1", "seg-run2"]
+# The segmented schedule with integer wave tiles (option p2g_ints): runs rounded to each wave's
+# power-of-two units and added by LDS integer atomics.
+INTS = [dict(p2g=1, g2p=2, run=4, p2g_ints=1), dict(p2g=1, g2p=2, run=2, p2g_ints=1),
+        dict(p2g=1, g2p=1, run=1, p2g_ints=1)]
+INT_IDS = ["ints-run4", "ints-run2", "ints-run1"]
+SCHEDULES += INTS
+IDS += INT_IDS
 
 
 @pytest.mark.parametrize("opts", SCHEDULES, ids=IDS)
@@ -39,7 +46,7 @@ def test_walls_hold_3d(gpu):
     cases.case_walls_hold_3d(maker3())
 
 
-@pytest.mark.parametrize("opts", SCHEDULES[:3], ids=IDS[:3])
+@pytest.mark.parametrize("opts", SCHEDULES[:3] + INTS[:1], ids=IDS[:3] + INT_IDS[:1])
 def test_dam_break_slab_3d(gpu, opts):
     r = cases.case_dam_break_slab_3d(maker3(**opts))
     print(f"[flip3] GPU {opts}: moved {r[0]:.2f} of {r[1]:.2f}, J {r[2]:.3f}..{r[3]:.3f}, "
@@ -105,6 +112,61 @@ def test_scatter_parity_3d(gpu, opts):
     assert abs(float(np.sum(rig.read("gm"), dtype=np.float64)) - mass) <= 1e-6 * mass
 
 
+@pytest.mark.parametrize("opts", [SCHEDULES[2]] + INTS, ids=[IDS[2]] + INT_IDS)
+def test_sorted_scatter_parity_3d(gpu, opts):
+    """P2G of a dense block (8 particles a cell, the headline's density) after a sort, so most
+    waves take one brick's cells: every node within 1e-5 max(1, |e|) of the oracle's scatter of
+    the same sorted particles, mass to 1e-6."""
+    n = 40
+    x, y, z = scenes.jittered_block3((3, 3, 3), (21, 18, 17), 8, scenes.JavaRandomFloats(31))
+    count = len(x)
+    order = oracle.JavaRandom(32).shuffle_order(count)
+    rng = np.random.default_rng(5)
+    f = {"x": x[order], "y": y[order], "z": z[order], "m": np.full(count, 0.125, np.float32),
+         "j": (1.0 + 0.2 * rng.standard_normal(count)).astype(np.float32)}
+    for name in ("u", "v", "w"):
+        f[name] = (3.0 * rng.standard_normal(count)).astype(np.float32)
+    for name in oracle.FIELDS3[8:]:
+        f[name] = (0.5 * rng.standard_normal(count)).astype(np.float32)
+    ref = oracle.Mpm3(n, n, n, f, oracle.mpm3_params(1e-3, 0, -50, 0, 400.0, 1))
+    ref.sort()
+    ref.clear()
+    ref.scatter()
+    rig = maker3(**opts)(n, n, n, count)
+    cases.load3(rig, f)
+    rig.params(ref.params)
+    rig.run(["sort", "clear", "scatter"])
+    for name in ("x", "y", "z"):
+        np.testing.assert_array_equal(rig.read(name).view(np.int32), ref.p[name].view(np.int32), err_msg=name)
+    for g in ("gm", "gmu", "gmv", "gmw"):
+        _close(rig.read(g), ref.g[g], g)
+    mass = float(np.sum(f["m"], dtype=np.float64))
+    assert abs(float(np.sum(rig.read("gm"), dtype=np.float64)) - mass) <= 1e-6 * mass
+
+
+@pytest.mark.parametrize("opts", [SCHEDULES[2], INTS[0]], ids=[IDS[2], INT_IDS[0]])
+def test_scatter_carries_a_nan_3d(gpu, opts):
+    """A particle with a NaN velocity: the scatter leaves NaN in exactly the momentum nodes the
+    oracle's scatter does (its 27 nodes' u sums) and nothing else non-finite -- with integer tiles
+    its wave falls back to the direct atomics, which carry the NaN as the float schedules do."""
+    n, count = 32, 30_000
+    f = _state3(n, count, 11)
+    f["u"] = f["u"].copy()
+    f["u"][12_345] = np.float32("nan")
+    ref = oracle.Mpm3(n, n, n, f, oracle.mpm3_params(1e-3, 0, -50, 0, 400.0, 1))
+    ref.clear()
+    ref.scatter()
+    rig = maker3(**opts)(n, n, n, count)
+    cases.load3(rig, f)
+    rig.params(ref.params)
+    rig.run(["clear", "scatter"])
+    for g in ("gm", "gmu", "gmv", "gmw"):
+        got, exp = rig.read(g), ref.g[g]
+        np.testing.assert_array_equal(np.isnan(got), np.isnan(exp), err_msg=g)
+        ok = ~np.isnan(exp)
+        _close(got[ok], exp[ok], g)
+
+
 def test_grid_and_advect_bit_exact_3d(gpu):
     """From the oracle's own scatter, the device's grid update and both G2P variants are the
     oracle's results bit for bit."""
@@ -136,7 +198,7 @@ LAYERS = dict(p2g=3, g2p=2, run=4)
 
 
 @pytest.mark.parametrize("sort", [False, True], ids=["unsorted", "sorted"])
-@pytest.mark.parametrize("opts", SCHEDULES[:3] + [LAYERS], ids=IDS[:3] + ["layers"])
+@pytest.mark.parametrize("opts", SCHEDULES[:3] + INTS[:2] + [LAYERS], ids=IDS[:3] + INT_IDS[:2] + ["layers"])
 def test_steps_track_the_oracle_3d(gpu, opts, sort):
     """Twenty whole steps of a falling, compressing block: positions within 1e-5 relative of the
     oracle's, grid mass equal to particle mass to 1e-6. Sorted, the segmented P2G and the tiled

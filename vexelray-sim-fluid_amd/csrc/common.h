@@ -145,8 +145,9 @@ struct Fixed3 {
 int mpm3_clear_fixed(const Launch& l, const Mpm3Args& a, const Fixed3& fx);
 int mpm3_scatter_fixed(const Launch& l, const Mpm3Args& a, const Fixed3& fx);
 int mpm3_scatter_atomic(const Launch& l, const Mpm3Args& a);
-// Chunks of 256 * run particles, run in {1, 2, 4}.
-int mpm3_scatter_tiled(const Launch& l, const Mpm3Args& a, int run);
+// Chunks of 64 * run particles, run in {1, 2, 4}; `ints`: the wave tiles sum in 32-bit integers
+// (LDS integer atomics) instead of floats (read-add-write rounds).
+int mpm3_scatter_tiled(const Launch& l, const Mpm3Args& a, int run, bool ints);
 int mpm3_grid(const Launch& l, const Mpm3Args& a);
 int mpm3_advect(const Launch& l, const Mpm3Args& a);
 int mpm3_advect_tiled(const Launch& l, const Mpm3Args& a, int run, bool fast);

@@ -15,6 +15,7 @@
 // grid they reproduce the CPU restatement bit for bit; G2P reads its nodes from an LDS tile.
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 #include "common.h"
 
@@ -278,8 +279,11 @@ __device__ Box reduce_box(int (&lo)[3], int (&hi)[3], int* red, int cap) {
 // reference's (w m)(v + C d) + w s d (Flip3.java:225-243) with d = n - off, regrouped so the
 // per-node work is one add and one fused multiply-add per field. The weights are the
 // reference's products exactly; the regrouping moves each amount by a few ulps.
-__device__ __forceinline__ void p2g_fold(const Axis (&ax)[3], float m, const float (&vel)[3], const float (&c)[9],
-                                         float s, float (&acc)[108]) {
+// `mm` is the mass the mass sums take and `m`, `s` those the momentum sums take: the particle's own
+// for the float tiles, each times a power of two (exact) for the integer tiles, whose mass and
+// momentum sums are counted in different units.
+__device__ __forceinline__ void p2g_fold(const Axis (&ax)[3], float mm, float m, const float (&vel)[3],
+                                         const float (&c)[9], float s, float (&acc)[108]) {
     float B[3][3], E[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -307,12 +311,17 @@ __device__ __forceinline__ void p2g_fold(const Axis (&ax)[3], float m, const flo
             for (int i = 0; i < 3; ++i) {
                 const int k = i + 3 * j + 9 * l;
                 const float wgt = ax[0].w[i] * wyz;
-                acc[4 * k] = fmaf(wgt, m, acc[4 * k]);
+                acc[4 * k] = fmaf(wgt, mm, acc[4 * k]);
 #pragma unroll
                 for (int a = 0; a < 3; ++a) acc[4 * k + 1 + a] = fmaf(wgt, plus(T[a], B[a][0], i), acc[4 * k + 1 + a]);
             }
         }
     }
+}
+
+__device__ __forceinline__ void p2g_fold(const Axis (&ax)[3], float m, const float (&vel)[3], const float (&c)[9],
+                                         float s, float (&acc)[108]) {
+    p2g_fold(ax, m, m, vel, c, s, acc);
 }
 
 // Lanes of one wave exchange values through LDS below without atomics. The hardware keeps a
@@ -501,16 +510,14 @@ __device__ __forceinline__ void take(const Mpm3Args& a, const float (&law)[3], f
     }
 }
 
-// The tile for the base cells [lo, hi] of the wave's lanes (reduced over the wave): its bricks
-// marked, its nodes cleared.
-__device__ __forceinline__ Slab enter_tile(const Mpm3Args& a, float4* tile, int (&lo)[3], int (&hi)[3]) {
+// The tile for the base cells [lo, hi] of the wave's lanes (reduced over the wave), its bricks
+// marked (the caller clears its nodes).
+__device__ __forceinline__ Slab enter_tile(const Mpm3Args& a, int (&lo)[3], int (&hi)[3]) {
     const Box box = wave_box(lo, hi, kWaveTile);
     const int nlo[3] = {box.lo[0], box.lo[1], box.lo[2]};
     const int nhi[3] = {box.hi[0] + 2, box.hi[1] + 2, box.hi[2] + 2};
     mark_bricks(a, nlo, nhi, threadIdx.x, kWave);
-    const Slab sl = tile_of(box);
-    clear_tile(tile, sl);
-    return sl;
+    return tile_of(box);
 }
 
 // The tile's nodes into the grid: one global float atomic per touched node and field.
@@ -535,19 +542,162 @@ __device__ __forceinline__ void leave_tile(const Mpm3Args& a, const float4* tile
     }
 }
 
+// atomic_particle as a loop over the 27 nodes, the same amounts in the same arithmetic, for a rare
+// path: unrolled, its 108 node addresses would be computed up front, beside the caller's state.
+__device__ __forceinline__ void atomic_particle_rolled(const Mpm3Args& a, const Axis (&ax)[3], float m,
+                                                       const float (&vel)[3], const float (&c)[9], float s) {
+    const int64_t base = ((int64_t)ax[2].cell * a.ny + ax[1].cell) * a.nx + ax[0].cell;
+    const int64_t plane = (int64_t)a.nx * a.ny;
+    auto pick = [](const Axis& x, int i) { return i == 0 ? x.w[0] : (i == 1 ? x.w[1] : x.w[2]); };
+#pragma unroll 1
+    for (int k = 0; k < 27; ++k) {
+        const int i = k % 3, j = (k / 3) % 3, l = k / 9;
+        const float wgt = pick(ax[0], i) * (pick(ax[1], j) * pick(ax[2], l));
+        const float d[3] = {(float)i - ax[0].off, (float)j - ax[1].off, (float)l - ax[2].off};
+        const float wm = wgt * m;
+        const int64_t node = base + l * plane + j * a.nx + i;
+        atomic_add(a.g[0] + node, wm);
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            const float at3 = vel[e] + (c[3 * e] * d[0] + (c[3 * e + 1] * d[1] + c[3 * e + 2] * d[2]));
+            const float push = s * d[e];
+            atomic_add(a.g[1 + e] + node, wm * at3 + wgt * push);
+        }
+    }
+}
+
+// The float tile: a float4 (m, mu, mv, mw) a node, runs added by read-add-write in claim rounds.
+struct FloatTile {
+    float4* tile;
+    int* claim;
+    __device__ void clear(const Slab& sl) const { clear_tile(tile, sl); }
+    __device__ void take(const Mpm3Args& a, const float (&law)[3], const Slab& sl, bool in_pass, float x, float y,
+                         float z, float m, float j, const float (&vel)[3], const float (&c)[9], int& run_base,
+                         float (&acc)[108]) const {
+        ::vxr::take(a, law, tile, claim, sl, in_pass, x, y, z, m, j, vel, c, run_base, acc);
+    }
+    __device__ void last(const Slab& sl, int& run_base, float (&acc)[108]) const {
+        merge_pairs(run_base, acc);
+        flush_run(tile, claim, sl, run_base, run_base >= 0, acc);
+    }
+    __device__ void leave(const Mpm3Args& a, const Slab& sl) const { leave_tile(a, tile, sl); }
+};
+
+// x rounded to the nearest integer, ties to even; |x| < 2^31. (Halves up in one instruction,
+// floor(x + 1/2), would be biased: a run's float sum near 2^23 units has only a bit or two below
+// the unit, so halves are common there, and all of them would round up.)
+__device__ __forceinline__ int round_even(float x) { return (int)__builtin_rintf(x); }
+
+// The integer tile: four planes of kWaveTile 32-bit sums, mass in units of q[0] and momentum in
+// units of q[1], powers of two chosen per wave so that no node's sum can reach 2^31 (see the
+// kernel). A run leaves as 108 rounded integers by LDS integer atomics, which need no claim
+// rounds (lanes flushing one base meet at the same words in one instruction, and the atomics
+// serialise them) and no read before the write; each run's amounts are rounded once, to the
+// nearest unit.
+struct IntTile {
+    int* t;
+    float q[2], inv[2];
+    __device__ void clear(const Slab& sl) const {
+        lds_order();
+        for (int n = threadIdx.x; n < sl.nodes; n += kWave)
+#pragma unroll
+            for (int f = 0; f < 4; ++f) t[f * kWaveTile + n] = 0;
+    }
+    __device__ void flush(const Slab& sl, int run, bool mine, float (&acc)[108]) const {
+        if (mine) {
+            const int ex = sl.dims[0], ey = sl.dims[1];
+#pragma unroll
+            for (int k = 0; k < 27; ++k) {
+                const int node = run + ((k / 9) * ey + (k / 3) % 3) * ex + k % 3;
+#pragma unroll
+                for (int f = 0; f < 4; ++f) atomicAdd(t + f * kWaveTile + node, round_even(acc[4 * k + f]));
+                // a node at a time: scheduled freely, the 108 conversions would all be held at once
+                __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll
+            for (int i = 0; i < 108; ++i) acc[i] = 0.0f;
+        }
+    }
+    __device__ void take(const Mpm3Args& a, const float (&law)[3], const Slab& sl, bool in_pass, float x, float y,
+                         float z, float m, float j, const float (&vel)[3], const float (&c)[9], int& run_base,
+                         float (&acc)[108]) const {
+        Axis ax[3] = {axis(x, a.nx), axis(y, a.ny), axis(z, a.nz)};
+        const int base = ((ax[2].cell - sl.lo[2]) * sl.dims[1] + (ax[1].cell - sl.lo[1])) * sl.dims[0] +
+                         (ax[0].cell - sl.lo[0]);
+        const bool change = in_pass && base != run_base;
+        flush(sl, run_base, change && run_base >= 0, acc);
+        if (change) run_base = base;
+        if (in_pass) {
+            const float s = (4.0f * (law[0] * (m / law[2]))) * (law[1] * (1.0f - j));
+            p2g_fold(ax, m * inv[0], m * inv[1], vel, c, s * inv[1], acc);
+        }
+    }
+    __device__ void last(const Slab& sl, int& run_base, float (&acc)[108]) const {
+        flush(sl, run_base, run_base >= 0, acc);
+    }
+    // The tile's nodes into the grid: (float)sum * unit, one global float atomic per nonzero node
+    // and field. A wave's LDS operations run in order, so the reads see every atomic before them.
+    __device__ void leave(const Mpm3Args& a, const Slab& sl) const {
+        lds_order();
+        const int ex = sl.dims[0], ey = sl.dims[1];
+        const float inv_ex = 1.0f / (float)ex, inv_ey = 1.0f / (float)ey;
+        for (int n = threadIdx.x; n < sl.nodes; n += kWave) {
+            const int im = t[n];
+            if (im == 0) continue;  // no mass: no live particle weighs on the node (or only by < half a unit)
+            const float v[4] = {(float)im * q[0], (float)t[kWaveTile + n] * q[1], (float)t[2 * kWaveTile + n] * q[1],
+                                (float)t[3 * kWaveTile + n] * q[1]};
+            const int row = (int)(((float)n + 0.5f) * inv_ex);
+            const int lx = n - row * ex;
+            const int lz = (int)(((float)row + 0.5f) * inv_ey);
+            const int ly = row - lz * ey;
+            const int64_t node = ((int64_t)(sl.lo[2] + lz) * a.ny + (sl.lo[1] + ly)) * a.nx + (sl.lo[0] + lx);
+#pragma unroll
+            for (int f = 0; f < 4; ++f) atomic_add(a.g[f] + node, v[f]);
+        }
+    }
+};
+
+// The sum over a wave's 64 lanes, in every lane (all lanes active): DPP within rows of 16 (swap
+// neighbours, swap pairs, mirror within 8, mirror within 16), then the four rows' sums.
+__device__ __forceinline__ float wave_sum(float v) {
+    v = v + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+    v = v + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
+    v = v + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, false));
+    v = v + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x140, 0xF, 0xF, false));
+    auto at = [&](int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); };
+    return (at(0) + at(16)) + (at(32) + at(48));
+}
+
+// The unit of an integer tile's sums for a wave whose amounts are bounded by `total` (0.4219 total
+// bounds any node's sum): 2^(e - 30) for e = floor(log2 total), so a node's sum stays below 2^30
+// units; false when total is not finite.
+__device__ __forceinline__ bool tile_unit(float total, float& q, float& inv) {
+    int e = ((__float_as_int(total) >> 23) & 0xff) - 127;
+    if (e == 128) return false;
+    e = min(max(e, -96), 126);
+    q = __int_as_float((127 + e - 30) << 23);
+    inv = __int_as_float((127 + 30 - e) << 23);
+    return true;
+}
+
 // A wave takes a chunk of 64 * R particles, lane l the R consecutive ones from l * R, each field
 // read with one R-wide load so the wave's loads are whole cache lines, all in flight at once.
 // Runs of particles sharing a stencil base fold into 108 register sums. A run leaves through the
-// wave's own node tile in LDS, a float4 (m, mu, mv, mw) per node, by plain read-modify-write
-// (LDS float atomics run at a small fraction of LDS bandwidth on gfx950): lanes flushing the same
-// base take turns -- each writes its lane id into claim[base], the lane it reads back wins this
-// round -- so within a round no two lanes touch one node at one stencil offset, and the rounds,
-// the offsets and the wave's LDS operations are in program order. The tile then leaves with one
-// global float atomic per touched node and field.
-template <int R>
+// wave's own node tile in LDS, then the tile leaves with one global float atomic per touched node
+// and field. Two tiles (kInts):
+// - the float tile, a float4 (m, mu, mv, mw) per node, by plain read-modify-write (LDS float
+//   atomics run at a small fraction of LDS bandwidth on gfx950): lanes flushing the same base take
+//   turns -- each writes its lane id into claim[base], the lane it reads back wins this round -- so
+//   within a round no two lanes touch one node at one stencil offset, and the rounds, the offsets
+//   and the wave's LDS operations are in program order;
+// - the integer tile (IntTile): each run's sums rounded to the wave's units and added by LDS
+//   integer atomics, with no rounds and no chains of dependent LDS reads. The units are powers of
+//   two from bounds on the wave's amounts: a particle's mass amounts are w m and its momentum
+//   amounts w (m v_a + B_a . d) with |d_b| <= 3/2, w <= 0.75^3 = 0.4219, so 0.4219 times the sum over
+//   the wave of m, and of U = max_a (m |v_a| + 3/2 sum_b |B_ab|), bound any node's sums.
+template <int R, bool kInts>
 __global__ __launch_bounds__(kWave) void p2g3_wave_kernel(Mpm3Args a) {
-    __shared__ float4 tile[kWaveTile];
-    __shared__ int claim[kWaveTile];
+    __shared__ __align__(16) unsigned char lds[kInts ? 16 * kWaveTile : 20 * kWaveTile];
     const int lane = threadIdx.x;
     if ((int64_t)blockIdx.x * kWave * R >= *a.active) return;  // a wave of inactive slots
     // dt, bulk and rho0, read before the kernel stores anything, so once and into scalar registers
@@ -566,75 +716,118 @@ __global__ __launch_bounds__(kWave) void p2g3_wave_kernel(Mpm3Args a) {
         for (int i = 0; i < 3; ++i) c[i] = base_cell(f[i][r], size[i]);
         bk[r] = f[6][r] > 0.0f ? brick_of_cell(c, a.nx, a.ny) : INT_MAX;
     }
-    const int first_brick = lanes_min(min_of(bk, INT_MIN));
-    if (first_brick == INT_MAX) return;
-    float acc[108];
-#pragma unroll
-    for (int i = 0; i < 108; ++i) acc[i] = 0.0f;
-    if (lanes_max(max_of(bk)) == first_brick) {
-        // the common case: one brick, the particles from the registers they were loaded into
-        int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    typename std::conditional<kInts, IntTile, FloatTile>::type tile;
+    if constexpr (kInts) {
+        tile.t = reinterpret_cast<int*>(lds);
+        float sm = 0.0f, su = 0.0f;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if (bk[r] == INT_MAX) continue;
+            const float m = f[6][r];
+            const float s = (4.0f * (law[0] * (m / law[2]))) * (law[1] * (1.0f - f[7][r]));
+            // poison: the plain sum of the bounds, non-finite with any of them (fmaxf would drop a NaN)
+            float u = 0.0f, poison = 0.0f;
 #pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                const int c = base_cell(f[i][r], size[i]);
-                lo[i] = min(lo[i], c);
-                hi[i] = max(hi[i], c);
+            for (int e = 0; e < 3; ++e) {
+                float row = 0.0f;
+#pragma unroll
+                for (int b = 0; b < 3; ++b) row += fabsf(e == b ? fmaf(m, f[8 + 3 * e + b][r], s) : m * f[8 + 3 * e + b][r]);
+                const float t = fmaf(m, fabsf(f[3 + e][r]), 1.5f * row);
+                u = fmaxf(u, t);
+                poison += t;
             }
+            sm += m;
+            su += fmaf(0.0f, poison, u);
         }
-        const Slab sl = enter_tile(a, tile, lo, hi);
-        int run_base = -1;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const float vel[3] = {f[3][r], f[4][r], f[5][r]};
-            float c[9];
-#pragma unroll
-            for (int i = 0; i < 9; ++i) c[i] = f[8 + i][r];
-            take(a, law, tile, claim, sl, bk[r] != INT_MAX, f[0][r], f[1][r], f[2][r], f[6][r], f[7][r], vel, c,
-                 run_base, acc);
+        // rounding in these sums is far inside the factor 0.4219 x 2 of headroom the units leave
+        if (!tile_unit(wave_sum(sm), tile.q[0], tile.inv[0]) | !tile_unit(wave_sum(su), tile.q[1], tile.inv[1])) {
+            // a non-finite amount: the wave's particles by the reference's direct atomics, so it
+            // reaches the grid as the reference's schedule would carry it
+#pragma unroll 1
+            for (int r = 0; r < R; ++r) {
+                if (bk[r] == INT_MAX) continue;
+                Particle p;
+                load_particle(a, first + r, law[0], law[1], law[2], p);
+                mark_one(a, p.ax);
+                atomic_particle_rolled(a, p.ax, p.m, p.vel, p.c, p.s);
+            }
+            return;
         }
-        merge_pairs(run_base, acc);
-        flush_run(tile, claim, sl, run_base, run_base >= 0, acc);
-        leave_tile(a, tile, sl);
-        return;
+    } else {
+        tile.tile = reinterpret_cast<float4*>(lds);
+        tile.claim = reinterpret_cast<int*>(lds + 16 * kWaveTile);
     }
-    // Several bricks (a wave straddling a brick boundary, or unsorted input): one pass through
-    // the tile per brick, lowest first, each re-reading its particles (from cache); the stencils
-    // of one brick's particles cover at most 10 x 10 x 10 nodes, so they always fit. Past
-    // kMaxPasses bricks, the rest go by the reference's direct atomics.
+    const int first_brick = lanes_min(min_of(bk, INT_MIN));
+    if (first_brick == INT_MAX) return;
     int cur = first_brick;
-    for (int pass = 0; pass < kMaxPasses && cur != INT_MAX; ++pass) {
-        int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    {
+        float acc[108];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            if (bk[r] != cur) continue;
+        for (int i = 0; i < 108; ++i) acc[i] = 0.0f;
+        if (lanes_max(max_of(bk)) == first_brick) {
+            // the common case: one brick, the particles from the registers they were loaded into
+            int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
 #pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                const int c = base_cell(a.p[i][first + r], size[i]);
-                lo[i] = min(lo[i], c);
-                hi[i] = max(hi[i], c);
+            for (int r = 0; r < R; ++r) {
+                if (bk[r] == INT_MAX) continue;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const int c = base_cell(f[i][r], size[i]);
+                    lo[i] = min(lo[i], c);
+                    hi[i] = max(hi[i], c);
+                }
             }
-        }
-        const Slab sl = enter_tile(a, tile, lo, hi);
-        int run_base = -1;
+            const Slab sl = enter_tile(a, lo, hi);
+            tile.clear(sl);
+            int run_base = -1;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int64_t q = first + r;
-            const float vel[3] = {a.p[3][q], a.p[4][q], a.p[5][q]};
-            float c[9];
+            for (int r = 0; r < R; ++r) {
+                const float vel[3] = {f[3][r], f[4][r], f[5][r]};
+                float c[9];
 #pragma unroll
-            for (int i = 0; i < 9; ++i) c[i] = a.p[8 + i][q];
-            take(a, law, tile, claim, sl, bk[r] == cur, a.p[0][q], a.p[1][q], a.p[2][q], a.p[6][q], a.p[7][q], vel, c,
-                 run_base, acc);
+                for (int i = 0; i < 9; ++i) c[i] = f[8 + i][r];
+                tile.take(a, law, sl, bk[r] != INT_MAX, f[0][r], f[1][r], f[2][r], f[6][r], f[7][r], vel, c, run_base,
+                          acc);
+            }
+            tile.last(sl, run_base, acc);
+            tile.leave(a, sl);
+            return;
         }
-        merge_pairs(run_base, acc);
-        flush_run(tile, claim, sl, run_base, run_base >= 0, acc);
-        leave_tile(a, tile, sl);
-        cur = lanes_min(min_of(bk, cur));
+        // Several bricks (a wave straddling a brick boundary, or unsorted input): one pass through
+        // the tile per brick, lowest first, each re-reading its particles (from cache); the stencils
+        // of one brick's particles cover at most 10 x 10 x 10 nodes, so they always fit. Past
+        // kMaxPasses bricks, the rest go by the reference's direct atomics.
+        for (int pass = 0; pass < kMaxPasses && cur != INT_MAX; ++pass) {
+            int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (bk[r] != cur) continue;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const int c = base_cell(a.p[i][first + r], size[i]);
+                    lo[i] = min(lo[i], c);
+                    hi[i] = max(hi[i], c);
+                }
+            }
+            const Slab sl = enter_tile(a, lo, hi);
+            tile.clear(sl);
+            int run_base = -1;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int64_t q = first + r;
+                const float vel[3] = {a.p[3][q], a.p[4][q], a.p[5][q]};
+                float c[9];
+#pragma unroll
+                for (int i = 0; i < 9; ++i) c[i] = a.p[8 + i][q];
+                tile.take(a, law, sl, bk[r] == cur, a.p[0][q], a.p[1][q], a.p[2][q], a.p[6][q], a.p[7][q], vel, c,
+                          run_base, acc);
+            }
+            tile.last(sl, run_base, acc);
+            tile.leave(a, sl);
+            cur = lanes_min(min_of(bk, cur));
+        }
+        if (cur == INT_MAX) return;
     }
-    if (cur == INT_MAX) return;
     const float dt = law[0], bulk = law[1], rho0 = law[2];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -1704,16 +1897,18 @@ int mpm3_scatter_atomic(const Launch& l, const Mpm3Args& a) {
 }
 
 template <int R>
-static void launch_p2g(const Launch& l, const Mpm3Args& a) {
-    hipLaunchKernelGGL(p2g3_wave_kernel<R>, dim3(blocks_for(a.count, kWave * R)), dim3(kWave), 0, l.stream, a);
+static void launch_p2g(const Launch& l, const Mpm3Args& a, bool ints) {
+    const dim3 grid(blocks_for(a.count, kWave * R));
+    if (ints) hipLaunchKernelGGL((p2g3_wave_kernel<R, true>), grid, dim3(kWave), 0, l.stream, a);
+    else hipLaunchKernelGGL((p2g3_wave_kernel<R, false>), grid, dim3(kWave), 0, l.stream, a);
 }
 
-int mpm3_scatter_tiled(const Launch& l, const Mpm3Args& a, int run) {
+int mpm3_scatter_tiled(const Launch& l, const Mpm3Args& a, int run, bool ints) {
     if (a.count == 0) return 0;
     timed_begin(l, "p2g3_seg");
-    if (run == 4) launch_p2g<4>(l, a);
-    else if (run == 2) launch_p2g<2>(l, a);
-    else launch_p2g<1>(l, a);
+    if (run == 4) launch_p2g<4>(l, a, ints);
+    else if (run == 2) launch_p2g<2>(l, a, ints);
+    else launch_p2g<1>(l, a, ints);
     timed_end(l, "p2g3_seg");
     return check_launch("p2g3_seg");
 }

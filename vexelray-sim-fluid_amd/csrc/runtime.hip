@@ -458,7 +458,7 @@ int run_pass(vxr_sim* s, const std::string& pass) {
             if (p2g == 0) return vxr::mpm3_scatter_atomic(l, a);
             if (p2g == 2) return vxr::mpm3_scatter_fixed(l, a, fx);
             if (p2g == 3 && s->layers_valid) return vxr::mpm3_scatter_layers(l, a, s->layers);
-            return vxr::mpm3_scatter_tiled(l, a, (int)opt(s, "run"));
+            return vxr::mpm3_scatter_tiled(l, a, (int)opt(s, "run"), opt(s, "p2g_ints") != 0);
         }
         if (pass == "grid") {
             if (!s->tension) return vxr::mpm3_grid(l, a);
@@ -799,6 +799,7 @@ struct OptionSpec {
 const OptionSpec kOptions[] = {
     {"p2g", 0, 3, 1},   {"g2p", 0, 2, 2},        {"run", 1, 4, 4},   {"radix_bits", 4, 8, 8},
     {"rows", 1, 256, 32}, {"blocks", 0, 1 << 20, 0}, {"graph", 0, 1, 0}, {"hll_fast", 0, 1, 0},
+    {"p2g_ints", 0, 1, 0},
 };
 
 
