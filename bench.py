@@ -77,7 +77,7 @@ def kernel_table(sim, names, units, bytes_override=None):
     return out
 
 
-def bench_mpm3(steps, warmup, p2g, g2p, run, tension=False, lod_level=None, p2g_ints=0):
+def bench_mpm3(steps, warmup, p2g, g2p, run, tension=False, lod_level=None, p2g_ints=1):
     """The 3D substep on the two-cubes scene; with ``tension`` Flip3Step's tension step (six
     blurs, the stress and the grid with the capillary force), sigma = 0.01 B 8, the scale
     Tension3Test gives a drop of radius 8 nodes. With ``lod_level`` every cell's group of 8 is
@@ -505,7 +505,7 @@ def main():
     ap.add_argument("--p2g", type=int, default=1)
     ap.add_argument("--g2p", type=int, default=2)
     ap.add_argument("--run", type=int, default=4)
-    ap.add_argument("--p2g-ints", type=int, default=0)
+    ap.add_argument("--p2g-ints", type=int, default=1)
     ap.add_argument("--hll-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hll", action="store_true")
@@ -554,7 +554,7 @@ def main():
                                "256^3 nodes, dt=5e-5, sorted-segmented P2G, sort every 10 substeps",
                    "particles": count, "grid": [256, 256, 256], "sort_every": SORT_EVERY,
                    "p2g": ["atomic", "sorted-segmented", "fixed-point", "layers"][args.p2g], "g2p": ["gather", "lds-tile", "lds-tile-contracted"][args.g2p],
-                   "run": args.run},
+                   "run": args.run, "p2g_tile": ["float read-add-write", "int32 LDS atomics"][args.p2g_ints]},
         "roofline": roofline, "kernels": table, "grid_mass_drift": drift,
     }
     if not args.no_atomic:
@@ -585,7 +585,8 @@ def main():
                           "kernels": ftable, "grid_mass_drift": fdrift}
     if not args.no_tension:
         print(f"[bench] 3D MPM with tension: same scene, {args.tension_steps} steps", file=sys.stderr, flush=True)
-        tcount, tsecs, ttable, tdrift = bench_mpm3(args.tension_steps, 5, args.p2g, args.g2p, args.run, tension=True)
+        tcount, tsecs, ttable, tdrift = bench_mpm3(args.tension_steps, 5, args.p2g, args.g2p, args.run, tension=True,
+                                                   p2g_ints=args.p2g_ints)
         line["tension3"] = {"value": round(tcount * args.tension_steps / tsecs, 1), "unit": "particle-substeps/s",
                             "steps": args.tension_steps, "ms_per_step": round(tsecs / args.tension_steps * 1e3, 4),
                             "workload": "the same 3D scene stepped as Flip3Step's tension step (clear, P2G, six "
@@ -597,7 +598,8 @@ def main():
                                     "stop at the active count", "levels": {}}
         for level in (4, 1):
             print(f"[bench] 3D MPM thinned to {level} a cell, {args.tension_steps} steps", file=sys.stderr, flush=True)
-            lcount, lsecs, ltable, _ = bench_mpm3(args.tension_steps, 10, args.p2g, args.g2p, args.run, lod_level=level)
+            lcount, lsecs, ltable, _ = bench_mpm3(args.tension_steps, 10, args.p2g, args.g2p, args.run, lod_level=level,
+                                                 p2g_ints=args.p2g_ints)
             line["lod3"]["levels"][str(level)] = {
                 "active_particles": lcount, "ms_per_step": round(lsecs / args.tension_steps * 1e3, 4),
                 "active_particle_substeps_per_s": round(lcount * args.tension_steps / lsecs, 1),

@@ -207,7 +207,7 @@ int vxr_stream(vxr_sim* sim, void** stream);
  *                power-of-two units the wave chooses from bounds on its particles' amounts (mass
  *                and momentum apart), each run of particles rounded once to the nearest unit and
  *                added by LDS integer atomics; 0 = float tiles added by read-add-write rounds.
- *                Both within FP32 reassociation of the reference's sums (default 0)
+ *                Both within FP32 reassociation of the reference's sums (default 1)
  *   "radix_bits" sort digit width (4..8, default 8)
  *   "rows"       SWE: rows a wave marches per tile (1..256, default 32)
  *   "hll_fast"   SWE: 1 = the step with one refined hardware reciprocal per face and per cell in

@@ -11,10 +11,11 @@ from vexhip import _native, particles, scenes
 
 pytestmark = pytest.mark.gpu
 
-SCHEDULES = [dict(p2g=0, g2p=0), dict(p2g=1, g2p=1, run=4), dict(p2g=1, g2p=2, run=4), dict(p2g=1, g2p=2, run=1),
-             dict(p2g=1, g2p=2, run=2)]
+# The segmented schedule with float wave tiles (read-add-write rounds, option p2g_ints 0) ...
+SCHEDULES = [dict(p2g=0, g2p=0), dict(p2g=1, g2p=1, run=4, p2g_ints=0), dict(p2g=1, g2p=2, run=4, p2g_ints=0),
+             dict(p2g=1, g2p=2, run=1, p2g_ints=0), dict(p2g=1, g2p=2, run=2, p2g_ints=0)]
 IDS = ["atomic", "seg-exact", "seg-fast", "seg-run1", "seg-run2"]
-# The segmented schedule with integer wave tiles (option p2g_ints): runs rounded to each wave's
+# ... and with integer wave tiles (p2g_ints 1, the default): runs rounded to each wave's
 # power-of-two units and added by LDS integer atomics.
 INTS = [dict(p2g=1, g2p=2, run=4, p2g_ints=1), dict(p2g=1, g2p=2, run=2, p2g_ints=1),
         dict(p2g=1, g2p=1, run=1, p2g_ints=1)]

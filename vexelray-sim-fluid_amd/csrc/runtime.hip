@@ -799,7 +799,7 @@ struct OptionSpec {
 const OptionSpec kOptions[] = {
     {"p2g", 0, 3, 1},   {"g2p", 0, 2, 2},        {"run", 1, 4, 4},   {"radix_bits", 4, 8, 8},
     {"rows", 1, 256, 32}, {"blocks", 0, 1 << 20, 0}, {"graph", 0, 1, 0}, {"hll_fast", 0, 1, 0},
-    {"p2g_ints", 0, 1, 0},
+    {"p2g_ints", 0, 1, 1},
 };
 
 
