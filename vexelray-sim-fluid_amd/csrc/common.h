@@ -238,9 +238,9 @@ struct Batch2Args {
     unsigned long long* q[3];  // fixed-point sums: mass (2^-40), momentum u, v (2^-32)
     const int32_t* params;
     int cell_bits;
-    // Per 256-node chunk: P2G sets `touched` for every chunk a row of its tile box reaches; the
-    // grid pass moves it to `live` (chunks whose sums and velocities may be nonzero) and clears
-    // it. Clear and grid visit only chunks flagged in either.
+    // A byte for each unit of 64 nodes (four to a 256-node chunk): P2G sets `touched` for every
+    // unit a row of its tile box reaches; the grid pass moves it to `live` (units whose sums and
+    // velocities may be nonzero) and clears it. Clear and grid visit only units flagged in either.
     uint8_t* touched;
     uint8_t* live;
     // Per starts chunk: its first cell's key, its scene's first particle, how many of its cells
@@ -250,7 +250,8 @@ struct Batch2Args {
     const int32_t* sinfo;  // 4 words a chunk
     int32_t* bounds;
 };
-// `blocks` > 0 caps the grid (each block then strides over chunks); 0 is a block per chunk.
+// `blocks` > 0 caps the grid (each block then strides over chunks, or over windows of 64 unit
+// flags for clear, grid and decode); 0 is a block per chunk or window.
 int batch2_clear(const Launch& l, const Batch2Args& a, int blocks);
 int batch2_scatter(const Launch& l, const Batch2Args& a, int run, int blocks);
 // The fixed-point sums decoded into gm gmu gmv (no velocities), and, after a grid update that did

@@ -340,6 +340,8 @@ __device__ __forceinline__ void add_fixed(unsigned long long* p, long long v) {
 // LDS tile by 64-bit LDS atomics, and the tile's touched nodes to the grid by one 64-bit global
 // atomic each; otherwise straight to the grid. Integer sums: the same bits either way.
 constexpr int kTile2 = 2048;  // 3 x 8 B a node: 48 KB
+// Nodes a touched or live flag covers: four flags to a 256-node chunk, one byte each.
+constexpr int kUnit2 = 64;
 
 
 template <int kRun>
@@ -394,10 +396,10 @@ __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a) {
         const int w = box[2] - bx0 + 3, h = box[3] - by0 + 3;
         const bool tiled = box[2] >= 0 && w * h <= kTile2;
         if (box[2] >= 0) {
-            // the chunks each node row of the box reaches (the box lies inside the scene's grid)
+            // the units each node row of the box reaches (the box lies inside the scene's grid)
             for (int r = threadIdx.x; r < h; r += kBlock) {
                 const int64_t row0 = sc.goff + (int64_t)(by0 + r) * sc.nx + bx0;
-                for (int64_t ch = row0 / kBlock; ch <= (row0 + w - 1) / kBlock; ++ch) a.touched[ch] = 1;
+                for (int64_t u = row0 / kUnit2; u <= (row0 + w - 1) / kUnit2; ++u) a.touched[u] = 1;
             }
         }
         if (tiled) {
@@ -475,59 +477,127 @@ __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a) {
     }
 }
 
-// Zeroes the sums of every chunk flagged touched or live; the others hold zero sums already.
+// The visits of clear, grid and decode. A block takes a window of kWin2 units (16 chunks, 4,096
+// nodes): wave 0 reads the window's flags with one load a lane and lists its units flagged
+// touched or live in LDS, and wave 1 reads the scenes of the window's chunks (when any of its
+// flags is set) into an LDS table. Then the block's waves split the listed units, a node a lane,
+// four units a wave in flight at once. A window with no flag costs one round trip, and no
+// chunk's scene chain (chunk -> scene -> record -> parameters) stands between the flags and the
+// sums' reads.
+constexpr int kWin2 = 64;
+constexpr int kWinChunks = kWin2 * kUnit2 / kBlock;
+constexpr int kWaves2 = kBlock / kWave;
+static_assert(kWin2 == kWave, "a window's flags are one wave's lanes");
+
+struct WinScenes {
+    int64_t goff[kWinChunks];
+    int nx[kWinChunks], ny[kWinChunks];
+    Law2 law[kWinChunks];
+};
+
+__device__ __forceinline__ int64_t units2(const Batch2Args& a) { return a.gchunks * (kBlock / kUnit2); }
+
+// Wave 0: the window's units flagged touched or live, as unit numbers within the window in order,
+// into list[0, *count).
+__device__ __forceinline__ void window_units(const Batch2Args& a, int64_t win, int lane, int* list, int* count) {
+    const int64_t u = win * kWin2 + lane;
+    const bool on = u < units2(a) && (a.touched[u] | a.live[u]) != 0;
+    const unsigned long long mask = __ballot(on);
+    if (on) list[__popcll(mask & ((1ull << lane) - 1ull))] = lane;
+    if (lane == 0) *count = __popcll(mask);
+}
+
+// Wave 1: the scenes of the window's chunks, when any of the window's flags is set.
+__device__ __forceinline__ void window_scenes(const Batch2Args& a, int64_t win, int lane, WinScenes& ws) {
+    const int64_t c = win * kWinChunks + lane;
+    const bool in = lane < kWinChunks && c < a.gchunks;
+    uint32_t f = 0;
+    if (in) f = ((const uint32_t*)a.touched)[c] | ((const uint32_t*)a.live)[c];  // a chunk's 4 flags
+    if (__ballot(f != 0) == 0ull || !in) return;
+    const int s = a.gscene[c];
+    const Scene2 sc = a.scenes[s];
+    ws.goff[lane] = sc.goff;
+    ws.nx[lane] = sc.nx;
+    ws.ny[lane] = sc.ny;
+    ws.law[lane] = law2(a.params + 28 * (int64_t)s);
+}
+
+// Zeroes the sums of every unit flagged touched or live; the others hold zero sums already.
 __global__ __launch_bounds__(kBlock) void batch2_clear_kernel(Batch2Args a) {
-    for (int64_t c = blockIdx.x; c < a.gchunks; c += gridDim.x) {
-        if (!(a.touched[c] | a.live[c])) continue;
-        const int64_t i = c * kBlock + threadIdx.x;
-        a.q[0][i] = 0ull;
-        a.q[1][i] = 0ull;
-        a.q[2][i] = 0ull;
+    __shared__ int list[kWin2];
+    __shared__ int count;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int64_t windows = (units2(a) + kWin2 - 1) / kWin2;
+    for (int64_t win = blockIdx.x; win < windows; win += gridDim.x) {
+        if (wave == 0) window_units(a, win, lane, list, &count);
+        __syncthreads();
+        const int n = count;
+        for (int i = wave; i < n; i += kWaves2) {
+            const int64_t at = (win * kWin2 + list[i]) * kUnit2 + lane;
+            a.q[0][at] = 0ull;
+            a.q[1][at] = 0ull;
+            a.q[2][at] = 0ull;
+        }
+        __syncthreads();  // list and count serve the next window
     }
 }
 
-// After a grid pass: live := touched, touched := 0.
-__global__ void batch2_flags_kernel(uint8_t* __restrict__ touched, uint8_t* __restrict__ live, int64_t chunks) {
+// After a grid pass: live := touched, touched := 0, a chunk's four unit flags as one word.
+__global__ void batch2_flags_kernel(uint32_t* __restrict__ touched, uint32_t* __restrict__ live, int64_t chunks) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= chunks) return;
     live[c] = touched[c];
     touched[c] = 0;
 }
 
-// The sums decoded into gm gmu gmv, then Flip.grid on them, over the chunks flagged touched or
-// live. A live chunk P2G did not reach again holds zero sums, so it gets the zero velocity the
-// full pass would give it; a chunk in neither set already holds zero sums and velocities.
+// kVelocity: the sums decoded into gm gmu gmv, then Flip.grid on them, over the units flagged
+// touched or live. A live unit P2G did not reach again holds zero sums, so it gets the zero
+// velocity the full pass would give it; a unit in neither set already holds zero sums and
+// velocities. Without kVelocity, the decoding alone: for passes that read the mass before the
+// grid update.
+template <bool kVelocity>
 __global__ __launch_bounds__(kBlock) void batch2_grid_kernel(Batch2Args a) {
-    for (int64_t c = blockIdx.x; c < a.gchunks; c += gridDim.x) {
-        if (!(a.touched[c] | a.live[c])) continue;
-        const int s = a.gscene[c];
-        const Scene2 sc = a.scenes[s];
-        const int64_t node = c * kBlock + threadIdx.x - sc.goff;
-        if (node >= (int64_t)sc.nx * sc.ny) continue;
-        const int64_t at = sc.goff + node;
-        const Law2 law = law2(a.params + 28 * (int64_t)s);  // read before the stores below
-        const float m = dequantize(a.q[0][at], kMassBits);
-        const float mu = dequantize(a.q[1][at], kMomentumBits);
-        const float mv = dequantize(a.q[2][at], kMomentumBits);
-        a.g[0][at] = m;
-        a.g[1][at] = mu;
-        a.g[2][at] = mv;
-        grid2_node(node, sc.nx, law, m, mu, mv, sc.ny, a.g[3] + sc.goff, a.g[4] + sc.goff);
-    }
-}
-
-// The sums decoded into gm gmu gmv over the chunks flagged touched or live, as batch2_grid_kernel
-// decodes them, without the velocities: for passes that read the mass before the grid update.
-__global__ __launch_bounds__(kBlock) void batch2_decode_kernel(Batch2Args a) {
-    for (int64_t c = blockIdx.x; c < a.gchunks; c += gridDim.x) {
-        if (!(a.touched[c] | a.live[c])) continue;
-        const Scene2 sc = a.scenes[a.gscene[c]];
-        const int64_t node = c * kBlock + threadIdx.x - sc.goff;
-        if (node >= (int64_t)sc.nx * sc.ny) continue;
-        const int64_t at = sc.goff + node;
-        a.g[0][at] = dequantize(a.q[0][at], kMassBits);
-        a.g[1][at] = dequantize(a.q[1][at], kMomentumBits);
-        a.g[2][at] = dequantize(a.q[2][at], kMomentumBits);
+    __shared__ int list[kWin2];
+    __shared__ int count;
+    __shared__ WinScenes ws;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int64_t windows = (units2(a) + kWin2 - 1) / kWin2;
+    for (int64_t win = blockIdx.x; win < windows; win += gridDim.x) {
+        if (wave == 0) window_units(a, win, lane, list, &count);
+        else if (wave == 1) window_scenes(a, win, lane, ws);
+        __syncthreads();
+        const int n = count;
+        // wave w takes listed units w, w + 4, w + 8, w + 12, then the next 16: the sums of all four
+        // read before the first store (a short last group reads its last unit again)
+        for (int i0 = wave; i0 < n; i0 += 4 * kWaves2) {
+            unsigned long long v[4][3];
+            int64_t at[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int i = min(i0 + k * kWaves2, n - 1);
+                at[k] = (win * kWin2 + list[i]) * kUnit2 + lane;
+#pragma unroll
+                for (int f = 0; f < 3; ++f) v[k][f] = a.q[f][at[k]];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int i = i0 + k * kWaves2;
+                if (i >= n) break;
+                const int w = list[i] / (kBlock / kUnit2);  // the unit's chunk within the window
+                const int64_t goff = ws.goff[w];
+                const int nx = ws.nx[w], ny = ws.ny[w];
+                const int64_t node = at[k] - goff;
+                if (node >= (int64_t)nx * ny) continue;  // the padding past the scene's last node
+                const float m = dequantize(v[k][0], kMassBits);
+                const float mu = dequantize(v[k][1], kMomentumBits);
+                const float mv = dequantize(v[k][2], kMomentumBits);
+                a.g[0][at[k]] = m;
+                a.g[1][at[k]] = mu;
+                a.g[2][at[k]] = mv;
+                if (kVelocity) grid2_node(node, nx, ws.law[w], m, mu, mv, ny, a.g[3] + goff, a.g[4] + goff);
+            }
+        }
+        __syncthreads();  // list, count and the scene table serve the next window
     }
 }
 
@@ -769,10 +839,12 @@ unsigned grid_of(int64_t chunks, int blocks) {
     if (chunks <= 0) return 0;
     return (unsigned)(blocks > 0 && blocks < chunks ? blocks : chunks);
 }
+// Windows of kWin2 flags over the grid's units, which the clear, grid and decode blocks stride over.
+int64_t windows2(const Batch2Args& a) { return (a.gchunks * (kBlock / kUnit2) + kWin2 - 1) / kWin2; }
 }  // namespace
 
 int batch2_clear(const Launch& l, const Batch2Args& a, int blocks) {
-    const unsigned g = grid_of(a.gchunks, blocks);
+    const unsigned g = grid_of(windows2(a), blocks);
     if (!g) return 0;
     timed_begin(l, "batch2_clear");
     hipLaunchKernelGGL(batch2_clear_kernel, dim3(g), dim3(kBlock), 0, l.stream, a);
@@ -795,21 +867,21 @@ int batch2_scatter(const Launch& l, const Batch2Args& a, int run, int blocks) {
 }
 
 int batch2_grid(const Launch& l, const Batch2Args& a, int blocks) {
-    const unsigned g = grid_of(a.gchunks, blocks);
+    const unsigned g = grid_of(windows2(a), blocks);
     if (!g) return 0;
     timed_begin(l, "batch2_grid");
-    hipLaunchKernelGGL(batch2_grid_kernel, dim3(g), dim3(kBlock), 0, l.stream, a);
+    hipLaunchKernelGGL(batch2_grid_kernel<true>, dim3(g), dim3(kBlock), 0, l.stream, a);
     hipLaunchKernelGGL(batch2_flags_kernel, dim3((unsigned)((a.gchunks + 255) / 256)), dim3(256), 0, l.stream,
-                       a.touched, a.live, a.gchunks);
+                       (uint32_t*)a.touched, (uint32_t*)a.live, a.gchunks);
     timed_end(l, "batch2_grid");
     return check_launch("batch2_grid");
 }
 
 int batch2_decode(const Launch& l, const Batch2Args& a, int blocks) {
-    const unsigned g = grid_of(a.gchunks, blocks);
+    const unsigned g = grid_of(windows2(a), blocks);
     if (!g) return 0;
     timed_begin(l, "batch2_decode");
-    hipLaunchKernelGGL(batch2_decode_kernel, dim3(g), dim3(kBlock), 0, l.stream, a);
+    hipLaunchKernelGGL(batch2_grid_kernel<false>, dim3(g), dim3(kBlock), 0, l.stream, a);
     timed_end(l, "batch2_decode");
     return check_launch("batch2_decode");
 }
@@ -817,7 +889,7 @@ int batch2_decode(const Launch& l, const Batch2Args& a, int blocks) {
 int batch2_flags(const Launch& l, const Batch2Args& a) {
     if (a.gchunks <= 0) return 0;
     hipLaunchKernelGGL(batch2_flags_kernel, dim3((unsigned)((a.gchunks + 255) / 256)), dim3(256), 0, l.stream,
-                       a.touched, a.live, a.gchunks);
+                       (uint32_t*)a.touched, (uint32_t*)a.live, a.gchunks);
     return check_launch("batch2_flags");
 }
 

@@ -182,7 +182,7 @@ struct vxr_sim {
     vxr::Scene2* d_scenes = nullptr;
     int32_t *pscene = nullptr, *gscene = nullptr, *sscene = nullptr;
     int64_t pchunks = 0, gchunks = 0, schunks = 0;
-    uint8_t *btouched = nullptr, *blive = nullptr;  // 2D fixed-point path: chunk flags
+    uint8_t *btouched = nullptr, *blive = nullptr;  // 2D fixed-point path: a flag each 64 nodes
     int32_t* sinfo = nullptr;    // batch: per starts chunk, what the layout fixes (4 words)
     int32_t* sbounds = nullptr;  // batch: per starts chunk, its first particle (the sort's scratch)
     unsigned long long* q[3] = {};
@@ -349,8 +349,8 @@ vxr::Mpm3Args mpm3_args(vxr_sim* s) {
 // grid fields, or cleared everything). Never inside a captured graph.
 int mark_all_bricks(vxr_sim* s) {
     if (s->btouched) {
-        HIP_TRY(hipMemsetAsync(s->btouched, 1, s->gchunks, s->stream));
-        HIP_TRY(hipMemsetAsync(s->blive, 1, s->gchunks, s->stream));
+        HIP_TRY(hipMemsetAsync(s->btouched, 1, s->gchunks * 4, s->stream));  // four flags a chunk
+        HIP_TRY(hipMemsetAsync(s->blive, 1, s->gchunks * 4, s->stream));
     }
     if (!s->touched) return 0;
     HIP_TRY(hipMemsetAsync(s->touched, 1, s->bricks, s->stream));
@@ -852,9 +852,10 @@ int alloc_batch(vxr_sim* s) {
         if ((rc = alloc(s, s->gchunks * 256 * 2, &ptr))) return rc;
         s->q[f] = (unsigned long long*)ptr;
     }
-    if ((rc = alloc(s, (std::max<int64_t>(s->gchunks, 1) + 3) / 4, &ptr))) return rc;
+    // a byte for each 64 nodes: a word a chunk
+    if ((rc = alloc(s, std::max<int64_t>(s->gchunks, 1), &ptr))) return rc;
     s->btouched = (uint8_t*)ptr;
-    if ((rc = alloc(s, (std::max<int64_t>(s->gchunks, 1) + 3) / 4, &ptr))) return rc;
+    if ((rc = alloc(s, std::max<int64_t>(s->gchunks, 1), &ptr))) return rc;
     s->blive = (uint8_t*)ptr;
     if ((rc = alloc(s, std::max<int64_t>(s->schunks, 1), &ptr))) return rc;
     s->sbounds = (int32_t*)ptr;
