@@ -81,6 +81,43 @@ def test_fixed_p2g_bit_exact_with_oracle(gpu, run):
     np.testing.assert_array_equal(dev.read_words("starts"), oracle.mpm2_cell_starts(ref.p["x"], ref.p["y"], n, n))
 
 
+@pytest.mark.parametrize("case", ["heavy", "fast"])
+def test_fixed_p2g_wide_amounts(gpu, case):
+    """Rounded amounts of 2^40 and more: every particle at rho0 = 1000 (mass 250 at 4 a cell, the
+    same motion as rho0 = 1 with mass 0.25), or one particle in 97 at 8000 nodes/s. The blocks that
+    hold such amounts sum in whole 64-bit integers, the others in split 32-bit pairs; the oracle's
+    bits either way, over one pass and 20 steps with a sort every 10."""
+    n = 64
+    sc = _scene(n, (20, 30), 2400, 5, speed=20.0)
+    h = 1.0 / (n - 1)
+    if case == "heavy":
+        sc["m"] = np.full(len(sc["x"]), 250.0, np.float32)
+        params = oracle.mpm2_params(1e-4 * 127.0 / (n - 1), 0, -9.8 / h, 10.0 / h ** 2, 1000)
+    else:
+        sc["u"][::97] = np.float32(8000.0)
+        params = _params(n)
+    print(f"[batch] fixed p2g wide amounts: {case}", flush=True)
+    dev = _single(n, sc, params)
+    ref = oracle.Mpm2(n, n, sc, params, fixed=True)
+    for p in ("clear", "scatter", "grid"):
+        dev.run_pass(p)
+        getattr(ref, p)()
+    for g in ("gm", "gmu", "gmv", "gu", "gv"):
+        np.testing.assert_array_equal(dev.read(g).view(np.int32), ref.g[g].view(np.int32), err_msg=g)
+    # the largest amounts pass 2^40 (mass in units of 2^-40, momentum of 2^-32; a particle's largest
+    # weight is at least 1/4)
+    assert max(float(sc["m"].max()) * 2.0 ** 40, float((sc["m"] * np.abs(sc["u"])).max()) * 2.0 ** 32) / 4 >= 2.0 ** 40
+    dev.run_pass("advect")
+    ref.advect()
+    dev.advance(20, 10, 1)
+    ref.steps = 1
+    ref.advance(20, 10)
+    _assert_fields(dev.read, ref.p, f"{case}: 20 steps")
+    for g in ("gm", "gmu", "gmv", "gu", "gv"):
+        np.testing.assert_array_equal(dev.read(g).view(np.int32), ref.g[g].view(np.int32), err_msg=f"{case}: {g}")
+    dev.close()
+
+
 def _batch_case():
     # grids of several sizes, counts off the 1024 padding, one scene of a few particles
     spec = [(64, (20, 30), 2400, 11), (16, (4, 6), 5, 12), (128, (40, 70), 11000, 13), (40, (12, 20), 960, 14),

@@ -343,11 +343,28 @@ constexpr int kTile2 = 2048;  // 3 x 8 B a node: 48 KB
 // Nodes a touched or live flag covers: four flags to a 256-node chunk, one byte each.
 constexpr int kUnit2 = 64;
 
+// A rounded amount r (an integer in binary32, |r| < 2^40) as the pair h = floor(r / 2^20),
+// l = r - 2^20 h in [0, 2^20), h in the high word and l in the low one, every step exact (r
+// 2^-20 is a power-of-two scaling, and l, an integer below 2^20, is a binary32). Added as one
+// 64-bit integer, the low words' sum stays below 2^30 for up to 1,024 amounts, so no carry
+// reaches the high word, whose sum of |h| <= 2^20 stays in int32: the LDS tile sums a block's
+// amounts with the 64-bit LDS atomic as before, for five 32-bit instructions an amount instead
+// of a fold and split in double precision. unpair20 gives back the exact integer sum.
+constexpr float kPairLimit = 0x1.fep39f;  // below 2^40 by a margin for the bound's own rounding
+__device__ __forceinline__ unsigned long long pair20(float r) {
+    const float h = floorf(r * 0x1p-20f);
+    const float l = fmaf(h, -0x1p20f, r);
+    return ((unsigned long long)(unsigned)(int)h << 32) | (unsigned long long)(unsigned)(int)l;
+}
+__device__ __forceinline__ unsigned long long unpair20(unsigned long long t) {
+    return (unsigned long long)((long long)(int)(unsigned)(t >> 32) * (1ll << 20) + (long long)(t & 0xffffffffull));
+}
+
 
 template <int kRun>
 __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a) {
     __shared__ unsigned long long tile[3][kTile2];
-    __shared__ int box[4];
+    __shared__ int box[5];
     const float mscale = 0x1p40f, pscale = 0x1p32f;
     constexpr int kOwn = kBatchAlign / kBlock;  // a thread's particles: 4, consecutive
     static_assert(kOwn == 4 && kOwn % kRun == 0, "a thread's particles split into whole runs");
@@ -369,9 +386,14 @@ __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a) {
         if (threadIdx.x == 0) {
             box[0] = box[1] = 0x7fffffff;
             box[2] = box[3] = -1;
+            box[4] = 0;
         }
         __syncthreads();
         int x0 = 0x7fffffff, y0 = 0x7fffffff, x1 = -1, y1 = -1;
+        // a bound on the magnitude of every amount a particle rounds (Flip.java:283-300's terms:
+        // weights <= 0.5625, offsets <= 1.5), as a sum so a NaN carries through to "no bound"
+        const float inv_rho0 = 1.0f / fabsf(rho0);
+        float bound = 0.0f;
 #pragma unroll
         for (int r = 0; r < kOwn; ++r) {
             if (first + r >= live) break;
@@ -380,21 +402,30 @@ __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a) {
             y0 = min(y0, by);
             x1 = max(x1, bx);
             y1 = max(y1, by);
+            const float mp = fabsf(f[4][r] * pscale);
+            const float sb = 4.0f * fabsf(dt) * (mp * inv_rho0) * fabsf(bulk) * fabsf(1.0f - f[5][r]);
+            const float b = 0.5625f * (mp * (256.0f + fabsf(f[2][r]) + fabsf(f[3][r]) +
+                                             1.5f * (fabsf(f[6][r]) + fabsf(f[7][r]) + fabsf(f[8][r]) + fabsf(f[9][r]))) +
+                                       1.5f * sb);
+            bound = b < kPairLimit ? fmaxf(bound, b) : __builtin_inff();
         }
         x0 = wave_min(x0);
         y0 = wave_min(y0);
         x1 = wave_max(x1);
         y1 = wave_max(y1);
+        const int bound_bits = wave_max(__float_as_int(bound));  // non-negative: int order is float order
         if ((threadIdx.x & (kWave - 1)) == 0) {
             atomicMin(&box[0], x0);
             atomicMin(&box[1], y0);
             atomicMax(&box[2], x1);
             atomicMax(&box[3], y1);
+            atomicMax(&box[4], bound_bits);
         }
         __syncthreads();
         const int bx0 = box[0], by0 = box[1];
         const int w = box[2] - bx0 + 3, h = box[3] - by0 + 3;
         const bool tiled = box[2] >= 0 && w * h <= kTile2;
+        const bool pairs = tiled && __int_as_float(box[4]) < kPairLimit;
         if (box[2] >= 0) {
             // the units each node row of the box reaches (the box lies inside the scene's grid)
             for (int r = threadIdx.x; r < h; r += kBlock) {
@@ -410,6 +441,26 @@ __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a) {
         unsigned long long* q0 = a.q[0] + sc.goff;
         unsigned long long* q1 = a.q[1] + sc.goff;
         unsigned long long* q2 = a.q[2] + sc.goff;
+        if (pairs) {
+            // every amount of the block below 2^40: a particle at a time, each amount rounded and
+            // split into its pair, added to the tile
+#pragma unroll
+            for (int r = 0; r < kOwn; ++r) {
+                if (first + r >= live) break;
+                float v[10];
+#pragma unroll
+                for (int k = 0; k < 10; ++k) v[k] = f[k][r];
+                const Axis ax = axis(v[0], sc.nx), ay = axis(v[1], sc.ny);
+                const int at0 = (ay.cell - by0) * w + (ax.cell - bx0);
+                v[4] = v[4] * pscale;
+                amounts(v, ax, ay, dt, bulk, rho0, [&](int k, float m, float mu, float mv) {
+                    const int at = at0 + (k / 3) * w + k % 3;
+                    atomicAdd(&tile[0][at], pair20(rintf(m * (mscale / pscale))));
+                    atomicAdd(&tile[1][at], pair20(rintf(mu)));
+                    atomicAdd(&tile[2][at], pair20(rintf(mv)));
+                });
+            }
+        } else {
 #pragma unroll
         for (int pass = 0; pass < kOwn / kRun; ++pass) {
             double acc[27];  // the run's fixed-point sums, exact integers in double
@@ -462,10 +513,16 @@ __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a) {
             }
             if (rx >= 0) flush();
         }
+        }
         __syncthreads();
         if (tiled) {
             for (int i = threadIdx.x; i < w * h; i += kBlock) {
-                const unsigned long long v0 = tile[0][i], v1 = tile[1][i], v2 = tile[2][i];
+                unsigned long long v0 = tile[0][i], v1 = tile[1][i], v2 = tile[2][i];
+                if (pairs) {
+                    v0 = unpair20(v0);
+                    v1 = unpair20(v1);
+                    v2 = unpair20(v2);
+                }
                 if ((v0 | v1 | v2) == 0ull) continue;
                 const int64_t node = (int64_t)(by0 + i / w) * sc.nx + bx0 + i % w;
                 atomicAdd(q0 + node, v0);
