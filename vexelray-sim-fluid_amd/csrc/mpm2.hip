@@ -226,10 +226,10 @@ __device__ __forceinline__ void g2p2_gather(float* const* p, int64_t q, int nx, 
     g2p2_nodes(nx, ny, gu, gv, in);
 }
 
-// Flip.advectBody, Flip.java:494-545 (not relaxing), for particle q of one scene, from what
-// g2p2_gather read. Reads and stores are split so a thread can have several particles' reads
-// in flight before its first store (a read cannot be moved past a store that may alias it).
-__device__ __forceinline__ void g2p2_finish(float* const* p, int64_t q, int nx, int ny, float dt, const G2p2In& in) {
+// Flip.advectBody, Flip.java:494-545 (not relaxing), from what g2p2_gather read: the particle's
+// new fields in `o`, in the order of kG2pFields.
+constexpr int kG2pFields[9] = {0, 1, 2, 3, 6, 7, 8, 9, 5};  // x y u v c00 c01 c10 c11 j
+__device__ __forceinline__ void g2p2_compute(int nx, int ny, float dt, const G2p2In& in, float (&o)[9]) {
     const Axis& ax = in.ax;
     const Axis& ay = in.ay;
     float u = 0, w = 0, b00 = 0, b01 = 0, b10 = 0, b11 = 0;
@@ -252,16 +252,26 @@ __device__ __forceinline__ void g2p2_finish(float* const* p, int64_t q, int nx, 
     float y = in.y + dt * w;
     keep_inside(x, u, nx);
     keep_inside(y, w, ny);
-    p[0][q] = x;
-    p[1][q] = y;
-    p[2][q] = u;
-    p[3][q] = w;
-    p[6][q] = c00;
-    p[7][q] = c01;
-    p[8][q] = c10;
-    p[9][q] = c11;
     const float j = in.j * (1.0f + dt * (c00 + c11));
-    p[5][q] = clampf(j, 1.0f / kJBound, kJBound);
+    o[0] = x;
+    o[1] = y;
+    o[2] = u;
+    o[3] = w;
+    o[4] = c00;
+    o[5] = c01;
+    o[6] = c10;
+    o[7] = c11;
+    o[8] = clampf(j, 1.0f / kJBound, kJBound);
+}
+
+// The same for particle q, stored. Reads and stores are split so a thread can have several
+// particles' reads in flight before its first store (a read cannot be moved past a store that
+// may alias it).
+__device__ __forceinline__ void g2p2_finish(float* const* p, int64_t q, int nx, int ny, float dt, const G2p2In& in) {
+    float o[9];
+    g2p2_compute(nx, ny, dt, in, o);
+#pragma unroll
+    for (int f = 0; f < 9; ++f) p[kG2pFields[f]][q] = o[f];
 }
 
 __device__ __forceinline__ void g2p2_particle(float* const* p, int64_t q, int nx, int ny, const float* gu,
@@ -658,36 +668,109 @@ __global__ __launch_bounds__(kBlock) void batch2_grid_kernel(Batch2Args a) {
     }
 }
 
-// A block a chunk of kBatchAlign particles (one scene's), a thread kBatchAlign / kBlock of them:
-// every read of the thread's particles in flight, then their sums and stores.
+// G2P over a batch. A block takes a chunk of kBatchAlign particles (one scene's), a thread every
+// kBlock-th of them. The box of nodes their stencils reach is staged from gu and gv into LDS,
+// every load of it in flight before the first LDS store; each particle then reads its 9 nodes
+// from the box (from the grid when the box does not fit: unsorted particles) and stores its
+// fields, a particle at a time.
+constexpr int kTileG2 = 2048;  // gu and gv of a chunk's box: 16 KB
 __global__ __launch_bounds__(kBlock) void batch2_g2p_kernel(Batch2Args a) {
+    __shared__ float tile[2][kTileG2];
+    __shared__ int box[4];
     constexpr int kPer = kBatchAlign / kBlock;
+    constexpr int kLoads = kTileG2 / kBlock;
     for (int64_t pc = blockIdx.x; pc < a.pchunks; pc += gridDim.x) {
-        // positions and J first, beside the scene's number: the chunk lies inside its scene's
-        // range (padded to whole chunks), so these reads need nothing from the scene
-        G2p2In in[kPer];
+        // positions and J beside the scene's number: the chunk lies inside its scene's range
+        // (padded to whole chunks), so these reads need nothing from the scene
+        float px[kPer], py[kPer], pj[kPer];
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int64_t q = pc * kBatchAlign + k * kBlock + threadIdx.x;
-            in[k].x = a.p[0][q];
-            in[k].y = a.p[1][q];
-            in[k].j = a.p[5][q];
+            px[k] = a.p[0][q];
+            py[k] = a.p[1][q];
+            pj[k] = a.p[5][q];
         }
         const int s = a.pscene[pc];
         const Scene2 sc = a.scenes[s];
+        const float dt = bits_f(a.params[28 * (int64_t)s + DT]);
+        const int64_t count = sc.poff + sc.count - pc * kBatchAlign;  // the chunk's live particles
         const float* gu = a.g[3] + sc.goff;
         const float* gv = a.g[4] + sc.goff;
-        const float dt = bits_f(a.params[28 * (int64_t)s + DT]);  // read before the stores below
+
+        if (threadIdx.x == 0) {
+            box[0] = box[1] = 0x7fffffff;
+            box[2] = box[3] = -1;
+        }
+        __syncthreads();
+        int x0 = 0x7fffffff, y0 = 0x7fffffff, x1 = -1, y1 = -1;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
-            const int64_t q = pc * kBatchAlign + k * kBlock + threadIdx.x;
-            if (q - sc.poff < sc.count) g2p2_nodes(sc.nx, sc.ny, gu, gv, in[k]);
+            if (k * kBlock + threadIdx.x >= count) break;
+            const int cx = axis(px[k], sc.nx).cell, cy = axis(py[k], sc.ny).cell;
+            x0 = min(x0, cx);
+            y0 = min(y0, cy);
+            x1 = max(x1, cx);
+            y1 = max(y1, cy);
         }
+        x0 = wave_min(x0);
+        y0 = wave_min(y0);
+        x1 = wave_max(x1);
+        y1 = wave_max(y1);
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+            atomicMin(&box[0], x0);
+            atomicMin(&box[1], y0);
+            atomicMax(&box[2], x1);
+            atomicMax(&box[3], y1);
+        }
+        __syncthreads();
+        const int bx0 = box[0], by0 = box[1];
+        const int w = box[2] - bx0 + 3, h = box[3] - by0 + 3;
+        const bool tiled = box[2] >= 0 && w * h <= kTileG2;
+        if (tiled) {
+            // node i of the box: row floor((i + 1/2) / w) by a float reciprocal, exact for i < 2^12
+            const float inv_w = 1.0f / (float)w;
+            float vu[kLoads], vv[kLoads];
+#pragma unroll
+            for (int n = 0; n < kLoads; ++n) {
+                const int i = threadIdx.x + n * kBlock;
+                if (i >= w * h) break;
+                const int row = (int)(((float)i + 0.5f) * inv_w);
+                const int64_t node = (int64_t)(by0 + row) * sc.nx + bx0 + (i - row * w);
+                vu[n] = gu[node];
+                vv[n] = gv[node];
+            }
+#pragma unroll
+            for (int n = 0; n < kLoads; ++n) {
+                const int i = threadIdx.x + n * kBlock;
+                if (i >= w * h) break;
+                tile[0][i] = vu[n];
+                tile[1][i] = vv[n];
+            }
+        }
+        __syncthreads();
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
+            if (k * kBlock + threadIdx.x >= count) break;
             const int64_t q = pc * kBatchAlign + k * kBlock + threadIdx.x;
-            if (q - sc.poff < sc.count) g2p2_finish(a.p, q, sc.nx, sc.ny, dt, in[k]);
+            G2p2In in;
+            in.x = px[k];
+            in.y = py[k];
+            in.j = pj[k];
+            if (tiled) {
+                in.ax = axis(in.x, sc.nx);
+                in.ay = axis(in.y, sc.ny);
+                const int at = (in.ay.cell - by0) * w + (in.ax.cell - bx0);
+#pragma unroll
+                for (int n = 0; n < 9; ++n) {
+                    in.gu[n] = tile[0][at + (n / 3) * w + n % 3];
+                    in.gv[n] = tile[1][at + (n / 3) * w + n % 3];
+                }
+            } else {
+                g2p2_nodes(sc.nx, sc.ny, gu, gv, in);
+            }
+            g2p2_finish(a.p, q, sc.nx, sc.ny, dt, in);
         }
+        __syncthreads();  // the tile and box serve the next chunk
     }
 }
 
