@@ -544,16 +544,18 @@ __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a) {
     }
 }
 
-// The visits of clear, grid and decode. A block takes a window of kWin2 units (16 chunks, 4,096
-// nodes): wave 0 reads the window's flags with one load a lane and lists its units flagged
+// The visits of clear, grid and decode. A block takes a window of kW units (kW = 64: 16 chunks,
+// 4,096 nodes): wave 0 reads the window's flags with one load a lane and lists its units flagged
 // touched or live in LDS, and wave 1 reads the scenes of the window's chunks (when any of its
 // flags is set) into an LDS table. Then the block's waves split the listed units, a node a lane,
 // four units a wave in flight at once. A window with no flag costs one round trip, and no
 // chunk's scene chain (chunk -> scene -> record -> parameters) stands between the flags and the
-// sums' reads.
+// sums' reads. Grids of fewer than kSmallWindows such windows take windows of 16 units instead,
+// four times the blocks, so that a small scene's few windows are not each a long serial walk.
 constexpr int kWin2 = 64;
 constexpr int kWinChunks = kWin2 * kUnit2 / kBlock;
 constexpr int kWaves2 = kBlock / kWave;
+constexpr int kSmallWindows = 2048;
 static_assert(kWin2 == kWave, "a window's flags are one wave's lanes");
 
 struct WinScenes {
@@ -565,19 +567,28 @@ struct WinScenes {
 __device__ __forceinline__ int64_t units2(const Batch2Args& a) { return a.gchunks * (kBlock / kUnit2); }
 
 // Wave 0: the window's units flagged touched or live, as unit numbers within the window in order,
-// into list[0, *count).
-__device__ __forceinline__ void window_units(const Batch2Args& a, int64_t win, int lane, int* list, int* count) {
-    const int64_t u = win * kWin2 + lane;
-    const bool on = u < units2(a) && (a.touched[u] | a.live[u]) != 0;
+// into list[0, *count). Returns the lane's unit's touched flag.
+template <int kW>
+__device__ __forceinline__ uint8_t window_units(const Batch2Args& a, int64_t win, int lane, int* list, int* count) {
+    const int64_t u = win * kW + lane;
+    uint8_t t = 0, l = 0;
+    if (lane < kW && u < units2(a)) {
+        t = a.touched[u];
+        l = a.live[u];
+    }
+    const bool on = (t | l) != 0;
     const unsigned long long mask = __ballot(on);
     if (on) list[__popcll(mask & ((1ull << lane) - 1ull))] = lane;
     if (lane == 0) *count = __popcll(mask);
+    return t;
 }
 
 // Wave 1: the scenes of the window's chunks, when any of the window's flags is set.
+template <int kW>
 __device__ __forceinline__ void window_scenes(const Batch2Args& a, int64_t win, int lane, WinScenes& ws) {
-    const int64_t c = win * kWinChunks + lane;
-    const bool in = lane < kWinChunks && c < a.gchunks;
+    constexpr int kChunks = kW * kUnit2 / kBlock;
+    const int64_t c = win * kChunks + lane;
+    const bool in = lane < kChunks && c < a.gchunks;
     uint32_t f = 0;
     if (in) f = ((const uint32_t*)a.touched)[c] | ((const uint32_t*)a.live)[c];  // a chunk's 4 flags
     if (__ballot(f != 0) == 0ull || !in) return;
@@ -590,17 +601,18 @@ __device__ __forceinline__ void window_scenes(const Batch2Args& a, int64_t win, 
 }
 
 // Zeroes the sums of every unit flagged touched or live; the others hold zero sums already.
+template <int kW>
 __global__ __launch_bounds__(kBlock) void batch2_clear_kernel(Batch2Args a) {
     __shared__ int list[kWin2];
     __shared__ int count;
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const int64_t windows = (units2(a) + kWin2 - 1) / kWin2;
+    const int64_t windows = (units2(a) + kW - 1) / kW;
     for (int64_t win = blockIdx.x; win < windows; win += gridDim.x) {
-        if (wave == 0) window_units(a, win, lane, list, &count);
+        if (wave == 0) window_units<kW>(a, win, lane, list, &count);
         __syncthreads();
         const int n = count;
         for (int i = wave; i < n; i += kWaves2) {
-            const int64_t at = (win * kWin2 + list[i]) * kUnit2 + lane;
+            const int64_t at = (win * kW + list[i]) * kUnit2 + lane;
             a.q[0][at] = 0ull;
             a.q[1][at] = 0ull;
             a.q[2][at] = 0ull;
@@ -620,19 +632,29 @@ __global__ void batch2_flags_kernel(uint32_t* __restrict__ touched, uint32_t* __
 // kVelocity: the sums decoded into gm gmu gmv, then Flip.grid on them, over the units flagged
 // touched or live. A live unit P2G did not reach again holds zero sums, so it gets the zero
 // velocity the full pass would give it; a unit in neither set already holds zero sums and
-// velocities. Without kVelocity, the decoding alone: for passes that read the mass before the
-// grid update.
-template <bool kVelocity>
+// velocities. Each window's flags then move on (live := touched, touched := 0), by the block that
+// read them. Without kVelocity, the decoding alone, flags unchanged: for passes that read the mass
+// before the grid update.
+template <bool kVelocity, int kW>
 __global__ __launch_bounds__(kBlock) void batch2_grid_kernel(Batch2Args a) {
     __shared__ int list[kWin2];
     __shared__ int count;
     __shared__ WinScenes ws;
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const int64_t windows = (units2(a) + kWin2 - 1) / kWin2;
+    const int64_t windows = (units2(a) + kW - 1) / kW;
     for (int64_t win = blockIdx.x; win < windows; win += gridDim.x) {
-        if (wave == 0) window_units(a, win, lane, list, &count);
-        else if (wave == 1) window_scenes(a, win, lane, ws);
+        uint8_t touched = 0;
+        if (wave == 0) touched = window_units<kW>(a, win, lane, list, &count);
+        else if (wave == 1) window_scenes<kW>(a, win, lane, ws);
         __syncthreads();
+        if (kVelocity && wave == 0) {
+            // the flags move on once both waves have read them: live := touched, touched := 0
+            const int64_t u = win * kW + lane;
+            if (lane < kW && u < units2(a)) {
+                a.live[u] = touched;
+                a.touched[u] = 0;
+            }
+        }
         const int n = count;
         // wave w takes listed units w, w + 4, w + 8, w + 12, then the next 16: the sums of all four
         // read before the first store (a short last group reads its last unit again)
@@ -642,7 +664,7 @@ __global__ __launch_bounds__(kBlock) void batch2_grid_kernel(Batch2Args a) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const int i = min(i0 + k * kWaves2, n - 1);
-                at[k] = (win * kWin2 + list[i]) * kUnit2 + lane;
+                at[k] = (win * kW + list[i]) * kUnit2 + lane;
 #pragma unroll
                 for (int f = 0; f < 3; ++f) v[k][f] = a.q[f][at[k]];
             }
@@ -979,15 +1001,23 @@ unsigned grid_of(int64_t chunks, int blocks) {
     if (chunks <= 0) return 0;
     return (unsigned)(blocks > 0 && blocks < chunks ? blocks : chunks);
 }
-// Windows of kWin2 flags over the grid's units, which the clear, grid and decode blocks stride over.
-int64_t windows2(const Batch2Args& a) { return (a.gchunks * (kBlock / kUnit2) + kWin2 - 1) / kWin2; }
+// Windows of kW flags over the grid's units, which the clear, grid and decode blocks stride over:
+// 64 a window, or 16 for grids of fewer than kSmallWindows windows of 64.
+int window2(const Batch2Args& a) {
+    return (a.gchunks * (kBlock / kUnit2) + kWin2 - 1) / kWin2 < kSmallWindows ? 16 : kWin2;
+}
+int64_t windows2(const Batch2Args& a) {
+    const int w = window2(a);
+    return (a.gchunks * (kBlock / kUnit2) + w - 1) / w;
+}
 }  // namespace
 
 int batch2_clear(const Launch& l, const Batch2Args& a, int blocks) {
     const unsigned g = grid_of(windows2(a), blocks);
     if (!g) return 0;
     timed_begin(l, "batch2_clear");
-    hipLaunchKernelGGL(batch2_clear_kernel, dim3(g), dim3(kBlock), 0, l.stream, a);
+    if (window2(a) == 16) hipLaunchKernelGGL(batch2_clear_kernel<16>, dim3(g), dim3(kBlock), 0, l.stream, a);
+    else hipLaunchKernelGGL(batch2_clear_kernel<kWin2>, dim3(g), dim3(kBlock), 0, l.stream, a);
     timed_end(l, "batch2_clear");
     return check_launch("batch2_clear");
 }
@@ -1010,9 +1040,8 @@ int batch2_grid(const Launch& l, const Batch2Args& a, int blocks) {
     const unsigned g = grid_of(windows2(a), blocks);
     if (!g) return 0;
     timed_begin(l, "batch2_grid");
-    hipLaunchKernelGGL(batch2_grid_kernel<true>, dim3(g), dim3(kBlock), 0, l.stream, a);
-    hipLaunchKernelGGL(batch2_flags_kernel, dim3((unsigned)((a.gchunks + 255) / 256)), dim3(256), 0, l.stream,
-                       (uint32_t*)a.touched, (uint32_t*)a.live, a.gchunks);
+    if (window2(a) == 16) hipLaunchKernelGGL((batch2_grid_kernel<true, 16>), dim3(g), dim3(kBlock), 0, l.stream, a);
+    else hipLaunchKernelGGL((batch2_grid_kernel<true, kWin2>), dim3(g), dim3(kBlock), 0, l.stream, a);
     timed_end(l, "batch2_grid");
     return check_launch("batch2_grid");
 }
@@ -1021,7 +1050,8 @@ int batch2_decode(const Launch& l, const Batch2Args& a, int blocks) {
     const unsigned g = grid_of(windows2(a), blocks);
     if (!g) return 0;
     timed_begin(l, "batch2_decode");
-    hipLaunchKernelGGL(batch2_grid_kernel<false>, dim3(g), dim3(kBlock), 0, l.stream, a);
+    if (window2(a) == 16) hipLaunchKernelGGL((batch2_grid_kernel<false, 16>), dim3(g), dim3(kBlock), 0, l.stream, a);
+    else hipLaunchKernelGGL((batch2_grid_kernel<false, kWin2>), dim3(g), dim3(kBlock), 0, l.stream, a);
     timed_end(l, "batch2_decode");
     return check_launch("batch2_decode");
 }
