@@ -371,18 +371,21 @@ __device__ __forceinline__ unsigned long long unpair20(unsigned long long t) {
 }
 
 
-template <int kRun>
+// kOwn = 4: a block a chunk, a thread four consecutive particles. kOwn = 1: a block a quarter
+// chunk, a thread one particle, for scenes too small to fill the GPU with whole chunks.
+template <int kRun, int kOwn>
 __global__ __launch_bounds__(kBlock) void batch2_p2g_kernel(Batch2Args a) {
     __shared__ unsigned long long tile[3][kTile2];
     __shared__ int box[5];
     const float mscale = 0x1p40f, pscale = 0x1p32f;
-    constexpr int kOwn = kBatchAlign / kBlock;  // a thread's particles: 4, consecutive
-    static_assert(kOwn == 4 && kOwn % kRun == 0, "a thread's particles split into whole runs");
-    for (int64_t c = blockIdx.x; c < a.pchunks; c += gridDim.x) {
-        // the thread's particles, every field with one float4 load, all in flight at once and
-        // beside the scene's number (the chunk lies inside its scene's range, padded to whole
-        // chunks with massless slots, so these reads need nothing from the scene)
-        const int64_t first = c * kBatchAlign + (int64_t)threadIdx.x * kOwn;
+    constexpr int kParts = kBatchAlign / (kBlock * kOwn);  // a chunk's parts, a block each
+    static_assert(kBatchAlign % (kBlock * kOwn) == 0 && kOwn % kRun == 0, "a thread's particles split into whole runs");
+    for (int64_t bc = blockIdx.x; bc < a.pchunks * kParts; bc += gridDim.x) {
+        // the thread's particles, every field with one load, all in flight at once and beside the
+        // scene's number (the chunk lies inside its scene's range, padded to whole chunks with
+        // massless slots, so these reads need nothing from the scene)
+        const int64_t c = bc / kParts;
+        const int64_t first = c * kBatchAlign + (bc % kParts) * (kBlock * kOwn) + (int64_t)threadIdx.x * kOwn;
         float f[10][kOwn];
 #pragma unroll
         for (int k = 0; k < 10; ++k) load_run<kOwn>(a.p[k], first, f[k]);
@@ -556,6 +559,7 @@ constexpr int kWin2 = 64;
 constexpr int kWinChunks = kWin2 * kUnit2 / kBlock;
 constexpr int kWaves2 = kBlock / kWave;
 constexpr int kSmallWindows = 2048;
+constexpr int kSmallChunks = 256;  // particle chunks below which P2G takes a quarter chunk a block
 static_assert(kWin2 == kWave, "a window's flags are one wave's lanes");
 
 struct WinScenes {
@@ -1026,12 +1030,15 @@ int batch2_scatter(const Launch& l, const Batch2Args& a, int run, int blocks) {
     const unsigned g = grid_of(a.pchunks, blocks);
     if (!g) return 0;
     timed_begin(l, "batch2_p2g");
-    if (run == 4)
-        hipLaunchKernelGGL(batch2_p2g_kernel<4>, dim3(g), dim3(kBlock), 0, l.stream, a);
+    if (a.pchunks < kSmallChunks)
+        hipLaunchKernelGGL((batch2_p2g_kernel<1, 1>), dim3(grid_of(a.pchunks * 4, blocks)), dim3(kBlock), 0,
+                           l.stream, a);
+    else if (run == 4)
+        hipLaunchKernelGGL((batch2_p2g_kernel<4, 4>), dim3(g), dim3(kBlock), 0, l.stream, a);
     else if (run == 2)
-        hipLaunchKernelGGL(batch2_p2g_kernel<2>, dim3(g), dim3(kBlock), 0, l.stream, a);
+        hipLaunchKernelGGL((batch2_p2g_kernel<2, 4>), dim3(g), dim3(kBlock), 0, l.stream, a);
     else
-        hipLaunchKernelGGL(batch2_p2g_kernel<1>, dim3(g), dim3(kBlock), 0, l.stream, a);
+        hipLaunchKernelGGL((batch2_p2g_kernel<1, 4>), dim3(g), dim3(kBlock), 0, l.stream, a);
     timed_end(l, "batch2_p2g");
     return check_launch("batch2_p2g");
 }
