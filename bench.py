@@ -234,8 +234,19 @@ def bench_surrogate(iters, n=1024):
     hll.step(64)
     state = hll.state32()
     out = model.forward(*state)
+    model.advance(*state, out=out)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     reps = 20
+    # an application as the fallback rule runs it: the network, the device check, and the eight
+    # HLL steps queued behind it, gated on its flag (each exits at once when the check passes)
+    fallbacks_before = model.fallbacks()
+    ev[0].record()
+    for _ in range(reps):
+        model.advance(*state, out=out)
+    ev[1].record()
+    torch.cuda.synchronize()
+    app_ms = ev[0].elapsed_time(ev[1]) / reps
+    timed_fallbacks = model.fallbacks() - fallbacks_before
     ev[0].record()
     for _ in range(reps):
         model.forward(*state, out=out)
@@ -276,12 +287,16 @@ def bench_surrogate(iters, n=1024):
                         f"dam-and-bumps trajectories (seeds 0-1999), judged on seeds 10000-10199 vs FP64 HLL",
             "train_iters": iters, "train_seconds": round(train_s, 1), "final_loss": losses[-1],
             "quality": quality,
-            "surrogate_ms_per_application": round(fwd_ms, 4),
-            "surrogate_cell_updates_per_s": round(cells / (fwd_ms * 1e-3), 1),
+            "surrogate_ms_per_application": round(app_ms, 4),
+            "surrogate_application": "network + device check + the 8 gated fallback HLL launches; "
+                                     f"{timed_fallbacks} of the {reps} timed applications fell back",
+            "surrogate_forward_ms": round(fwd_ms, 4),
+            "surrogate_cell_updates_per_s": round(cells / (app_ms * 1e-3), 1),
             "hll_fp32_ms_per_8_steps": round(hll32_ms, 4), "hll_fp32_rows": hll_rows,
             "hll_fp32_cell_updates_per_s": round(cells / (hll32_ms * 1e-3), 1),
             "hll_fp64_ms_per_8_steps": round(hll64_ms, 4),
-            "surrogate_speed_vs_hll_fp32": round(hll32_ms / fwd_ms, 4),
+            "surrogate_speed_vs_hll_fp32": round(hll32_ms / app_ms, 4),
+            "surrogate_forward_speed_vs_hll_fp32": round(hll32_ms / fwd_ms, 4),
             "roofline": {"kernel": "conv3x3_rows 64->64", "bound": "mfma",
                          "achieved": round(conv_flop / (conv_ms * 1e-3) / 1e12, 1), "peak": 2500.0,
                          "unit": "TFLOP/s", "frac": round(conv_flop / (conv_ms * 1e-3) / 1e12 / 2500.0, 4),
