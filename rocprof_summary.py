@@ -60,6 +60,21 @@ def short_name(kernel: str) -> str:
     return head[-40:]
 
 
+def _labels(kernels) -> dict:
+    """A label for each demangled kernel name: its short name, with its template arguments
+    added where two instantiations share one."""
+    by_short = collections.defaultdict(list)
+    for k in kernels:
+        by_short[short_name(k)].append(k)
+    labels = {}
+    for short, names in by_short.items():
+        for k in names:
+            head = k.replace("(anonymous namespace)", "").split("(")[0]
+            args = head[head.index("<"):] if len(names) > 1 and "<" in head else ""
+            labels[k] = short + args.replace(" ", "")
+    return labels
+
+
 def _counters(directory):
     per = collections.defaultdict(list)
     with open(os.path.join(directory, "run_counter_collection.csv")) as f:
@@ -93,12 +108,13 @@ def traffic(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, copy_bytes, out,
     f, nf = _counters(fetch_dir)
     w, _ = _counters(write_dir)
     kernels = {}
+    labels = _labels(f)
     for k in f:
         if k not in w or "rocclr" in k:
             continue
         fb = f[k] * 1024.0 * fetch_scale
         wb = w[k] * 1024.0 * write_scale
-        kernels[short_name(k)] = {"kernel": k, "dispatches": nf[k], "fetch_kib_raw": round(f[k], 1),
+        kernels[labels[k]] = {"kernel": k, "dispatches": nf[k], "fetch_kib_raw": round(f[k], 1),
                                   "write_kib_raw": round(w[k], 1), "fetch_bytes": round(fb),
                                   "write_bytes": round(wb), "hbm_bytes_per_launch": round(fb + wb)}
     doc = {"command": command, "copy_bytes_each_way": copy_bytes, "fetch_scale": round(fetch_scale, 4),
@@ -119,10 +135,11 @@ def counts(directory, out, command, trace_dir=None):
             for r in csv.DictReader(f):
                 durations[r["Name"]] = float(r["AverageNs"]) * 1e-9
     kernels = {}
+    labels = _labels({k for k, _ in per})
     for (k, counter), values in sorted(per.items()):
         if "rocclr" in k:
             continue
-        entry = kernels.setdefault(short_name(k), {"kernel": k, "dispatches": len(values)})
+        entry = kernels.setdefault(labels[k], {"kernel": k, "dispatches": len(values)})
         mean = sum(values) / len(values)
         entry[counter] = round(mean, 1)
         if k in durations:
