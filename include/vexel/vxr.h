@@ -202,7 +202,8 @@ int vxr_stream(vxr_sim* sim, void** stream);
  *                arithmetic order (the CPU restatement's bits); 2 = the LDS tile with the sums
  *                contracted axis by axis in fused multiply-adds (default, within a few ulps)
  *   "run"        MPM: consecutive particles a thread takes in the segmented passes: 1, 2 or 4
- *                (default 4)
+ *                (default 4); for the 2D fixed-point scatter, the particles folded while they
+ *                share a cell in blocks whose amounts reach 2^40 (other blocks add each amount)
  *   "p2g_ints"   MPM3 with p2g 1: 1 = each wave's LDS node tile sums in 32-bit integers, in
  *                power-of-two units the wave chooses from bounds on its particles' amounts (mass
  *                and momentum apart), each run of particles rounded once to the nearest unit and
@@ -214,7 +215,9 @@ int vxr_stream(vxr_sim* sim, void** stream);
  *                place of its correctly rounded divisions, and the hardware square root: fewer
  *                instructions, within a few ulps a step of the reference's bits (default 0: the
  *                reference's bits)
- *   "blocks"     batch passes: blocks per launch, each striding over chunks (0 = one per chunk)
+ *   "blocks"     2D fixed-point and batch passes: blocks per launch, each striding over its
+ *                items (particle chunks, or windows of node flags for clear and grid; 0 = one
+ *                block per item)
  *   "graph"      1 = vxr_advance replays captured hipGraphs (default 0) */
 int vxr_set_option(vxr_sim* sim, const char* key, int64_t value);
 int vxr_get_option(vxr_sim* sim, const char* key, int64_t* value);
