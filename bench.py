@@ -77,6 +77,9 @@ def kernel_table(sim, names, units, bytes_override=None):
     return out
 
 
+P2G_KERNEL = {0: "p2g3_atomic", 1: "p2g3_seg", 2: "p2g3_fixed", 3: "p2g3_layer"}
+
+
 def bench_mpm3(steps, warmup, p2g, g2p, run, tension=False, lod_level=None, p2g_ints=1):
     """The 3D substep on the two-cubes scene; with ``tension`` Flip3Step's tension step (six
     blurs, the stress and the grid with the capillary force), sigma = 0.01 B 8, the scale
@@ -101,7 +104,10 @@ def bench_mpm3(steps, warmup, p2g, g2p, run, tension=False, lod_level=None, p2g_
         sim.sim.run_pass("lod", 60)
     sim.advance(warmup, sort_every=SORT_EVERY)
     sim.finish()
-    sim.sim.timing(True)
+    # HIP events between kernels keep the next launch from queueing behind the running one, so the
+    # timed region carries an event pair around the dominant kernel (P2G) alone
+    dominant = P2G_KERNEL[p2g]
+    sim.sim.timing_only(dominant)
     t0 = time.perf_counter()
     sim.advance(steps, sort_every=SORT_EVERY)
     sim.finish()
@@ -111,7 +117,14 @@ def bench_mpm3(steps, warmup, p2g, g2p, run, tension=False, lod_level=None, p2g_
              "keys3", "radix_hist", "radix_scan", "radix_scatter", "layers3", "permute", "tension3_blur",
              "tension3_stress", "grid3_tension"]
     units = {"particle": count, "node": 256 ** 3, "element": count}
+    live = kernel_table(sim.sim, [dominant], units)
+    # every other kernel's figures from two more sort periods with all of them timed
+    sim.sim.timing(True)
+    sim.advance(2 * SORT_EVERY, sort_every=SORT_EVERY)
+    sim.finish()
     table = kernel_table(sim.sim, names, units)
+    for k, v in live.items():
+        table[k] = dict(v, timed_region=True)
     sim.sim.timing(False)
     mass = float(np.sum(sim.grid()[0], dtype=np.float64))
     if lod_level is not None:
@@ -571,6 +584,9 @@ def main():
                    "p2g": ["atomic", "sorted-segmented", "fixed-point", "layers"][args.p2g], "g2p": ["gather", "lds-tile", "lds-tile-contracted"][args.g2p],
                    "run": args.run, "p2g_tile": ["float read-add-write", "int32 LDS atomics"][args.p2g_ints]},
         "roofline": roofline, "kernels": table, "grid_mass_drift": drift,
+        "kernel_timing": "roofline.kernel by HIP events around each of its launches in the timed region, no events "
+                         "around the other kernels; the other kernels over 20 further substeps with every launch "
+                         "timed (events between kernels stall the launch queue: about 7% of a substep)",
     }
     if not args.no_atomic:
         print(f"[bench] 3D MPM with the direct atomic P2G: same scene, {args.atomic_steps} steps", file=sys.stderr,

@@ -232,6 +232,10 @@ int vxr_autotune(vxr_sim* sim, int reps);
  * totals; vxr_kernel_time synchronizes and returns the summed milliseconds and launch
  * count of every launch of the named kernel since then. */
 int vxr_timing(vxr_sim* sim, int enable);
+/* Timing of the named kernel alone (an event pair around each of its launches and none around the
+ * others, whose launches then follow each other with no event between them); clears the totals.
+ * vxr_timing(sim, 0) turns it off. */
+int vxr_timing_only(vxr_sim* sim, const char* kernel);
 int vxr_kernel_time(vxr_sim* sim, const char* kernel, double* total_ms, int64_t* launches);
 
 #ifdef __cplusplus

@@ -538,6 +538,29 @@ def test_config_two_cubes_full_size(gpu):
     sim.close()
 
 
+def test_timing_one_kernel(gpu):
+    """vxr_timing_only: event pairs around the named kernel's launches alone; vxr_timing times
+    every kernel again."""
+    x, y, z = scenes.jittered_block3((2, 2, 2), (6, 8, 6), 8, scenes.JavaRandomFloats(3))
+    count = len(x)
+    sim = particles.ParticleSimulation3(16, 16, 16, count)
+    zero = np.zeros(count, np.float32)
+    sim.load((x, y, z), (zero, zero, zero), np.full(count, 0.125, np.float32),
+             particles.params3(1e-3, 0, -100, 0, 500, 1))
+    sim.sim.timing_only("p2g3_seg")
+    sim.advance(6, sort_every=5)
+    ms, n = sim.sim.kernel_time("p2g3_seg")
+    assert n == 6 and ms > 0
+    assert sim.sim.kernel_time("grid3") == (0.0, 0)
+    sim.sim.timing(True)
+    sim.advance(2, sort_every=5)
+    assert sim.sim.kernel_time("p2g3_seg")[1] == 2 and sim.sim.kernel_time("grid3")[1] == 2
+    sim.sim.timing(False)
+    with pytest.raises(_native.InvalidArgument):
+        sim.sim.timing_only("")
+    sim.close()
+
+
 def test_runner_api_3d(gpu):
     """ParticleSimulation3's surface: load, advance with a sort cadence, read-backs, active count."""
     x, y, z = scenes.jittered_block3((2, 2, 2), (6, 8, 6), 8, scenes.JavaRandomFloats(3))

@@ -64,6 +64,7 @@ int bits_for(int64_t values) {  // bits to hold 0 .. values-1
 namespace vxr {
 
 struct Timer {
+    std::string only;  // times this kernel alone when set
     std::vector<hipEvent_t> pool;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::map<std::string, std::pair<double, int64_t>> totals;
@@ -99,8 +100,9 @@ struct Timer {
     }
 };
 
-void timed_begin(const Launch& l, const char*) {
+void timed_begin(const Launch& l, const char* kernel) {
     if (!l.timer) return;
+    if (!l.timer->only.empty() && l.timer->only != kernel) return;
     l.timer->open = l.timer->take();
     if (l.timer->open) hipEventRecord(l.timer->open, l.stream);
 }
@@ -1439,6 +1441,15 @@ int vxr_timing(vxr_sim* s, int enable) {
     HIP_TRY(hipStreamSynchronize(s->stream));
     delete s->timer;
     s->timer = enable ? new vxr::Timer : nullptr;
+    return VXR_OK;
+}
+
+int vxr_timing_only(vxr_sim* s, const char* kernel) {
+    if (!s || !kernel || !kernel[0]) return fail(VXR_EINVAL, "null argument");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    delete s->timer;
+    s->timer = new vxr::Timer;
+    s->timer->only = kernel;
     return VXR_OK;
 }
 

@@ -69,6 +69,7 @@ def lib() -> ctypes.CDLL:
             "vxr_set_option": (ctypes.c_int, [vp, cs, i64]),
             "vxr_get_option": (ctypes.c_int, [vp, cs, ctypes.POINTER(i64)]),
             "vxr_timing": (ctypes.c_int, [vp, ctypes.c_int]),
+            "vxr_timing_only": (ctypes.c_int, [vp, ctypes.c_char_p]),
             "vxr_kernel_time": (ctypes.c_int, [vp, cs, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]),
             "vxr_create_batch": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                                 ctypes.POINTER(i64), ctypes.c_uint32, ctypes.POINTER(vp)]),
@@ -254,6 +255,10 @@ class Sim:
 
     def timing(self, enable: bool) -> None:
         check(lib().vxr_timing(self.h, 1 if enable else 0))
+
+    def timing_only(self, kernel: str) -> None:
+        """Times the named kernel's launches alone (no events around the others)."""
+        check(lib().vxr_timing_only(self.h, kernel.encode()))
 
     def kernel_time(self, kernel: str):
         ms = ctypes.c_double()
