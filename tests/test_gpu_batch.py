@@ -118,6 +118,37 @@ def test_fixed_p2g_wide_amounts(gpu, case):
     dev.close()
 
 
+@pytest.mark.parametrize("heavy", [False, True], ids=["pairs", "wide"])
+def test_fixed_p2g_whole_chunks(gpu, heavy):
+    """A scene of 270,112 particles (264 chunks of 1,024), so that P2G takes whole chunks a block
+    (four particles a thread) rather than the quarter chunks of small scenes: with mass 0.25 every
+    block adds split pairs, with rho0 = 1000 and mass 250 every block keeps the double-precision
+    fold. One pass and 10 steps with a sort, bit-exact with the oracle."""
+    n = 512
+    sc = _scene(n, (184, 367), 270112, 21, speed=10.0)
+    assert len(sc["x"]) == 270112
+    h = 1.0 / (n - 1)
+    params = _params(n)
+    if heavy:
+        sc["m"] = np.full(len(sc["x"]), 250.0, np.float32)
+        params = oracle.mpm2_params(1e-4 * 127.0 / (n - 1), 0, -9.8 / h, 10.0 / h ** 2, 1000)
+    print(f"[batch] fixed p2g whole chunks, heavy={heavy}", flush=True)
+    dev = _single(n, sc, params)
+    ref = oracle.Mpm2(n, n, sc, params, fixed=True)
+    for p in ("clear", "scatter", "grid"):
+        dev.run_pass(p)
+        getattr(ref, p)()
+    for g in ("gm", "gmu", "gmv", "gu", "gv"):
+        np.testing.assert_array_equal(dev.read(g).view(np.int32), ref.g[g].view(np.int32), err_msg=g)
+    dev.run_pass("advect")
+    ref.advect()
+    dev.advance(10, 10, 1)
+    ref.steps = 1
+    ref.advance(10, 10)
+    _assert_fields(dev.read, ref.p, f"heavy={heavy}: 10 steps")
+    dev.close()
+
+
 def _batch_case():
     # grids of several sizes, counts off the 1024 padding, one scene of a few particles
     spec = [(64, (20, 30), 2400, 11), (16, (4, 6), 5, 12), (128, (40, 70), 11000, 13), (40, (12, 20), 960, 14),
